@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""FMM tick throughput on one MI355X (particles per second per tick).
+
+Workload (default ``plummer1m``): N = 1,048,576 Plummer sphere (a = 1,
+truncated at 20a, isotropic, equal masses 1/N, float32 SoA), theta = 2.0,
+the reference's threaded-build traversal.  One step = one tick
+(ps_prepare_particles + ps_calc_forces, particles device-resident) followed
+by the caller's frame step on the device (id-gather of velocities,
+semi-implicit Euler dt = 1e-3), so every tick re-sorts and rebuilds.
+
+    python bench.py [--steps K] [--warmup W] [--workload plummer1m|twoplummer16m|plummer4m]
+
+Prints one JSON line.  `value` = N * K / (wall time of K steps), bracketed
+by device synchronisation.  `roofline` is the dominant kernel's algorithmic
+work per launch over its launch time, both from HIP events recorded on the
+engine's stream inside the timed steps.  `cpu_baseline` times the
+reference's own PS_MULTITHREADING CPU path (oracle/_ref, shipped-style
+flags) on this host's cores on a bounded sample of the same workload.
+`fp64_*` report the force error against an FP64 direct sum on 4096 seeded
+targets matched through id[] (tests/accuracy.c:505-531), for the GPU tick
+and for the reference on the same state.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FMM tick particles/sec at N=1M,16M @ fixed RMS force err vs FP64 direct"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOP_P2P = 20    # per particle pair (GPU Gems 3 ch. 31 convention)
+FLOP_M2L = 41    # per M2L pair, op count of the reference's M2L
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from polesitter_amd import workloads as W
+    if name == "plummer1m":
+        n = 1 << 20
+        x, y, z, m = W.plummer(n, seed=1)
+        v = [np.zeros(n, np.float32) for _ in range(3)]
+        desc = "Plummer a=1 r<=20a, N=1048576, m=1/N, v0=0"
+    elif name == "plummer4m":
+        n = 1 << 22
+        x, y, z, m = W.plummer(n, seed=1)
+        v = [np.zeros(n, np.float32) for _ in range(3)]
+        desc = "Plummer a=1 r<=20a, N=4194304, m=1/N, v0=0"
+    elif name == "twoplummer16m":
+        n = 1 << 24
+        x, y, z, m, vx, vy, vz = W.two_plummers(n, seed=1)
+        v = [vx, vy, vz]
+        desc = "two Plummers a=1 at (+-5,0,0), v=(-+0.3,0,0), N=16777216, m=1/N"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return n, x, y, z, m, v, desc
+
+
+def fp64_error(n, pos_sorted_forces, ids, ref_fn, seed=4096):
+    """RMS / max relative error on 4096 seeded original indices."""
+    from polesitter_amd import workloads as W
+    tg = W.sample_targets(n, 4096, seed=seed)
+    inv = np.empty(n, np.int64)
+    inv[ids] = np.arange(n)
+    pos = inv[tg]
+    ref = ref_fn(pos)
+    f = pos_sorted_forces[pos].astype(np.float64)
+    d = f - ref
+    e2 = (d * d).sum(1)
+    r2 = (ref * ref).sum(1)
+    ok = r2 > 1e-24
+    return float(np.sqrt(e2.sum() / r2.sum())), float(np.sqrt(e2[ok] / r2[ok]).max())
+
+
+def cpu_baseline(n, x, y, z, m, theta, ticks, gpu_ref_fn):
+    """The reference CPU path (oracle/_ref) on this host, bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    thr = max(1, min(16, cores or 1))
+    if O.ref_available("shipped_mt"):
+        build, kind = "shipped_mt", "reference"
+        t0 = time.time()
+        secs = O.ref_time_ticks(x, y, z, m, theta=theta, thrd_cnt=thr, n_ticks=ticks + 1,
+                                build=build)
+        wall = time.time() - t0
+        tick_s = float(np.median(secs[1:]))
+        # the reference's error on the same initial state
+        r = O.ref_tick(x, y, z, m, theta=theta, thrd_cnt=thr, build=build, dump=False)
+        f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
+        err = fp64_error(n, f, r["id"], lambda pos: gpu_ref_fn(r, pos))
+        sample = (f"{ticks} ticks (+1 warm-up) of the full workload, median; "
+                  f"src/polesitter.h built -O3 -ffast-math -march=x86-64-v3 -DPS_MULTITHREADING, "
+                  f"thrd_cnt={thr}; {wall:.1f} s of CPU time")
+        return dict(value=n / tick_s, unit="particles/s", cores=thr, kind=kind, sample=sample,
+                    tick_s=tick_s, fp64_rms_err=err[0], fp64_max_err=err[1])
+    # fallback: the single-threaded C restatement
+    t0 = time.time()
+    O.tick(x, y, z, m, theta=theta)
+    tick_s = time.time() - t0
+    return dict(value=n / tick_s, unit="particles/s", cores=1, kind="port",
+                sample="1 tick of the full workload, oracle/ps_oracle.c, 1 thread",
+                tick_s=tick_s)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plummer1m")
+    ap.add_argument("--theta", type=float, default=2.0)
+    ap.add_argument("--kahan", action="store_true")
+    ap.add_argument("--cpu-ticks", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {a.gpus} requested)", file=sys.stderr)
+        return 2
+
+    from polesitter_amd import api
+
+    n, x, y, z, m, v, desc = workload(a.workload)
+    log(f"bench: {api.device_name()} workload {a.workload}: {desc}")
+    L = api.hip_lib()
+    s = api.Solver(n, theta=a.theta, kahan=a.kahan, stats=True)
+    s.load(x, y, z, m)
+    vel = [api.DeviceArray.from_numpy(vv) for vv in v]
+    tmp = api.DeviceArray(3 * n, np.float32)
+    dt = 1e-3
+
+    def step():
+        s.reset_ids()
+        s.tick()
+        api.check("gather", L.ps_hip_step_gather(s.ctx.engine, s.id.ptr, vel[0].ptr, vel[1].ptr,
+                                                 vel[2].ptr, tmp.ptr, n))
+        api.check("euler", L.ps_hip_step_euler(s.ctx.engine, s.x.ptr, s.y.ptr, s.z.ptr,
+                                               vel[0].ptr, vel[1].ptr, vel[2].ptr, s.fx.ptr,
+                                               s.fy.ptr, s.fz.ptr, s.m.ptr, dt, n))
+        return s.stats()
+
+    # error of the first tick (initial state), untimed
+    err = None
+    if not a.no_error:
+        s.reset_ids()
+        s.tick()
+        r = s.result()
+        f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
+        err = fp64_error(n, f, r["id"], s.direct_f64)
+        log(f"bench: GPU FP64 error on 4096 targets rms={err[0]:.3e} max={err[1]:.3e}")
+        s.load(x, y, z, m)
+
+    for _ in range(a.warmup):
+        step()
+    api.check("sync", L.ps_hip_synchronize())
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        stats.append(step())
+    api.check("sync", L.ps_hip_synchronize())
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / a.steps
+    value = n * a.steps / wall
+
+    mean = lambda k: float(np.mean([st[k] for st in stats]))
+    kern = {
+        "trav_count": dict(ms=mean("ms_trav_count"), flop=FLOP_M2L * mean("m2l_pairs")),
+        "p2p": dict(ms=mean("ms_p2p"), flop=FLOP_P2P * mean("p2p_pairs")),
+    }
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    ach = kern[dom]["flop"] / (kern[dom]["ms"] * 1e-3) / 1e12
+    roof = dict(kernel=dom, bound="mfma", achieved=round(ach, 4), peak=FP32_PEAK_TFLOPS,
+                unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 6), traffic=None,
+                algorithmic=("41 FLOP per M2L pair" if dom == "trav_count"
+                             else "20 FLOP per particle pair"),
+                ms_per_tick=round(kern[dom]["ms"], 4),
+                note="FP32 VALU work (no MFMA); peak is the published FP32 figure, the "
+                     "non-packed VALU ceiling is half of it")
+
+    line = dict(metric=METRIC, value=round(value, 1), unit="particles/s", n_gpus=1,
+                steps=a.steps, warmup=a.warmup, ms_per_step=round(ms_step, 4),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+                data="synthetic",
+                config=dict(workload=a.workload, description=desc, n_particles=n,
+                            theta=a.theta, traversal="MT depth-2 jobs (PS_MULTITHREADING)",
+                            step="tick + id-gather + semi-implicit Euler dt=1e-3",
+                            kahan=bool(a.kahan), pointers="device-resident"),
+                roofline=roof,
+                stages_ms={k: round(mean("ms_" + k), 4) for k in
+                           ("sort", "build", "upward", "traverse", "l2l", "p2p", "total",
+                            "trav_count", "trav_write")},
+                counts=dict(nodes=int(stats[-1]["n_nodes"]), m2l_pairs=int(mean("m2l_pairs")),
+                            p2p_pairs=int(mean("p2p_pairs")),
+                            iterations=int(stats[-1]["iterations"])))
+    if err:
+        line["fp64_rms_err"], line["fp64_max_err"] = err
+    if not a.no_cpu_baseline:
+        def ref_fn(r, pos):
+            # FP64 direct sum on the device over the reference's sorted state
+            with api.Solver(n) as t:
+                t.load(r["x"], r["y"], r["z"], r["m"])
+                return t.direct_f64(pos)
+        try:
+            line["cpu_baseline"] = cpu_baseline(n, x, y, z, m, a.theta, a.cpu_ticks, ref_fn)
+            cb = line["cpu_baseline"]
+            line["gpu_over_cpu"] = round(value / cb["value"], 2)
+        except Exception as exc:  # report, never hide, a failed baseline
+            line["cpu_baseline"] = dict(value=None, error=str(exc))
+    s.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,160 @@
+/*
+ * polesitter_hip.h -- the C-ABI of libpolesitter_hip.so, the gfx950
+ * (MI355X) implementation of polesitter's FMM tick.
+ *
+ * Plain pointers and sizes only.  Particle arrays may be device pointers
+ * (used in place, zero copy) or host pointers (staged through device memory
+ * and copied back); each array is classified on its own.  Every call is
+ * synchronous on return, like the reference's API.
+ *
+ * The public C99 API (include/polesitter.h) is thin host code over these
+ * entry points.  Each declaration names the reference interface it replaces
+ * (src/polesitter.h = H).  Entry points marked "no reference counterpart" are
+ * extensions: device memory helpers for callers without a HIP runtime of
+ * their own, the on-device FP64 error oracle, statistics, white-box stages
+ * for tests, and the caller's frame step on the device.
+ *
+ * Return values equal ps_result_t: 0 OK, -1 out of memory, -2 invalid
+ * argument, -5 device error.
+ */
+#ifndef POLESITTER_HIP_H
+#define POLESITTER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PS_HIP_OK      0
+#define PS_HIP_EOOM    (-1)
+#define PS_HIP_EINVAL  (-2)
+#define PS_HIP_EDEVICE (-5)
+
+/* traversal roots: PS_MULTITHREADING's depth-2 jobs (H:1772-1801, H:1883)
+ * or the single (root, root) pair (H:1888,1905) */
+#define PS_HIP_TRAV_MT 0
+#define PS_HIP_TRAV_ST 1
+
+/* Morton quantisation: scalar truncation (H:2132-2157), or the AVX2 path's
+ * round-to-nearest-even for the first 8*floor(n/8) particles (H:2040-2072) */
+#define PS_HIP_MORTON_TRUNC 0
+#define PS_HIP_MORTON_AVX2  1
+
+/* engine options (ps_hip_engine_set_option) */
+#define PS_HIP_OPT_TRAVERSAL 1 /* PS_HIP_TRAV_MT / PS_HIP_TRAV_ST */
+#define PS_HIP_OPT_KAHAN     2 /* 1: compensated far-field and P2P sums */
+#define PS_HIP_OPT_THETA     3 /* MAC theta, like ps_config_t.theta */
+#define PS_HIP_OPT_STATS     4 /* 1: count P2P pairs and time each stage */
+
+typedef struct ps_hip_engine ps_hip_engine_t;
+
+typedef struct {
+    uint64_t n_particles;
+    uint64_t n_nodes;
+    uint64_t n_leaves;
+    uint32_t max_depth;
+    uint32_t iterations;      /* traversal iterations */
+    uint32_t level_start[12]; /* depth d nodes: [level_start[d], level_start[d+1]) */
+    uint64_t m2l_pairs;
+    uint64_t p2p_leaf_pairs;
+    uint64_t p2p_pairs;       /* sum over leaf pairs of n_t*n_s (stats on) */
+    uint64_t list_entries;    /* traversal list entries written */
+    /* milliseconds per stage of the last tick, HIP events on the engine's
+     * stream (stats on); ms_p2p brackets the single P2P kernel launch,
+     * ms_trav_count / ms_trav_write sum the traversal kernels' launches */
+    float ms_upload, ms_sort, ms_build, ms_upward, ms_traverse, ms_l2l, ms_p2p,
+        ms_download, ms_total;
+    float ms_trav_count, ms_trav_write;
+} ps_hip_stats_t;
+
+/* ps_init, H:1699-1767: device state for up to max_particles particles and
+ * max_nodes octree nodes on the current device. */
+int ps_hip_engine_create(ps_hip_engine_t** out, size_t max_particles,
+                         size_t max_nodes, float theta, int traversal);
+
+/* ps_destroy, H:2172-2192 */
+int ps_hip_engine_destroy(ps_hip_engine_t* e);
+
+/* ps_config_t.theta (H:304-310) and compile-time PS_MULTITHREADING (H:207)
+ * made run-time; PS_HIP_OPT_KAHAN / _STATS have no reference counterpart */
+int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value);
+
+/* no reference counterpart: counters and stage times of the last tick */
+int ps_hip_engine_get_stats(const ps_hip_engine_t* e, ps_hip_stats_t* out);
+
+/* ps_prepare_particles, H:1911-2170 (stateless, like the reference) */
+int ps_hip_prepare(const float* x, const float* y, const float* z, size_t n,
+                   uint32_t* codes, float* out_min, float* out_max,
+                   float* out_range, int morton_mode);
+
+/* ps_calc_forces, H:1840-1909: sorts x, y, z, mass, id and codes in place
+ * by code and writes fx, fy, fz in the sorted order. */
+int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z,
+                       float* mass, float* fx, float* fy, float* fz,
+                       uint32_t* id, size_t n, uint32_t* codes, float cx,
+                       float cy, float cz, float hw);
+
+/* no reference counterpart: copies the last tick's octree (canonical level
+ * order) to host arrays of capacity cap: geom4 = (x,y,z,hw), mp4, loc4 (total
+ * field after L2L), link2 = (first child | first particle, mask | leaf flag
+ * 0x80000000), nend, parent. */
+int ps_hip_engine_export_tree(ps_hip_engine_t* e, size_t cap, float* geom4,
+                              float* mp4, float* loc4, uint32_t* link2,
+                              uint32_t* nend, uint32_t* parent,
+                              uint32_t* n_nodes);
+
+/* White-box stages over device arrays (no reference counterpart; the
+ * reference's own tests call its static passes the same way,
+ * tests/main.c:125-476).  bind, then any prefix of:
+ * 1 sort, 2 build, 3 upward, 4 interaction (M2L), 5 downward (L2L + L2P,
+ * f = m*field), 6 p2p (f += near field). */
+int ps_hip_debug_bind(ps_hip_engine_t* e, float* x, float* y, float* z,
+                      float* mass, float* fx, float* fy, float* fz,
+                      uint32_t* id, size_t n, uint32_t* codes, float cx,
+                      float cy, float cz, float hw);
+int ps_hip_debug_stage(ps_hip_engine_t* e, int stage);
+int ps_hip_debug_set_local(ps_hip_engine_t* e, uint32_t node,
+                           const float* field3);
+
+/* no reference counterpart: MAC + M2L of one cell pair (H:1056-1095) on the
+ * device; out4 = (far, Ex, Ey, Ez) */
+int ps_hip_op_m2l(const float* tgeom4, const float* sgeom4, const float* smp4,
+                  float theta, float* out4);
+
+/* no reference counterpart: the Morton interleave of all 2^30 (x,y,z)
+ * against a bit loop; out2 = (mismatches, sum of codes) */
+int ps_hip_op_morton_exhaustive(uint64_t* out2);
+
+/* no reference counterpart: FP64 direct sum (tests/accuracy.c:42-72) on the
+ * device for nt target positions (host array) against all n sources (device
+ * arrays); out3 (host) = 3 doubles per target. */
+int ps_hip_direct_f64(const float* x, const float* y, const float* z,
+                      const float* mass, size_t n, const uint32_t* targets,
+                      size_t nt, double* out3);
+
+/* no reference counterpart: the caller's frame step of README.md:94-161 on
+ * device arrays -- reset ids, gather velocities by id, semi-implicit Euler.
+ * Runs on the engine's stream and synchronises. */
+int ps_hip_step_reset_ids(ps_hip_engine_t* e, uint32_t* id, size_t n);
+int ps_hip_step_gather(ps_hip_engine_t* e, const uint32_t* id, float* vx,
+                       float* vy, float* vz, float* tmp3, size_t n);
+int ps_hip_step_euler(ps_hip_engine_t* e, float* x, float* y, float* z,
+                      float* vx, float* vy, float* vz, const float* fx,
+                      const float* fy, const float* fz, const float* mass,
+                      float dt, size_t n);
+
+/* no reference counterpart: device memory for callers without a HIP runtime
+ * of their own (kind: 1 host->device, 2 device->host, 3 device->device) */
+void* ps_hip_malloc(size_t bytes);
+int   ps_hip_free(void* p);
+int   ps_hip_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int   ps_hip_synchronize(void);
+int   ps_hip_device_name(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POLESITTER_HIP_H */

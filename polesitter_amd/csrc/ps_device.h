@@ -1,0 +1,119 @@
+// ps_device.h -- device-side definitions shared by the gfx950 kernels.
+//
+// Node layout in HBM (structure of arrays, canonical level order: depth-major,
+// ascending Morton prefix inside a depth, so the children of one node are
+// contiguous and in octant order):
+//   geom[n]   float4  (x, y, z, half_width)          ps_node_t x..half_width, H:364-367
+//   mp[n]     float4  (m0, px, py, pz)               ps_node_t multipole,     H:370
+//   loc[n]    float4  (unused, Ex, Ey, Ez)           ctx->local_exp,          H:496
+//   link[n]   uint2   internal: (first child, slot mask)
+//                     leaf:     (first particle, PS_LEAF)                     H:371-381
+//   nend[n]   uint32  one past the node's last particle
+//   parent[n] uint32
+// Particles after the sort: the caller's SoA arrays plus pos4[i] =
+// (x, y, z, mass) and leaf_of[i] (the leaf that holds particle i).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PS_MAX_DEPTH 10            // H:882
+#define PS_LEAF      0x80000000u   // link.y flag of a leaf
+#define PS_NONE      0xFFFFFFFFu
+#define PS_SOFT2     0.1f          // softening eps^2, H:1065,1302,1433
+
+// ---------------------------------------------------------------------------
+// Morton code, H:777-799: bit k of each 10-bit axis goes to bit 3k (+0 x,
+// +1 y, +2 z).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ps_spread10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t ps_morton3(uint32_t x, uint32_t y, uint32_t z) {
+    return ps_spread10(x) | (ps_spread10(y) << 1) | (ps_spread10(z) << 2);
+}
+
+// ---------------------------------------------------------------------------
+// MAC, H:1056-1064 and H:1265-1273.  The comparison decides discrete pair
+// sets, so the float operation sequence must match the reference exactly:
+// no fused multiply-add, left-to-right sums.  dx is source minus target.
+// theta2 is fl(theta * theta), as the reference evaluates it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool ps_far(float4 t, float4 s, float theta2) {
+#pragma clang fp contract(off)
+    float dx = s.x - t.x;
+    float dy = s.y - t.y;
+    float dz = s.z - t.z;
+    float d2 = (dx * dx + dy * dy) + dz * dz;
+    float hs = t.w + s.w;
+    return d2 > theta2 * (hs * hs);
+}
+
+// ---------------------------------------------------------------------------
+// M2L, H:1065-1092: first-order (monopole + dipole) softened field of the
+// source cell at the target centre, in the reference's operation order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ps_m2l(float4 t, float4 s, float4 smp, float& ex, float& ey,
+                                       float& ez) {
+#pragma clang fp contract(off)
+    float dx = s.x - t.x;
+    float dy = s.y - t.y;
+    float dz = s.z - t.z;
+    float d2 = (dx * dx + dy * dy) + dz * dz;
+    float soft = d2 + PS_SOFT2;
+    float dist = sqrtf(soft);
+    float inv_r3 = 1.0f / (dist * soft);
+    float inv_r5 = inv_r3 / soft;
+    float fmx = smp.x * dx * inv_r3;
+    float fmy = smp.x * dy * inv_r3;
+    float fmz = smp.x * dz * inv_r3;
+    float mdr = (smp.y * dx + smp.z * dy) + smp.w * dz;
+    float coeff = 3.0f * mdr * inv_r5;
+    float fdx = smp.y * inv_r3 - dx * coeff;
+    float fdy = smp.z * inv_r3 - dy * coeff;
+    float fdz = smp.w * inv_r3 - dz * coeff;
+    ex += fmx + fdx;
+    ey += fmy + fdy;
+    ez += fmz + fdz;
+}
+
+// ---------------------------------------------------------------------------
+// One softened particle pair of the P2P inner loop, H:1427-1443:
+// a += m_j * d / (|d|^2 + eps^2)^(3/2), d = source - target.  v_rsq_f32 is
+// the FP32 reciprocal square root (within 1 ulp); contraction is allowed
+// here: the pair sum is compared within a tolerance, never bit for bit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ps_pair(float4 t, float4 s, float& ax, float& ay, float& az) {
+    float dx = s.x - t.x;
+    float dy = s.y - t.y;
+    float dz = s.z - t.z;
+    float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, PS_SOFT2)));
+    float inv = __builtin_amdgcn_rsqf(d2);
+    float f = s.w * (inv * inv * inv);
+    ax = fmaf(dx, f, ax);
+    ay = fmaf(dy, f, ay);
+    az = fmaf(dz, f, az);
+}
+
+// Kahan-compensated accumulation of one term.
+__device__ __forceinline__ void ps_kahan_add(float& sum, float& comp, float term) {
+#pragma clang fp contract(off)
+    float y = term - comp;
+    float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+__device__ __forceinline__ uint32_t ps_child_count(uint2 link) {
+    return (link.y & PS_LEAF) ? 0u : (uint32_t)__popc(link.y & 0xFFu);
+}
+
+__device__ __forceinline__ uint64_t ps_lanemask_lt() {
+    return (__lane_id() == 0) ? 0ull : ((~0ull) >> (64 - __lane_id()));
+}

@@ -1,0 +1,768 @@
+// ps_engine.cpp -- host orchestration of the gfx950 FMM tick and the C-ABI
+// of libpolesitter_hip.so (include/polesitter_hip.h).
+//
+// One engine per ps_context_t: a HIP stream, particle-sized device buffers
+// for max_particles (sort scratch, pos4, leaf_of, host-pointer staging),
+// node arrays for max_nodes, and traversal buffers grown on demand (kept
+// across ticks).  A tick is a stream-ordered kernel sequence,
+//   sort -> build -> upward -> traversal (M2L + P2P lists) -> L2L -> P2P+L2P
+// with host synchronisation only where sizes are data dependent (the node
+// count, and each traversal iteration's output sizes).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/polesitter_hip.h"
+#include "ps_kernels.h"
+
+namespace {
+
+int map_err(hipError_t e) {
+    if (e == hipSuccess) return PS_HIP_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return PS_HIP_EOOM;
+    if (e == hipErrorInvalidValue) return PS_HIP_EINVAL;
+    return PS_HIP_EDEVICE;
+}
+
+#define PS_CK(expr)                                   \
+    do {                                              \
+        hipError_t _e = (expr);                       \
+        if (_e != hipSuccess) return map_err(_e);     \
+    } while (0)
+
+#define PS_RC(expr)                                   \
+    do {                                              \
+        int _r = (expr);                              \
+        if (_r != PS_HIP_OK) return _r;               \
+    } while (0)
+
+bool is_device_ptr(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+// grow-only device buffer; synchronises the stream before replacing memory
+hipError_t grow(DevBuf& b, size_t need, hipStream_t s) {
+    if (need <= b.bytes && b.p) return hipSuccess;
+    if (b.p) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t nb = align256(std::max<size_t>(need + need / 2, 256));
+    hipError_t e = hipMalloc(&b.p, nb);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return e;
+    }
+    b.bytes = nb;
+    return hipSuccess;
+}
+
+void release(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+// most nodes n particles can produce: at most one new node per particle per
+// depth, and at most 8^d nodes at depth d
+size_t worst_case_nodes(size_t n) {
+    size_t total = 0, width = 1;
+    for (int d = 0; d <= 10; ++d) {
+        total += std::min(width, std::max<size_t>(n, 1));
+        width = width > (SIZE_MAX / 8) ? SIZE_MAX : width * 8;
+    }
+    return total;
+}
+
+enum Stage { kSort = 1, kBuild = 2, kUpward = 3, kInteract = 4, kDownward = 5, kP2P = 6 };
+
+}  // namespace
+
+struct ps_hip_engine {
+    int device = 0;
+    hipStream_t s = nullptr;
+    size_t max_p = 0;
+    size_t node_cap = 0;
+    float theta = 1.0f;
+    int trav = PS_HIP_TRAV_MT;
+    int kahan = 0;
+    int stats_on = 0;
+
+    // host-pointer staging, one buffer per array (allocated on first use)
+    DevBuf stage[9];
+    // sort scratch: two sets of (key, id, x, y, z, m)
+    DevBuf sort_t;
+    DevBuf sort_tmp, tree_tmp;
+    DevBuf pos4_b, leaf_of_b;
+    // nodes
+    DevBuf geom_b, mp_b, loc_b, link_b, nend_b, parent_b, p2p_off_b, p2p_len_b;
+    // traversal
+    DevBuf ent[2][3];
+    DevBuf lst[2];
+    DevBuf cnt_b, pos_b, scan_tmp, p2p_src_b;
+    // small device / pinned host scalars
+    DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs
+    uint32_t* h_small = nullptr;  // pinned mirror
+
+    // arrays of the current tick
+    float *x = nullptr, *y = nullptr, *z = nullptr, *m = nullptr;
+    float *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    uint32_t *id = nullptr, *code = nullptr;
+    size_t n = 0;
+    float4 root = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool bound = false, built = false;
+
+    uint32_t ls[12] = {0};
+    uint32_t n_nodes = 0, max_depth = 0;
+    ps_hip_stats_t st;
+    hipEvent_t ev[12] = {nullptr};
+
+    uint32_t* small_u32() { return reinterpret_cast<uint32_t*>(small_b.p); }
+    unsigned long long* totals() {
+        return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(small_b.p) + 64);
+    }
+    unsigned long long* pairs() { return totals() + 4; }
+    float4* pos4() { return reinterpret_cast<float4*>(pos4_b.p); }
+    uint32_t* leaf_of() { return reinterpret_cast<uint32_t*>(leaf_of_b.p); }
+    float4* geom() { return reinterpret_cast<float4*>(geom_b.p); }
+    float4* mp() { return reinterpret_cast<float4*>(mp_b.p); }
+    float4* loc() { return reinterpret_cast<float4*>(loc_b.p); }
+    uint2* link() { return reinterpret_cast<uint2*>(link_b.p); }
+    uint32_t* nend() { return reinterpret_cast<uint32_t*>(nend_b.p); }
+    uint32_t* parent() { return reinterpret_cast<uint32_t*>(parent_b.p); }
+    uint32_t* p2p_off() { return reinterpret_cast<uint32_t*>(p2p_off_b.p); }
+    uint32_t* p2p_len() { return reinterpret_cast<uint32_t*>(p2p_len_b.p); }
+    uint2* p2p_src() { return reinterpret_cast<uint2*>(p2p_src_b.p); }
+
+    PsSortArrays sort_set(int k) {
+        char* base = reinterpret_cast<char*>(sort_t.p) + (size_t)k * 6 * max_p * 4;
+        PsSortArrays a;
+        a.key = reinterpret_cast<uint32_t*>(base + 0 * max_p * 4);
+        a.id = reinterpret_cast<uint32_t*>(base + 1 * max_p * 4);
+        a.x = reinterpret_cast<float*>(base + 2 * max_p * 4);
+        a.y = reinterpret_cast<float*>(base + 3 * max_p * 4);
+        a.z = reinterpret_cast<float*>(base + 4 * max_p * 4);
+        a.m = reinterpret_cast<float*>(base + 5 * max_p * 4);
+        return a;
+    }
+
+    void record(int k) {
+        if (stats_on) (void)hipEventRecord(ev[k], s);
+    }
+    float elapsed(int a, int b) {
+        float ms = 0.f;
+        if (stats_on && hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess) return ms;
+        return 0.f;
+    }
+
+    void free_all() {
+        for (auto& b : stage) release(b);
+        DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
+                         &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
+                         &p2p_len_b, &lst[0],  &lst[1],    &cnt_b,     &pos_b,     &scan_tmp,
+                         &p2p_src_b, &small_b};
+        for (DevBuf* b : all) release(*b);
+        for (auto& es : ent)
+            for (auto& b : es) release(b);
+        if (h_small) (void)hipHostFree(h_small);
+        h_small = nullptr;
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+        s = nullptr;
+    }
+};
+
+namespace {
+
+hipError_t alloc_fixed(ps_hip_engine* e) {
+    const size_t P = e->max_p, M = e->node_cap;
+    hipError_t r;
+#define PS_AL(buf, bytes)                                     \
+    if ((r = grow(e->buf, (bytes), e->s)) != hipSuccess) return r;
+    PS_AL(sort_t, 2 * 6 * P * 4);
+    PS_AL(sort_tmp, ps_sort_tmp_bytes(P));
+    PS_AL(tree_tmp, ps_tree_tmp_bytes(P));
+    PS_AL(pos4_b, 16 * P);
+    PS_AL(leaf_of_b, 4 * P);
+    PS_AL(geom_b, 16 * M);
+    PS_AL(mp_b, 16 * M);
+    PS_AL(loc_b, 16 * M);
+    PS_AL(link_b, 8 * M);
+    PS_AL(nend_b, 4 * M);
+    PS_AL(parent_b, 4 * M);
+    PS_AL(p2p_off_b, 4 * M);
+    PS_AL(p2p_len_b, 4 * M);
+    PS_AL(small_b, 256);
+#undef PS_AL
+    r = hipHostMalloc(reinterpret_cast<void**>(&e->h_small), 256, hipHostMallocDefault);
+    return r;
+}
+
+// bind one caller array: device pointers are used in place, host arrays go
+// through the engine's staging buffer k
+template <class T>
+int bind_array(ps_hip_engine* e, int k, T* p, size_t n, bool upload, T** out) {
+    if (!p) return PS_HIP_EINVAL;
+    if (is_device_ptr(p)) {
+        *out = p;
+        return PS_HIP_OK;
+    }
+    PS_CK(grow(e->stage[k], std::max<size_t>(e->max_p, 1) * sizeof(T), e->s));
+    *out = reinterpret_cast<T*>(e->stage[k].p);
+    if (upload) PS_CK(hipMemcpyAsync(*out, p, n * sizeof(T), hipMemcpyHostToDevice, e->s));
+    return PS_HIP_OK;
+}
+
+int stage_sort(ps_hip_engine* e) {
+    PsSortArrays io = {e->code, e->id, e->x, e->y, e->z, e->m};
+    PS_CK(ps_radix_sort(io, e->sort_set(0), e->sort_set(1), e->pos4(), e->n, e->sort_tmp.p,
+                        e->s));
+    return PS_HIP_OK;
+}
+
+int stage_build(ps_hip_engine* e) {
+    uint32_t* small = e->small_u32();
+    PS_CK(ps_tree_count(e->code, e->n, small, small + 12, e->totals(), e->tree_tmp.p, e->s));
+    PS_CK(hipMemcpyAsync(e->h_small, e->small_b.p, 96, hipMemcpyDeviceToHost, e->s));
+    PS_CK(hipStreamSynchronize(e->s));
+    const uint32_t flags = e->h_small[12];
+    const unsigned long long total =
+        *reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(e->h_small) + 64);
+    if (flags & 1u) return PS_HIP_EINVAL;  // codes must be 30-bit Morton codes
+    if (total > e->node_cap) return PS_HIP_EOOM;
+    std::memcpy(e->ls, e->h_small, sizeof(e->ls));
+    e->n_nodes = (uint32_t)total;
+    e->max_depth = 0;
+    for (int d = 0; d <= 10; ++d)
+        if (e->ls[d + 1] > e->ls[d]) e->max_depth = (uint32_t)d;
+    PS_CK(ps_tree_write(e->code, e->n, e->root, e->tree_tmp.p, e->geom(), e->link(), e->nend(),
+                        e->parent(), e->leaf_of(), e->n_nodes, e->s));
+    ps_fill_u32(reinterpret_cast<uint32_t*>(e->loc()), 0u, 4 * (size_t)e->n_nodes, e->s);
+    ps_fill_u32(e->p2p_len(), 0u, e->n_nodes, e->s);
+    e->built = true;
+    return PS_HIP_OK;
+}
+
+int stage_upward(ps_hip_engine* e) {
+    for (int d = (int)e->max_depth; d >= 0; --d)
+        ps_launch_upward_level(e->ls[d], e->ls[d + 1], e->pos4(), e->geom(), e->link(), e->nend(),
+                               e->mp(), e->s);
+    return map_err(hipGetLastError());
+}
+
+int stage_traverse(ps_hip_engine* e) {
+    float theta2;
+    {
+        volatile float t = e->theta;  // fl(theta * theta), H:1064
+        theta2 = t * t;
+    }
+    const bool mt = e->trav == PS_HIP_TRAV_MT && e->n_nodes > 1;
+    uint32_t n_ent = mt ? (e->ls[3] - e->ls[1]) : 1u;
+    const int c0 = 0;
+    PS_CK(grow(e->ent[c0][0], 4 * (size_t)n_ent, e->s));
+    PS_CK(grow(e->ent[c0][1], 4 * (size_t)n_ent, e->s));
+    PS_CK(grow(e->ent[c0][2], 4 * (size_t)n_ent, e->s));
+    PS_CK(grow(e->lst[c0], 4, e->s));
+    ps_fill_u32(reinterpret_cast<uint32_t*>(e->lst[c0].p), 0u, 1, e->s);  // {root}
+    auto u32 = [](DevBuf& b) { return reinterpret_cast<uint32_t*>(b.p); };
+    if (mt)
+        ps_launch_trav_init(e->ls[1], e->ls[3], e->ls[2], e->link(), u32(e->ent[c0][0]),
+                            u32(e->ent[c0][1]), u32(e->ent[c0][2]), e->s);
+    else
+        ps_launch_trav_init(0u, 1u, 0u, e->link(), u32(e->ent[c0][0]), u32(e->ent[c0][1]),
+                            u32(e->ent[c0][2]), e->s);
+
+    PsTravTree t;
+    t.geom = e->geom();
+    t.mp = e->mp();
+    t.link = e->link();
+    t.nend = e->nend();
+    t.loc = e->loc();
+    t.p2p_off = e->p2p_off();
+    t.p2p_len = e->p2p_len();
+
+    unsigned long long p2p_base = 0, m2l = 0, lists = 0;
+    uint32_t iters = 0;
+    int cur = 0;
+    while (n_ent > 0) {
+        const int nxt = cur ^ 1;
+        PS_CK(grow(e->cnt_b, 16 * (size_t)n_ent, e->s));
+        PS_CK(grow(e->pos_b, 16 * (size_t)n_ent, e->s));
+        PS_CK(grow(e->scan_tmp, ps_scan_tmp_bytes_u4(n_ent), e->s));
+        uint4* cnt = reinterpret_cast<uint4*>(e->cnt_b.p);
+        uint4* pos = reinterpret_cast<uint4*>(e->pos_b.p);
+        e->record(8);
+        ps_launch_trav_count(t, theta2, u32(e->ent[cur][0]), u32(e->ent[cur][1]),
+                             u32(e->ent[cur][2]), u32(e->lst[cur]), n_ent, cnt, e->kahan, e->s);
+        e->record(9);
+        PS_CK(ps_scan_u4(cnt, pos, n_ent, e->totals(), e->scan_tmp.p, e->s));
+        PS_CK(hipMemcpyAsync(e->h_small, e->totals(), 32, hipMemcpyDeviceToHost, e->s));
+        PS_CK(hipStreamSynchronize(e->s));
+        e->st.ms_trav_count += e->elapsed(8, 9);
+        if (iters > 0) e->st.ms_trav_write += e->elapsed(10, 11);
+        const unsigned long long* tot = reinterpret_cast<unsigned long long*>(e->h_small);
+        const unsigned long long list_n = tot[0], ent_n = tot[1], p2p_n = tot[2];
+        m2l += tot[3];
+        lists += list_n;
+        if (list_n > 0xFFFFFFFFull || ent_n > 0xFFFFFFFFull ||
+            p2p_base + p2p_n > 0xFFFFFFFFull)
+            return PS_HIP_EOOM;
+        PS_CK(grow(e->lst[nxt], 4 * std::max<unsigned long long>(list_n, 1), e->s));
+        for (int k = 0; k < 3; ++k)
+            PS_CK(grow(e->ent[nxt][k], 4 * std::max<unsigned long long>(ent_n, 1), e->s));
+        PS_CK(grow(e->p2p_src_b, 8 * std::max<unsigned long long>(p2p_base + p2p_n, 1), e->s));
+        t.p2p_src = e->p2p_src();
+        e->record(10);
+        ps_launch_trav_write(t, theta2, u32(e->ent[cur][0]), u32(e->ent[cur][1]),
+                             u32(e->ent[cur][2]), u32(e->lst[cur]), n_ent, cnt, pos,
+                             (uint32_t)p2p_base, u32(e->ent[nxt][0]), u32(e->ent[nxt][1]),
+                             u32(e->ent[nxt][2]), u32(e->lst[nxt]), e->s);
+        e->record(11);
+        p2p_base += p2p_n;
+        n_ent = (uint32_t)ent_n;
+        cur = nxt;
+        ++iters;
+    }
+    if (iters > 0 && e->stats_on) {
+        PS_CK(hipEventSynchronize(e->ev[11]));
+        e->st.ms_trav_write += e->elapsed(10, 11);
+    }
+    e->st.m2l_pairs = m2l;
+    e->st.p2p_leaf_pairs = p2p_base;
+    e->st.list_entries = lists;
+    e->st.iterations = iters;
+    if (!e->p2p_src_b.p) PS_CK(grow(e->p2p_src_b, 8, e->s));
+    return map_err(hipGetLastError());
+}
+
+int stage_l2l(ps_hip_engine* e) {
+    for (uint32_t d = 1; d <= e->max_depth; ++d)
+        ps_launch_l2l_level(e->ls[d], e->ls[d + 1], e->parent(), e->loc(), e->s);
+    return map_err(hipGetLastError());
+}
+
+int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
+    unsigned long long* pc = nullptr;
+    if (e->stats_on) {
+        pc = e->pairs();
+        ps_fill_u32(reinterpret_cast<uint32_t*>(pc), 0u, 2, e->s);
+    }
+    if (!e->p2p_src_b.p) PS_CK(grow(e->p2p_src_b, 8, e->s));
+    if (ev_before >= 0) e->record(ev_before);
+    ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
+                  e->p2p_src(), e->fx, e->fy, e->fz, mode, e->kahan, pc, e->s);
+    return map_err(hipGetLastError());
+}
+
+int run_tick(ps_hip_engine* e) {
+    e->record(1);
+    PS_RC(stage_sort(e));
+    e->record(2);
+    PS_RC(stage_build(e));
+    e->record(3);
+    PS_RC(stage_upward(e));
+    e->record(4);
+    PS_RC(stage_traverse(e));
+    e->record(5);
+    PS_RC(stage_l2l(e));
+    PS_RC(stage_p2p(e, 0, 6));  // records event 6 right before the P2P launch
+    e->record(7);
+    return PS_HIP_OK;
+}
+
+struct PrepWS {
+    int dev = -1;
+    hipStream_t s = nullptr;
+    DevBuf part, prm, x, y, z, code;
+    float* h_out = nullptr;
+};
+
+std::mutex g_prep_mu;
+std::vector<PrepWS*> g_prep;
+
+PrepWS* prep_ws(int dev, hipError_t* err) {
+    for (PrepWS* w : g_prep)
+        if (w->dev == dev) return w;
+    PrepWS* w = new (std::nothrow) PrepWS();
+    if (!w) {
+        *err = hipErrorOutOfMemory;
+        return nullptr;
+    }
+    w->dev = dev;
+    if ((*err = hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking)) != hipSuccess ||
+        (*err = grow(w->part, 1024 * sizeof(float2), w->s)) != hipSuccess ||
+        (*err = grow(w->prm, 16, w->s)) != hipSuccess ||
+        (*err = hipHostMalloc(reinterpret_cast<void**>(&w->h_out), 64, hipHostMallocDefault)) !=
+            hipSuccess) {
+        delete w;
+        return nullptr;
+    }
+    g_prep.push_back(w);
+    return w;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int ps_hip_engine_create(ps_hip_engine_t** out, size_t max_particles, size_t max_nodes,
+                         float theta, int traversal) {
+    if (!out) return PS_HIP_EINVAL;
+    *out = nullptr;
+    ps_hip_engine* e = new (std::nothrow) ps_hip_engine();
+    if (!e) return PS_HIP_EOOM;
+    std::memset(&e->st, 0, sizeof(e->st));
+    e->max_p = std::max<size_t>(max_particles, 1);
+    e->node_cap = std::max<size_t>(std::min(max_nodes, worst_case_nodes(e->max_p)), 1);
+    e->theta = theta > 0.0f ? theta : 1.0f;
+    e->trav = traversal == PS_HIP_TRAV_ST ? PS_HIP_TRAV_ST : PS_HIP_TRAV_MT;
+    hipError_t r = hipGetDevice(&e->device);
+    if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking);
+    for (int k = 0; k < 12 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev[k]);
+    if (r == hipSuccess) r = alloc_fixed(e);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        e->free_all();
+        delete e;
+        return map_err(r) == PS_HIP_EINVAL ? PS_HIP_EDEVICE : map_err(r);
+    }
+    *out = e;
+    return PS_HIP_OK;
+}
+
+int ps_hip_engine_destroy(ps_hip_engine_t* e) {
+    if (!e) return PS_HIP_EINVAL;
+    if (e->s) (void)hipStreamSynchronize(e->s);
+    e->free_all();
+    delete e;
+    return PS_HIP_OK;
+}
+
+int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
+    if (!e) return PS_HIP_EINVAL;
+    switch (key) {
+        case PS_HIP_OPT_TRAVERSAL:
+            if (value != PS_HIP_TRAV_MT && value != PS_HIP_TRAV_ST) return PS_HIP_EINVAL;
+            e->trav = (int)value;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_KAHAN:
+            e->kahan = value != 0.0;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_THETA:
+            e->theta = value > 0.0 ? (float)value : 1.0f;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_STATS:
+            e->stats_on = value != 0.0;
+            return PS_HIP_OK;
+        default:
+            return PS_HIP_EINVAL;
+    }
+}
+
+int ps_hip_engine_get_stats(const ps_hip_engine_t* e, ps_hip_stats_t* out) {
+    if (!e || !out) return PS_HIP_EINVAL;
+    *out = e->st;
+    return PS_HIP_OK;
+}
+
+int ps_hip_prepare(const float* x, const float* y, const float* z, size_t n, uint32_t* codes,
+                   float* out_min, float* out_max, float* out_range, int morton_mode) {
+    if (!x || !y || !z || !codes || n == 0) return PS_HIP_EINVAL;  // H:1914
+    if (morton_mode != PS_HIP_MORTON_TRUNC && morton_mode != PS_HIP_MORTON_AVX2)
+        return PS_HIP_EINVAL;
+    int dev = 0;
+    PS_CK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_prep_mu);
+    hipError_t err = hipSuccess;
+    PrepWS* w = prep_ws(dev, &err);
+    if (!w) return map_err(err);
+    const float* in[3] = {x, y, z};
+    DevBuf* stg[3] = {&w->x, &w->y, &w->z};
+    const float* d[3];
+    for (int a = 0; a < 3; ++a) {
+        if (is_device_ptr(in[a])) {
+            d[a] = in[a];
+        } else {
+            PS_CK(grow(*stg[a], n * 4, w->s));
+            PS_CK(hipMemcpyAsync(stg[a]->p, in[a], n * 4, hipMemcpyHostToDevice, w->s));
+            d[a] = reinterpret_cast<const float*>(stg[a]->p);
+        }
+    }
+    uint32_t* dcode = codes;
+    const bool host_codes = !is_device_ptr(codes);
+    if (host_codes) {
+        PS_CK(grow(w->code, n * 4, w->s));
+        dcode = reinterpret_cast<uint32_t*>(w->code.p);
+    }
+    float* prm = reinterpret_cast<float*>(w->prm.p);
+    ps_launch_bbox(d[0], d[1], d[2], n, reinterpret_cast<float2*>(w->part.p), prm, w->s);
+    ps_launch_morton(d[0], d[1], d[2], n, prm, dcode, morton_mode, w->s);
+    PS_CK(hipMemcpyAsync(w->h_out, prm, 16, hipMemcpyDeviceToHost, w->s));
+    if (host_codes) PS_CK(hipMemcpyAsync(codes, dcode, n * 4, hipMemcpyDeviceToHost, w->s));
+    PS_CK(hipStreamSynchronize(w->s));
+    if (out_min) *out_min = w->h_out[0];
+    if (out_max) *out_max = w->h_out[1];
+    if (out_range) *out_range = w->h_out[2];
+    return PS_HIP_OK;
+}
+
+static int bind_all(ps_hip_engine_t* e, float* x, float* y, float* z, float* mass, float* fx,
+                    float* fy, float* fz, uint32_t* id, size_t n, uint32_t* codes, bool upload) {
+    PS_RC(bind_array(e, 0, x, n, upload, &e->x));
+    PS_RC(bind_array(e, 1, y, n, upload, &e->y));
+    PS_RC(bind_array(e, 2, z, n, upload, &e->z));
+    PS_RC(bind_array(e, 3, mass, n, upload, &e->m));
+    PS_RC(bind_array(e, 4, fx, n, false, &e->fx));
+    PS_RC(bind_array(e, 5, fy, n, false, &e->fy));
+    PS_RC(bind_array(e, 6, fz, n, false, &e->fz));
+    PS_RC(bind_array(e, 7, id, n, upload, &e->id));
+    PS_RC(bind_array(e, 8, codes, n, upload, &e->code));
+    return PS_HIP_OK;
+}
+
+int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* mass, float* fx,
+                       float* fy, float* fz, uint32_t* id, size_t n, uint32_t* codes, float cx,
+                       float cy, float cz, float hw) {
+    if (!e) return PS_HIP_EINVAL;
+    if (n == 0) return PS_HIP_OK;  // nothing to sort or build (H:1592)
+    if (n > e->max_p) return PS_HIP_EINVAL;
+    std::memset(&e->st, 0, sizeof(e->st));
+    e->record(0);
+    e->n = n;
+    e->root = make_float4(cx, cy, cz, hw);
+    PS_RC(bind_all(e, x, y, z, mass, fx, fy, fz, id, n, codes, true));
+    e->bound = true;
+    int rc = run_tick(e);
+    if (rc != PS_HIP_OK) {
+        (void)hipStreamSynchronize(e->s);
+        return rc;
+    }
+    // copy staged arrays back: the sorted payload and the forces
+    void* host[9] = {x, y, z, mass, fx, fy, fz, id, codes};
+    void* dev[9] = {e->x, e->y, e->z, e->m, e->fx, e->fy, e->fz, e->id, e->code};
+    for (int k = 0; k < 9; ++k)
+        if (host[k] != dev[k])
+            PS_CK(hipMemcpyAsync(host[k], dev[k], n * 4, hipMemcpyDeviceToHost, e->s));
+    if (e->stats_on) PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 8, hipMemcpyDeviceToHost, e->s));
+    PS_CK(hipEventRecord(e->ev[7], e->s));
+    PS_CK(hipStreamSynchronize(e->s));
+    e->st.n_particles = n;
+    e->st.n_nodes = e->n_nodes;
+    e->st.max_depth = e->max_depth;
+    std::memcpy(e->st.level_start, e->ls, sizeof(e->ls));
+    if (e->stats_on) {
+        e->st.p2p_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 32);
+        e->st.ms_sort = e->elapsed(1, 2);
+        e->st.ms_build = e->elapsed(2, 3);
+        e->st.ms_upward = e->elapsed(3, 4);
+        e->st.ms_traverse = e->elapsed(4, 5);
+        e->st.ms_l2l = e->elapsed(5, 6);
+        e->st.ms_p2p = e->elapsed(6, 7);
+        e->st.ms_upload = e->elapsed(0, 1);
+        e->st.ms_total = e->elapsed(0, 7);
+    }
+    return PS_HIP_OK;
+}
+
+int ps_hip_engine_export_tree(ps_hip_engine_t* e, size_t cap, float* geom4, float* mp4,
+                              float* loc4, uint32_t* link2, uint32_t* nend, uint32_t* parent,
+                              uint32_t* n_nodes) {
+    if (!e || !e->built) return PS_HIP_EINVAL;
+    if (n_nodes) *n_nodes = e->n_nodes;
+    const size_t M = e->n_nodes;
+    if (cap < M) return PS_HIP_EOOM;
+    PS_CK(hipStreamSynchronize(e->s));
+    if (geom4) PS_CK(hipMemcpy(geom4, e->geom(), 16 * M, hipMemcpyDeviceToHost));
+    if (mp4) PS_CK(hipMemcpy(mp4, e->mp(), 16 * M, hipMemcpyDeviceToHost));
+    if (loc4) PS_CK(hipMemcpy(loc4, e->loc(), 16 * M, hipMemcpyDeviceToHost));
+    if (link2) PS_CK(hipMemcpy(link2, e->link(), 8 * M, hipMemcpyDeviceToHost));
+    if (nend) PS_CK(hipMemcpy(nend, e->nend(), 4 * M, hipMemcpyDeviceToHost));
+    if (parent) PS_CK(hipMemcpy(parent, e->parent(), 4 * M, hipMemcpyDeviceToHost));
+    return PS_HIP_OK;
+}
+
+int ps_hip_debug_bind(ps_hip_engine_t* e, float* x, float* y, float* z, float* mass, float* fx,
+                      float* fy, float* fz, uint32_t* id, size_t n, uint32_t* codes, float cx,
+                      float cy, float cz, float hw) {
+    if (!e || n == 0 || n > e->max_p) return PS_HIP_EINVAL;
+    void* arrs[9] = {x, y, z, mass, fx, fy, fz, id, codes};
+    for (void* p : arrs)
+        if (!is_device_ptr(p)) return PS_HIP_EINVAL;
+    e->n = n;
+    e->root = make_float4(cx, cy, cz, hw);
+    PS_RC(bind_all(e, x, y, z, mass, fx, fy, fz, id, n, codes, false));
+    // pos4 for stages that skip the sort
+    ps_launch_pack_pos4(e->x, e->y, e->z, e->m, n, e->pos4(), e->s);
+    e->bound = true;
+    e->built = false;
+    return map_err(hipGetLastError());
+}
+
+int ps_hip_debug_stage(ps_hip_engine_t* e, int stage) {
+    if (!e || !e->bound) return PS_HIP_EINVAL;
+    if (stage > kBuild && !e->built) return PS_HIP_EINVAL;
+    int rc;
+    switch (stage) {
+        case kSort: rc = stage_sort(e); break;
+        case kBuild: rc = stage_build(e); break;
+        case kUpward: rc = stage_upward(e); break;
+        case kInteract: rc = stage_traverse(e); break;
+        case kDownward:
+            rc = stage_l2l(e);
+            if (rc == PS_HIP_OK) rc = stage_p2p(e, 1);
+            break;
+        case kP2P: rc = stage_p2p(e, 2); break;
+        default: return PS_HIP_EINVAL;
+    }
+    if (rc != PS_HIP_OK) return rc;
+    PS_CK(hipStreamSynchronize(e->s));
+    return PS_HIP_OK;
+}
+
+int ps_hip_debug_set_local(ps_hip_engine_t* e, uint32_t node, const float* field3) {
+    if (!e || !e->built || !field3 || node >= e->n_nodes) return PS_HIP_EINVAL;
+    float v[4] = {0.f, field3[0], field3[1], field3[2]};
+    PS_CK(hipMemcpyAsync(e->loc() + node, v, 16, hipMemcpyHostToDevice, e->s));
+    PS_CK(hipStreamSynchronize(e->s));
+    return PS_HIP_OK;
+}
+
+int ps_hip_op_m2l(const float* tgeom4, const float* sgeom4, const float* smp4, float theta,
+                  float* out4) {
+    if (!tgeom4 || !sgeom4 || !smp4 || !out4) return PS_HIP_EINVAL;
+    float* d = nullptr;
+    PS_CK(hipMalloc(reinterpret_cast<void**>(&d), 16));
+    hipError_t r = ps_op_m2l(tgeom4, sgeom4, smp4, theta, d, nullptr);
+    if (r == hipSuccess) r = hipMemcpy(out4, d, 16, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return map_err(r);
+}
+
+int ps_hip_op_morton_exhaustive(uint64_t* out2) {
+    if (!out2) return PS_HIP_EINVAL;
+    unsigned long long* d = nullptr;
+    PS_CK(hipMalloc(reinterpret_cast<void**>(&d), 16));
+    hipError_t r = ps_op_morton_exhaustive(d, nullptr);
+    if (r == hipSuccess) r = hipMemcpy(out2, d, 16, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return map_err(r);
+}
+
+int ps_hip_direct_f64(const float* x, const float* y, const float* z, const float* mass,
+                      size_t n, const uint32_t* targets, size_t nt, double* out3) {
+    if (!x || !y || !z || !mass || !targets || !out3 || n == 0) return PS_HIP_EINVAL;
+    if (!is_device_ptr(x) || !is_device_ptr(y) || !is_device_ptr(z) || !is_device_ptr(mass))
+        return PS_HIP_EINVAL;
+    for (size_t k = 0; k < nt; ++k)
+        if (targets[k] >= n) return PS_HIP_EINVAL;
+    if (nt == 0) return PS_HIP_OK;
+    float4* pos4 = nullptr;
+    uint32_t* dt = nullptr;
+    double* dout = nullptr;
+    hipError_t r = hipMalloc(reinterpret_cast<void**>(&pos4), 16 * n);
+    if (r == hipSuccess) r = hipMalloc(reinterpret_cast<void**>(&dt), 4 * nt);
+    if (r == hipSuccess) r = hipMalloc(reinterpret_cast<void**>(&dout), 24 * nt);
+    if (r == hipSuccess) r = hipMemcpy(dt, targets, 4 * nt, hipMemcpyHostToDevice);
+    if (r == hipSuccess) {
+        ps_launch_pack_pos4(x, y, z, mass, n, pos4, nullptr);
+        ps_launch_direct_f64(pos4, n, dt, nt, dout, nullptr);
+        r = hipGetLastError();
+    }
+    if (r == hipSuccess) r = hipMemcpy(out3, dout, 24 * nt, hipMemcpyDeviceToHost);
+    if (pos4) (void)hipFree(pos4);
+    if (dt) (void)hipFree(dt);
+    if (dout) (void)hipFree(dout);
+    return map_err(r);
+}
+
+int ps_hip_step_reset_ids(ps_hip_engine_t* e, uint32_t* id, size_t n) {
+    if (!e || !id || !is_device_ptr(id)) return PS_HIP_EINVAL;
+    ps_launch_iota(id, n, e->s);
+    PS_CK(hipStreamSynchronize(e->s));
+    return PS_HIP_OK;
+}
+
+int ps_hip_step_gather(ps_hip_engine_t* e, const uint32_t* id, float* vx, float* vy, float* vz,
+                       float* tmp3, size_t n) {
+    if (!e || !id || !vx || !vy || !vz || !tmp3) return PS_HIP_EINVAL;
+    ps_launch_gather_vel(id, vx, vy, vz, tmp3, tmp3 + n, tmp3 + 2 * n, n, e->s);
+    PS_CK(hipMemcpyAsync(vx, tmp3, 4 * n, hipMemcpyDeviceToDevice, e->s));
+    PS_CK(hipMemcpyAsync(vy, tmp3 + n, 4 * n, hipMemcpyDeviceToDevice, e->s));
+    PS_CK(hipMemcpyAsync(vz, tmp3 + 2 * n, 4 * n, hipMemcpyDeviceToDevice, e->s));
+    PS_CK(hipStreamSynchronize(e->s));
+    return PS_HIP_OK;
+}
+
+int ps_hip_step_euler(ps_hip_engine_t* e, float* x, float* y, float* z, float* vx, float* vy,
+                      float* vz, const float* fx, const float* fy, const float* fz,
+                      const float* mass, float dt, size_t n) {
+    if (!e || !x || !y || !z || !vx || !vy || !vz || !fx || !fy || !fz || !mass)
+        return PS_HIP_EINVAL;
+    ps_launch_euler(x, y, z, vx, vy, vz, fx, fy, fz, mass, dt, n, e->s);
+    PS_CK(hipStreamSynchronize(e->s));
+    return PS_HIP_OK;
+}
+
+void* ps_hip_malloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+int ps_hip_free(void* p) {
+    if (!p) return PS_HIP_OK;
+    return map_err(hipFree(p));
+}
+
+int ps_hip_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+    if (!dst || !src) return PS_HIP_EINVAL;
+    hipMemcpyKind k = kind == 1   ? hipMemcpyHostToDevice
+                      : kind == 2 ? hipMemcpyDeviceToHost
+                      : kind == 3 ? hipMemcpyDeviceToDevice
+                                  : hipMemcpyDefault;
+    return map_err(hipMemcpy(dst, src, bytes, k));
+}
+
+int ps_hip_synchronize(void) { return map_err(hipDeviceSynchronize()); }
+
+int ps_hip_device_name(char* buf, size_t len) {
+    if (!buf || len == 0) return PS_HIP_EINVAL;
+    int dev = 0;
+    PS_CK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    PS_CK(hipGetDeviceProperties(&p, dev));
+    std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return PS_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// ps_kernels.h -- host-side launchers of the gfx950 kernels (one .hip file
+// per stage).  All launchers are asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+// ---- ps_scan.hip: device-wide exclusive prefix sums ------------------------
+// out[i] = in[0] + ... + in[i-1] (component-wise for uint4); in == out is
+// allowed.  When total64 is given it is overwritten with the exact 64-bit sum
+// of every component (1 value for uint32, 4 for uint4).
+size_t ps_scan_tmp_bytes_u32(size_t n);
+size_t ps_scan_tmp_bytes_u4(size_t n);
+hipError_t ps_scan_u32(const uint32_t* in, uint32_t* out, size_t n,
+                       unsigned long long* total64, void* tmp, hipStream_t s);
+hipError_t ps_scan_u4(const uint4* in, uint4* out, size_t n, unsigned long long* total64,
+                      void* tmp, hipStream_t s);
+// graph-safe fills (no memset nodes)
+void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+
+// ---- ps_prepare.hip: bounding cube + Morton codes, H:1911-2170 --------------
+// prm (device, 4 floats): min, max, range, 1023/range.  part: >= 1024 float2.
+void ps_launch_bbox(const float* x, const float* y, const float* z, size_t n, float2* part,
+                    float* prm, hipStream_t s);
+// mode 0: scalar truncation for every particle (H:2132-2157);
+// mode 1: AVX2 rounding for the first 8*floor(n/8), scalar tail (H:2040-2072)
+void ps_launch_morton(const float* x, const float* y, const float* z, size_t n,
+                      const float* prm, uint32_t* codes, int mode, hipStream_t s);
+
+// ---- ps_sort.hip: stable LSD radix sort (3 x 10-bit digits), H:1588-1693 ----
+struct PsSortArrays {
+    uint32_t* key;
+    uint32_t* id;
+    float* x;
+    float* y;
+    float* z;
+    float* m;
+};
+size_t ps_sort_tmp_bytes(size_t n);
+// Sorts `io` in place by key bits 0..29 (keys must be < 2^30), moving the
+// payload; t0/t1 are two scratch sets of n elements.  pos4 (optional) gets the
+// sorted (x, y, z, m).
+hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
+                         size_t n, void* tmp, hipStream_t s);
+
+// ---- ps_tree.hip: octree build, upward and L2L passes ------------------------
+size_t ps_tree_tmp_bytes(size_t n);
+// Phase 1: per-depth node counts.  Writes level_start[0..11] (device) and
+// flags[0] |= 1 if any key has bits 30/31 set.  total64[0] = node count.
+hipError_t ps_tree_count(const uint32_t* codes, size_t n, uint32_t* level_start,
+                         uint32_t* flags, unsigned long long* total64, void* tmp,
+                         hipStream_t s);
+// Phase 2: writes every node (needs phase 1's tmp untouched).
+hipError_t ps_tree_write(const uint32_t* codes, size_t n, float4 root, void* tmp,
+                         float4* geom, uint2* link, uint32_t* nend, uint32_t* parent,
+                         uint32_t* leaf_of, size_t n_nodes, hipStream_t s);
+void ps_launch_upward_level(uint32_t begin, uint32_t end, const float4* pos4,
+                            const float4* geom, const uint2* link, const uint32_t* nend,
+                            float4* mp, hipStream_t s);
+void ps_launch_l2l_level(uint32_t begin, uint32_t end, const uint32_t* parent, float4* loc,
+                         hipStream_t s);
+
+// ---- ps_traverse.hip: dual-tree traversal, M2L, P2P lists -------------------
+struct PsTravTree {
+    const float4* geom;
+    const float4* mp;
+    const uint2* link;
+    const uint32_t* nend;
+    float4* loc;
+    uint32_t* p2p_off;
+    uint32_t* p2p_len;
+    uint2* p2p_src;
+};
+// iteration-0 entries: depth-1 leaves and all depth-2 nodes (MT) with list {root}
+void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* link,
+                         uint32_t* tgt, uint32_t* off, uint32_t* len, hipStream_t s);
+void ps_launch_trav_count(PsTravTree t, float theta2, const uint32_t* tgt, const uint32_t* off,
+                          const uint32_t* len, const uint32_t* list, uint32_t n_ent,
+                          uint4* cnt, int kahan, hipStream_t s);
+void ps_launch_trav_write(PsTravTree t, float theta2, const uint32_t* tgt, const uint32_t* off,
+                          const uint32_t* len, const uint32_t* list, uint32_t n_ent,
+                          const uint4* cnt, const uint4* pos, uint32_t p2p_base,
+                          uint32_t* ntgt, uint32_t* noff, uint32_t* nlen, uint32_t* nlist,
+                          hipStream_t s);
+
+// ---- ps_p2p.hip: near field + L2P ------------------------------------------
+// mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);
+// 2: f += m*sum (P2P only).
+void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
+                   const uint32_t* p2p_off, const uint32_t* p2p_len, const uint2* p2p_src,
+                   float* fx, float* fy, float* fz, int mode, int kahan,
+                   unsigned long long* pair_count, hipStream_t s);
+
+// ---- ps_direct.hip: FP64 direct sum (the error oracle on the device) -------
+void ps_launch_direct_f64(const float4* pos4, size_t n, const uint32_t* targets, size_t nt,
+                          double* out3, hipStream_t s);
+void ps_launch_pack_pos4(const float* x, const float* y, const float* z, const float* m,
+                         size_t n, float4* pos4, hipStream_t s);
+
+// ---- ps_step.hip: the caller's step around the tick (README.md:128-161) ----
+void ps_launch_gather_vel(const uint32_t* id, const float* vx, const float* vy,
+                          const float* vz, float* ox, float* oy, float* oz, size_t n,
+                          hipStream_t s);
+void ps_launch_euler(float* x, float* y, float* z, float* vx, float* vy, float* vz,
+                     const float* fx, const float* fy, const float* fz, const float* m,
+                     float dt, size_t n, hipStream_t s);
+void ps_launch_iota(uint32_t* id, size_t n, hipStream_t s);
+
+// ---- ps_ops.hip: single-operator hooks for the known-answer tests ----------
+hipError_t ps_op_m2l(const float* tgeom, const float* sgeom, const float* smp, float theta,
+                     float* out4, hipStream_t s);
+hipError_t ps_op_morton_exhaustive(unsigned long long* out2, hipStream_t s);

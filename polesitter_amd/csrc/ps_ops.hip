@@ -1,0 +1,71 @@
+// ps_ops.hip -- single-operator entry points for the known-answer tests.
+// They run the very device functions the tick kernels use (ps_far, ps_m2l,
+// ps_morton3) on explicit inputs, mirroring the white-box tests of
+// tests/main.c:50-72,282-323 and tests/accuracy.c:74-109.
+#include "ps_device.h"
+#include "ps_kernels.h"
+
+namespace {
+
+// MAC + M2L of ps_impl_fmm_interaction_pass for one (target, source) pair,
+// H:1056-1095: out = (far, Ex, Ey, Ez)
+__global__ void op_m2l_kernel(float4 tg, float4 sg, float4 smp, float theta2, float* out) {
+    float ex = 0.f, ey = 0.f, ez = 0.f;
+    bool far = ps_far(tg, sg, theta2);
+    if (far) ps_m2l(tg, sg, smp, ex, ey, ez);
+    out[0] = far ? 1.0f : 0.0f;
+    out[1] = ex;
+    out[2] = ey;
+    out[3] = ez;
+}
+
+// every (x, y, z) in [0,1024)^3: ps_morton3 against a plain bit loop
+__global__ void op_morton_kernel(unsigned long long* out2) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;  // 30 bits: z|y|x
+    const uint32_t x = v & 1023u, y = (v >> 10) & 1023u, z = (v >> 20) & 1023u;
+    const uint32_t code = ps_morton3(x, y, z);
+    uint32_t naive = 0;
+    for (int b = 0; b < 10; ++b) {
+        naive |= ((x >> b) & 1u) << (3 * b);
+        naive |= ((y >> b) & 1u) << (3 * b + 1);
+        naive |= ((z >> b) & 1u) << (3 * b + 2);
+    }
+    unsigned long long bad = (code != naive) ? 1ull : 0ull;
+    unsigned long long sum = code;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        bad += __shfl_down(bad, d, 64);
+        sum += __shfl_down(sum, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (bad) atomicAdd(&out2[0], bad);
+        atomicAdd(&out2[1], sum);
+    }
+}
+
+__global__ void zero2_kernel(unsigned long long* p) {
+    p[0] = 0;
+    p[1] = 0;
+}
+
+}  // namespace
+
+hipError_t ps_op_m2l(const float* tgeom, const float* sgeom, const float* smp, float theta,
+                     float* out4, hipStream_t s) {
+    float theta2;
+    {
+#pragma clang fp contract(off)
+        theta2 = theta * theta;
+    }
+    op_m2l_kernel<<<1, 1, 0, s>>>(make_float4(tgeom[0], tgeom[1], tgeom[2], tgeom[3]),
+                                  make_float4(sgeom[0], sgeom[1], sgeom[2], sgeom[3]),
+                                  make_float4(smp[0], smp[1], smp[2], smp[3]), theta2, out4);
+    return hipGetLastError();
+}
+
+hipError_t ps_op_morton_exhaustive(unsigned long long* out2, hipStream_t s) {
+    zero2_kernel<<<1, 1, 0, s>>>(out2);
+    // 2^30 threads: 2^22 blocks of 256
+    op_morton_kernel<<<1u << 22, 256, 0, s>>>(out2);
+    return hipGetLastError();
+}

@@ -1,0 +1,114 @@
+// ps_p2p.hip -- near field (P2P) fused with L2P (replaces the leaf x leaf
+// loop of H:1279-1452 and L2P of H:1158-1235).
+//
+// One thread per target particle, particles in sorted order: the lanes of a
+// wave are spatially adjacent, and in dense leaves they share one leaf and so
+// the same source list -- every source load is the same address across the
+// wave.  The source list of a leaf is the CSR range the traversal wrote:
+// (first, end) particle ranges of its near leaves, self included (the self
+// term is zero: d = 0).  Output per particle, in one write:
+//   f = m * field(leaf) + m * sum_j m_j d / (|d|^2 + eps^2)^(3/2)
+// FP32 with v_rsq_f32; optional Kahan-compensated accumulation.
+#include "ps_device.h"
+#include "ps_kernels.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <int kMode, bool kKahan>
+__global__ __launch_bounds__(kBlock) void p2p_kernel(
+    size_t n, const float4* __restrict__ pos4, const uint32_t* __restrict__ leaf_of,
+    const float4* __restrict__ loc, const uint32_t* __restrict__ p2p_off,
+    const uint32_t* __restrict__ p2p_len, const uint2* __restrict__ p2p_src,
+    float* __restrict__ fx, float* __restrict__ fy, float* __restrict__ fz,
+    unsigned long long* pair_count) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    uint32_t pairs = 0;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t lf = 0;
+    if (valid) {
+        lf = leaf_of[i];
+        p = pos4[i];
+        if (kMode != 1) {
+            const uint32_t o = p2p_off[lf];
+            const uint32_t L = p2p_len[lf];
+            for (uint32_t k = 0; k < L; ++k) {
+                const uint2 r = p2p_src[o + k];
+                pairs += r.y - r.x;
+                for (uint32_t j = r.x; j < r.y; ++j) {
+                    const float4 q = pos4[j];
+                    if (kKahan) {
+                        float tx = 0.f, ty = 0.f, tz = 0.f;
+                        ps_pair(p, q, tx, ty, tz);
+                        ps_kahan_add(ax, cx, tx);
+                        ps_kahan_add(ay, cy, ty);
+                        ps_kahan_add(az, cz, tz);
+                    } else {
+                        ps_pair(p, q, ax, ay, az);
+                    }
+                }
+            }
+        }
+    }
+    if (pair_count) {
+        unsigned long long v = pairs;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(pair_count, v);
+    }
+    if (!valid) return;
+    {
+#pragma clang fp contract(off)
+        if (kMode == 0) {  // L2P (first write, H:1230-1232) then the P2P sum
+            const float4 fl = loc[lf];
+            fx[i] = p.w * fl.y + p.w * ax;
+            fy[i] = p.w * fl.z + p.w * ay;
+            fz[i] = p.w * fl.w + p.w * az;
+        } else if (kMode == 1) {
+            const float4 fl = loc[lf];
+            fx[i] = p.w * fl.y;
+            fy[i] = p.w * fl.z;
+            fz[i] = p.w * fl.w;
+        } else {
+            fx[i] = fx[i] + p.w * ax;
+            fy[i] = fy[i] + p.w * ay;
+            fz[i] = fz[i] + p.w * az;
+        }
+    }
+}
+
+template <int kMode>
+void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
+                 const uint32_t* p2p_off, const uint32_t* p2p_len, const uint2* p2p_src,
+                 float* fx, float* fy, float* fz, int kahan, unsigned long long* pc,
+                 hipStream_t s) {
+    unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    if (kahan)
+        p2p_kernel<kMode, true><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
+                                                          p2p_len, p2p_src, fx, fy, fz, pc);
+    else
+        p2p_kernel<kMode, false><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
+                                                           p2p_len, p2p_src, fx, fy, fz, pc);
+}
+
+}  // namespace
+
+void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
+                   const uint32_t* p2p_off, const uint32_t* p2p_len, const uint2* p2p_src,
+                   float* fx, float* fy, float* fz, int mode, int kahan,
+                   unsigned long long* pair_count, hipStream_t s) {
+    if (n == 0) return;
+    if (mode == 0)
+        launch_mode<0>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_src, fx, fy, fz, kahan,
+                       pair_count, s);
+    else if (mode == 1)
+        launch_mode<1>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_src, fx, fy, fz, kahan,
+                       pair_count, s);
+    else
+        launch_mode<2>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_src, fx, fy, fz, kahan,
+                       pair_count, s);
+}

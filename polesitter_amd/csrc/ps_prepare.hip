@@ -1,0 +1,142 @@
+// ps_prepare.hip -- ps_prepare_particles on the device (H:1911-2170).
+//
+// Two HBM-bound passes: a min/max reduction over x, y and z (one cube, exact,
+// order independent), then Morton quantisation + bit interleave, one thread
+// per particle.  Algorithmic traffic: 12 B read per particle per pass + 4 B
+// of codes written.
+#include <float.h>
+
+#include "ps_device.h"
+#include "ps_kernels.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxParts = 1024;
+
+__device__ __forceinline__ void wave_minmax(float& lo, float& hi) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        lo = fminf(lo, __shfl_down(lo, d, 64));
+        hi = fmaxf(hi, __shfl_down(hi, d, 64));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void bbox_partial_kernel(const float* __restrict__ x,
+                                                              const float* __restrict__ y,
+                                                              const float* __restrict__ z,
+                                                              size_t n, float2* part) {
+    __shared__ float slo[kBlock / 64], shi[kBlock / 64];
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        float a = x[i], b = y[i], c = z[i];
+        lo = fminf(lo, fminf(a, fminf(b, c)));
+        hi = fmaxf(hi, fmaxf(a, fmaxf(b, c)));
+    }
+    wave_minmax(lo, hi);
+    if ((threadIdx.x & 63) == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) {
+            lo = fminf(lo, slo[w]);
+            hi = fmaxf(hi, shi[w]);
+        }
+        part[blockIdx.x] = make_float2(lo, hi);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void bbox_final_kernel(const float2* part, int np,
+                                                            float* prm) {
+    __shared__ float slo[kBlock / 64], shi[kBlock / 64];
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    for (int i = threadIdx.x; i < np; i += kBlock) {
+        lo = fminf(lo, part[i].x);
+        hi = fmaxf(hi, part[i].y);
+    }
+    wave_minmax(lo, hi);
+    if ((threadIdx.x & 63) == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma clang fp contract(off)
+        for (int w = 1; w < kBlock / 64; ++w) {
+            lo = fminf(lo, slo[w]);
+            hi = fmaxf(hi, shi[w]);
+        }
+        float range = hi - lo;  // H:2008-2011
+        if (range < 0.001f) range = 0.001f;
+        prm[0] = lo;
+        prm[1] = hi;
+        prm[2] = range;
+        prm[3] = 1023.0f / range;  // H:2017
+    }
+}
+
+__device__ __forceinline__ uint32_t q_trunc(float v, float lo, float range) {
+#pragma clang fp contract(off)
+    // H:2133-2153: (int)(((v - min) / range) * 1023), clamped to [0, 1023]
+    int q = (int)(((v - lo) / range) * 1023.0f);
+    q = q < 0 ? 0 : q;
+    q = q > 1023 ? 1023 : q;
+    return (uint32_t)q;
+}
+
+__device__ __forceinline__ uint32_t q_rne(float v, float lo, float scale) {
+#pragma clang fp contract(off)
+    // H:2046-2058: (v - min) * fl(1023/range), min(., 1023), max(0, .), then
+    // round-to-nearest-even conversion
+    float t = (v - lo) * scale;
+    t = t < 1023.0f ? t : 1023.0f;
+    t = 0.0f > t ? 0.0f : t;
+    return (uint32_t)(int)rintf(t);
+}
+
+__global__ __launch_bounds__(kBlock) void morton_kernel(const float* __restrict__ x,
+                                                        const float* __restrict__ y,
+                                                        const float* __restrict__ z, size_t n,
+                                                        const float* __restrict__ prm,
+                                                        uint32_t* __restrict__ codes, int mode,
+                                                        size_t n_vec) {
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float lo = prm[0];
+    const float range = prm[2];
+    const float scale = prm[3];
+    float vx = x[i], vy = y[i], vz = z[i];
+    uint32_t qx, qy, qz;
+    if (mode == 1 && i < n_vec) {
+        qx = q_rne(vx, lo, scale);
+        qy = q_rne(vy, lo, scale);
+        qz = q_rne(vz, lo, scale);
+    } else {
+        qx = q_trunc(vx, lo, range);
+        qy = q_trunc(vy, lo, range);
+        qz = q_trunc(vz, lo, range);
+    }
+    codes[i] = ps_morton3(qx, qy, qz);
+}
+
+}  // namespace
+
+void ps_launch_bbox(const float* x, const float* y, const float* z, size_t n, float2* part,
+                    float* prm, hipStream_t s) {
+    size_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > (size_t)kMaxParts) blocks = kMaxParts;
+    if (blocks == 0) blocks = 1;
+    bbox_partial_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(x, y, z, n, part);
+    bbox_final_kernel<<<1, kBlock, 0, s>>>(part, (int)blocks, prm);
+}
+
+void ps_launch_morton(const float* x, const float* y, const float* z, size_t n,
+                      const float* prm, uint32_t* codes, int mode, hipStream_t s) {
+    if (n == 0) return;
+    size_t blocks = (n + kBlock - 1) / kBlock;
+    morton_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(x, y, z, n, prm, codes, mode,
+                                                      n - (n % 8u));
+}

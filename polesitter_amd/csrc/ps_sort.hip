@@ -1,0 +1,200 @@
+// ps_sort.hip -- stable LSD radix sort of the particles by Morton code
+// (replaces ps_impl_sort_particles, H:1588-1693).
+//
+// Keys are 30-bit Morton codes, so three passes of 10-bit digits cover them
+// (the reference uses four 8-bit passes over 32 bits; any stable sort yields
+// the same permutation).  Each pass:
+//   upsweep   per-tile digit histogram in LDS -> table[digit][tile]
+//   scan      exclusive scan of the digit-major table: the start of every
+//             (digit, tile) run in the output -- bucket-major, tile-minor,
+//             the same offset order as H:1661-1667
+//   downsweep the tile is ranked stably inside LDS (wave-wide match of equal
+//             digits with 10 ballots, per-wave counts, in-order rounds), then
+//             written out digit run by digit run, so consecutive lanes store
+//             to consecutive addresses; the payload (id, x, y, z, mass) is
+//             gathered from the tile's own input range, which is L2-resident.
+// Tiles are 4096 keys (256 threads x 16).  Ping-pong: io -> t0 -> t1 -> io,
+// so the sorted data ends in the caller's arrays (H:1688-1690).
+#include "ps_device.h"
+#include "ps_kernels.h"
+
+namespace {
+
+constexpr int kRadixBits = 10;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;
+constexpr int kWaves = kBlock / 64;
+
+__global__ __launch_bounds__(kBlock) void sort_upsweep_kernel(const uint32_t* __restrict__ key,
+                                                              size_t n, int shift,
+                                                              uint32_t* __restrict__ table,
+                                                              uint32_t n_tiles) {
+    __shared__ uint32_t h[kRadix];
+    for (int d = threadIdx.x; d < kRadix; d += kBlock) h[d] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kTile;
+#pragma unroll 4
+    for (int r = 0; r < kItems; ++r) {
+        size_t i = base + (size_t)r * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(key[i] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kRadix; d += kBlock)
+        table[(size_t)d * n_tiles + blockIdx.x] = h[d];
+}
+
+__global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ iin,
+    const float* __restrict__ xin, const float* __restrict__ yin, const float* __restrict__ zin,
+    const float* __restrict__ min_, uint32_t* __restrict__ kout, uint32_t* __restrict__ iout,
+    float* __restrict__ xout, float* __restrict__ yout, float* __restrict__ zout,
+    float* __restrict__ mout, float4* __restrict__ pos4, size_t n, int shift,
+    const uint32_t* __restrict__ offs, uint32_t n_tiles) {
+    __shared__ uint32_t keys[kTile];
+    __shared__ uint16_t sidx[kTile];
+    __shared__ uint32_t tstart[kRadix];
+    __shared__ uint32_t goff[kRadix];
+    __shared__ uint32_t run[kRadix];
+    __shared__ uint16_t wcnt[kWaves][kRadix];
+    __shared__ uint32_t wsum[kWaves];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const size_t base = (size_t)blockIdx.x * kTile;
+    const uint32_t valid_n = (uint32_t)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+
+    for (int d = tid; d < kRadix; d += kBlock) {
+        tstart[d] = 0;
+        run[d] = 0;
+        goff[d] = offs[(size_t)d * n_tiles + blockIdx.x];
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) wcnt[w][d] = 0;
+    }
+    __syncthreads();
+
+    // 1. load the tile, count digits
+#pragma unroll 4
+    for (int r = 0; r < kItems; ++r) {
+        uint32_t idx = (uint32_t)(r * kBlock + tid);
+        uint32_t k = 0xFFFFFFFFu;
+        if (idx < valid_n) {
+            k = kin[base + idx];
+            atomicAdd(&tstart[(k >> shift) & (kRadix - 1)], 1u);
+        }
+        keys[idx] = k;
+    }
+    __syncthreads();
+
+    // 2. exclusive scan of the tile histogram (4 digits per thread)
+    {
+        uint32_t a0 = tstart[4 * tid + 0], a1 = tstart[4 * tid + 1];
+        uint32_t a2 = tstart[4 * tid + 2], a3 = tstart[4 * tid + 3];
+        uint32_t s = a0 + a1 + a2 + a3;
+        uint32_t inc = s;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        uint32_t ex = inc - s;
+        for (int w = 0; w < wave; ++w) ex += wsum[w];
+        tstart[4 * tid + 0] = ex;
+        tstart[4 * tid + 1] = ex + a0;
+        tstart[4 * tid + 2] = ex + a0 + a1;
+        tstart[4 * tid + 3] = ex + a0 + a1 + a2;
+    }
+    __syncthreads();
+
+    // 3. stable ranking, in original order: round r covers keys
+    //    [r*256, r*256+256), lanes in order inside a wave, waves in order
+    const uint64_t lt = ps_lanemask_lt();
+    for (int r = 0; r < kItems; ++r) {
+        if ((uint32_t)(r * kBlock) >= valid_n) break;  // block-uniform
+        uint32_t idx = (uint32_t)(r * kBlock + tid);
+        bool valid = idx < valid_n;
+        uint32_t d = (keys[idx] >> shift) & (kRadix - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(valid && bit);
+            peers &= bit ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        uint32_t cnt = (uint32_t)__popcll(peers);
+        bool leader = valid && rank == 0;
+        if (leader) wcnt[wave][d] = (uint16_t)cnt;
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = tstart[d] + run[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            sidx[pos] = (uint16_t)idx;
+        }
+        __syncthreads();
+        if (leader) {
+            atomicAdd(&run[d], cnt);
+            wcnt[wave][d] = 0;
+        }
+        __syncthreads();
+    }
+
+    // 4. write out digit runs: position p of the tile goes to
+    //    goff[d] + (p - tstart[d])
+#pragma unroll 4
+    for (int r = 0; r < kItems; ++r) {
+        uint32_t p = (uint32_t)(r * kBlock + tid);
+        if (p >= valid_n) break;
+        uint32_t sidx_p = sidx[p];
+        uint32_t k = keys[sidx_p];
+        uint32_t d = (k >> shift) & (kRadix - 1);
+        size_t g = (size_t)goff[d] + (p - tstart[d]);
+        size_t src = base + sidx_p;
+        float vx = xin[src], vy = yin[src], vz = zin[src], vm = min_[src];
+        kout[g] = k;
+        iout[g] = iin[src];
+        xout[g] = vx;
+        yout[g] = vy;
+        zout[g] = vz;
+        mout[g] = vm;
+        if (pos4) pos4[g] = make_float4(vx, vy, vz, vm);
+    }
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+size_t ps_sort_tmp_bytes(size_t n) {
+    size_t tiles = (n + kTile - 1) / kTile;
+    size_t table = (size_t)kRadix * (tiles ? tiles : 1);
+    return align256(table * sizeof(uint32_t)) + ps_scan_tmp_bytes_u32(table);
+}
+
+hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
+                         size_t n, void* tmp, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
+    const size_t table_n = (size_t)kRadix * tiles;
+    uint32_t* table = reinterpret_cast<uint32_t*>(tmp);
+    void* scan_tmp = reinterpret_cast<char*>(tmp) + align256(table_n * sizeof(uint32_t));
+    const PsSortArrays* src[3] = {&io, &t0, &t1};
+    const PsSortArrays* dst[3] = {&t0, &t1, &io};
+    for (int pass = 0; pass < 3; ++pass) {
+        const int shift = pass * kRadixBits;
+        const PsSortArrays& a = *src[pass];
+        const PsSortArrays& b = *dst[pass];
+        sort_upsweep_kernel<<<tiles, kBlock, 0, s>>>(a.key, n, shift, table, tiles);
+        hipError_t e = ps_scan_u32(table, table, table_n, nullptr, scan_tmp, s);
+        if (e != hipSuccess) return e;
+        sort_downsweep_kernel<<<tiles, kBlock, 0, s>>>(a.key, a.id, a.x, a.y, a.z, a.m, b.key,
+                                                       b.id, b.x, b.y, b.z, b.m,
+                                                       pass == 2 ? pos4 : nullptr, n, shift,
+                                                       table, tiles);
+    }
+    return hipGetLastError();
+}

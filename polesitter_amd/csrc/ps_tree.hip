@@ -1,0 +1,262 @@
+// ps_tree.hip -- octree build from sorted Morton codes, upward pass (P2M /
+// M2M) and L2L, replacing H:882-967, H:969-1046 and H:1238-1253.
+//
+// Build.  The reference builds top-down by binary search (H:886-967): a node
+// at depth d+1 exists for every non-empty octant of a node at depth d that
+// holds >= 2 particles, d < 10; a node is a leaf at depth 10 or with <= 1
+// particle.  Equivalently, with k(i, j) = the number of leading octal digits
+// (3-bit groups from bit 29 down) that codes i and j share:
+//   D_i      = min(10, max(k(i-1,i), k(i,i+1)) + 1)  depth of i's leaf
+//   start_d  = d <= D_i and d > k(i-1,i)            i opens a node at depth d
+//   end_d    = d <= D_i and d > k(i,i+1)            i closes a node at depth d
+// so every particle knows locally, from its two neighbours, at which depths
+// it starts or ends a node.  One scan of the per-(depth, block) start counts,
+// flattened depth-major, then gives each node its index in canonical level
+// order directly.  Centres follow the reference's float recurrence
+// c_child = c_parent +/- fl(hw_parent * 0.5) (H:935-939) exactly.
+//
+// Upward and L2L run one launch per depth (bottom-up / top-down) and keep the
+// reference's summation order (particles in index order, children in slot
+// order), so multipoles are bit-identical to the reference's.
+#include "ps_device.h"
+#include "ps_kernels.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kDepths = PS_MAX_DEPTH + 1;
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+__device__ __forceinline__ int octal_lcp(uint32_t a, uint32_t b) {
+    uint32_t x = (a ^ b) & 0x3FFFFFFFu;
+    if (x == 0) return PS_MAX_DEPTH;
+    int hb = 31 - __clz(x);  // highest differing bit, 0..29
+    return (29 - hb) / 3;
+}
+
+struct PartDepths {
+    int leaf;      // D_i
+    int start_lo;  // starts at depths [start_lo, leaf]
+    int end_lo;    // ends at depths [end_lo, leaf]
+};
+
+__device__ __forceinline__ PartDepths part_depths(const uint32_t* codes, size_t n, size_t i,
+                                                  uint32_t c) {
+    int kp = (i > 0) ? octal_lcp(codes[i - 1], c) : -1;
+    int kn = (i + 1 < n) ? octal_lcp(c, codes[i + 1]) : -1;
+    int K = kp > kn ? kp : kn;
+    PartDepths r;
+    r.leaf = (K + 1) < PS_MAX_DEPTH ? (K + 1) : PS_MAX_DEPTH;
+    r.start_lo = kp + 1;
+    r.end_lo = kn + 1;
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void tree_count_kernel(const uint32_t* __restrict__ codes,
+                                                            size_t n, uint32_t* __restrict__ cnt,
+                                                            uint32_t nb, uint32_t* flags) {
+    __shared__ uint32_t ws[kWaves][kDepths];
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    PartDepths pd = {-1, kDepths, kDepths};
+    if (valid) {
+        uint32_t c = codes[i];
+        if (c >> 30) atomicOr(flags, 1u);
+        pd = part_depths(codes, n, i, c);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < kDepths; ++d) {
+        uint64_t b = __ballot(valid && d >= pd.start_lo && d <= pd.leaf);
+        if (lane == 0) ws[wave][d] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (threadIdx.x < kDepths) {
+        uint32_t s = 0;
+        for (int w = 0; w < kWaves; ++w) s += ws[w][threadIdx.x];
+        cnt[(size_t)threadIdx.x * nb + blockIdx.x] = s;
+    }
+}
+
+__global__ void tree_levels_kernel(const uint32_t* off, uint32_t nb,
+                                   const unsigned long long* total, uint32_t* level_start) {
+    int d = threadIdx.x;
+    if (d < kDepths) level_start[d] = off[(size_t)d * nb];
+    if (d == kDepths) level_start[kDepths] = (uint32_t)total[0];
+}
+
+__global__ __launch_bounds__(kBlock) void tree_write_kernel(
+    const uint32_t* __restrict__ codes, size_t n, const uint32_t* __restrict__ off, uint32_t nb,
+    float4 root, float4* __restrict__ geom, uint2* __restrict__ link,
+    uint32_t* __restrict__ nend, uint32_t* __restrict__ parent, uint32_t* __restrict__ leaf_of) {
+#pragma clang fp contract(off)
+    __shared__ uint32_t ws[kWaves][kDepths];
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t c = 0;
+    PartDepths pd = {-1, kDepths, kDepths};
+    if (valid) {
+        c = codes[i];
+        pd = part_depths(codes, n, i, c);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t le = ps_lanemask_lt() | (1ull << lane);
+    uint32_t inc[kDepths];  // this lane's inclusive start count per depth
+#pragma unroll
+    for (int d = 0; d < kDepths; ++d) {
+        uint64_t b = __ballot(valid && d >= pd.start_lo && d <= pd.leaf);
+        if (lane == 0) ws[wave][d] = (uint32_t)__popcll(b);
+        inc[d] = (uint32_t)__popcll(b & le);
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    uint32_t node[kDepths];
+#pragma unroll
+    for (int d = 0; d < kDepths; ++d) {
+        uint32_t p = off[(size_t)d * nb + blockIdx.x] + inc[d];
+        for (int w = 0; w < wave; ++w) p += ws[w][d];
+        node[d] = p - 1u;  // the depth-d node that contains particle i
+    }
+
+    uint32_t* link32 = reinterpret_cast<uint32_t*>(link);
+    float4 g = root;
+#pragma unroll
+    for (int d = 0; d < kDepths; ++d) {
+        if (d > pd.leaf) continue;
+        uint32_t oct = 0;
+        if (d > 0) {
+            oct = (c >> (30 - 3 * d)) & 7u;
+            float hw = g.w * 0.5f;
+            g.x = g.x + ((oct & 1u) ? hw : -hw);
+            g.y = g.y + ((oct & 2u) ? hw : -hw);
+            g.z = g.z + ((oct & 4u) ? hw : -hw);
+            g.w = hw;
+        }
+        if (d >= pd.start_lo) {  // i opens this node
+            const uint32_t nd = node[d];
+            geom[nd] = g;
+            parent[nd] = d > 0 ? node[d > 0 ? d - 1 : 0] : PS_NONE;
+            if (d == pd.leaf) {
+                link[nd] = make_uint2((uint32_t)i, PS_LEAF);
+            } else {
+                link32[2 * (size_t)nd] = node[d + 1 < kDepths ? d + 1 : d];
+            }
+            if (d > 0) atomicOr(&link32[2 * (size_t)node[d - 1] + 1], 1u << oct);
+        }
+        if (d >= pd.end_lo) nend[node[d]] = (uint32_t)i + 1u;
+    }
+    uint32_t lf = node[0];
+#pragma unroll
+    for (int d = 1; d < kDepths; ++d)
+        if (d <= pd.leaf) lf = node[d];
+    leaf_of[i] = lf;
+}
+
+__global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, uint32_t end,
+                                                              const float4* __restrict__ pos4,
+                                                              const float4* __restrict__ geom,
+                                                              const uint2* __restrict__ link,
+                                                              const uint32_t* __restrict__ nend,
+                                                              float4* __restrict__ mp) {
+#pragma clang fp contract(off)
+    const uint32_t nidx = begin + blockIdx.x * kBlock + threadIdx.x;
+    if (nidx >= end) return;
+    const float4 g = geom[nidx];
+    const uint2 l = link[nidx];
+    float m0 = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+    if (l.y & PS_LEAF) {  // P2M about the cell centre, H:978-1004
+        const uint32_t e = nend[nidx];
+        for (uint32_t i = l.x; i < e; ++i) {
+            float4 p = pos4[i];
+            float dx = p.x - g.x;
+            float dy = p.y - g.y;
+            float dz = p.z - g.z;
+            m0 += p.w;
+            mx += p.w * dx;
+            my += p.w * dy;
+            mz += p.w * dz;
+        }
+    } else {  // M2M in slot order, H:1013-1039
+        uint32_t ch = l.x;
+        for (uint32_t s = 0; s < 8; ++s) {
+            if (!((l.y >> s) & 1u)) continue;
+            float4 cg = geom[ch];
+            float4 cm = mp[ch];
+            ++ch;
+            float dx = cg.x - g.x;
+            float dy = cg.y - g.y;
+            float dz = cg.z - g.z;
+            m0 += cm.x;
+            mx += cm.y + cm.x * dx;
+            my += cm.z + cm.x * dy;
+            mz += cm.w + cm.x * dz;
+        }
+    }
+    mp[nidx] = make_float4(m0, mx, my, mz);
+}
+
+__global__ __launch_bounds__(kBlock) void l2l_level_kernel(uint32_t begin, uint32_t end,
+                                                           const uint32_t* __restrict__ parent,
+                                                           float4* __restrict__ loc) {
+    const uint32_t nidx = begin + blockIdx.x * kBlock + threadIdx.x;
+    if (nidx >= end) return;
+    float4 pl = loc[parent[nidx]];
+    float4 cl = loc[nidx];
+    cl.y = cl.y + pl.y;  // child_local += parent_local, H:1248-1250
+    cl.z = cl.z + pl.z;
+    cl.w = cl.w + pl.w;
+    loc[nidx] = cl;
+}
+
+}  // namespace
+
+size_t ps_tree_tmp_bytes(size_t n) {
+    size_t nb = (n + kBlock - 1) / kBlock;
+    if (nb == 0) nb = 1;
+    size_t t = (size_t)kDepths * nb;
+    return align256(t * sizeof(uint32_t)) + ps_scan_tmp_bytes_u32(t);
+}
+
+hipError_t ps_tree_count(const uint32_t* codes, size_t n, uint32_t* level_start,
+                         uint32_t* flags, unsigned long long* total64, void* tmp,
+                         hipStream_t s) {
+    const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(tmp);
+    const size_t t = (size_t)kDepths * nb;
+    void* scan_tmp = reinterpret_cast<char*>(tmp) + align256(t * sizeof(uint32_t));
+    ps_fill_u32(flags, 0u, 1, s);
+    tree_count_kernel<<<nb, kBlock, 0, s>>>(codes, n, cnt, nb, flags);
+    hipError_t e = ps_scan_u32(cnt, cnt, t, total64, scan_tmp, s);
+    if (e != hipSuccess) return e;
+    tree_levels_kernel<<<1, 64, 0, s>>>(cnt, nb, total64, level_start);
+    return hipGetLastError();
+}
+
+hipError_t ps_tree_write(const uint32_t* codes, size_t n, float4 root, void* tmp,
+                         float4* geom, uint2* link, uint32_t* nend, uint32_t* parent,
+                         uint32_t* leaf_of, size_t n_nodes, hipStream_t s) {
+    const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
+    const uint32_t* off = reinterpret_cast<const uint32_t*>(tmp);
+    ps_fill_u32(reinterpret_cast<uint32_t*>(link), 0u, 2 * n_nodes, s);
+    tree_write_kernel<<<nb, kBlock, 0, s>>>(codes, n, off, nb, root, geom, link, nend, parent,
+                                            leaf_of);
+    return hipGetLastError();
+}
+
+void ps_launch_upward_level(uint32_t begin, uint32_t end, const float4* pos4,
+                            const float4* geom, const uint2* link, const uint32_t* nend,
+                            float4* mp, hipStream_t s) {
+    if (end <= begin) return;
+    uint32_t blocks = (end - begin + kBlock - 1) / kBlock;
+    upward_level_kernel<<<blocks, kBlock, 0, s>>>(begin, end, pos4, geom, link, nend, mp);
+}
+
+void ps_launch_l2l_level(uint32_t begin, uint32_t end, const uint32_t* parent, float4* loc,
+                         hipStream_t s) {
+    if (end <= begin) return;
+    uint32_t blocks = (end - begin + kBlock - 1) / kBlock;
+    l2l_level_kernel<<<blocks, kBlock, 0, s>>>(begin, end, parent, loc);
+}

@@ -1,0 +1,83 @@
+"""The C-ABI boundary without a GPU: both libraries load, export every
+function that include/*.h declares, and the ctypes mirrors of the
+reference's structs have the reference's layout (src/polesitter.h:304-322)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+from polesitter_amd import api
+
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def declared(header):
+    text = open(os.path.join(INCLUDE, header)).read()
+    text = text.split("#ifdef POLESITTER_IMPLEMENTATION")[0]
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:int|void\s*\*|ps_result_t|struct\s+ps_hip_engine\s*\*)\s*"
+                       r"(\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_hip_library_exports_everything_declared():
+    L = api.hip_lib()
+    names = declared("polesitter_hip.h")
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_host_library_exports_the_reference_api():
+    L = api.host_lib()
+    names = declared("polesitter.h")
+    for ref in ("ps_init", "ps_calc_forces", "ps_prepare_particles", "ps_destroy"):
+        assert ref in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_layout_matches_reference():
+    # field order of H:304-310 and H:312-322 (LP64)
+    assert [f[0] for f in api.PsConfig._fields_] == ["buff", "buff_size", "max_particles",
+                                                     "theta", "thrd_cnt"]
+    assert C.sizeof(api.PsConfig) == 40 and api.PsConfig.theta.offset == 24
+    assert [f[0] for f in api.PsParticleArrs._fields_] == ["x", "y", "z", "mass", "fx", "fy",
+                                                           "fz", "id", "cnt"]
+    assert C.sizeof(api.PsParticleArrs) == 72
+
+
+def test_header_compiles_as_c99_with_positional_initialisers(tmp_path):
+    # the reference's callers initialise ps_config_t positionally
+    # (tests/main.c:82): the drop-in header must accept that unchanged
+    src = tmp_path / "t.c"
+    src.write_text('#define PS_MULTITHREADING\n#define POLESITTER_IMPLEMENTATION\n'
+                   '#include "polesitter.h"\n'
+                   'int main(void){ static char b[1<<20]; ps_context_t* c = 0;\n'
+                   ' ps_config_t cfg = {b, sizeof b, 4, 3.4641F, 4};\n'
+                   ' ps_particle_arrs_t a = {0,0,0,0,0,0,0,0,0};\n'
+                   ' (void)a; return ps_init(0, &cfg) == PS_EINVAL && c == 0 ? 0 : 1; }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                        f"-I{INCLUDE}", "-c", str(src), "-o", str(tmp_path / "t.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_invalid_arguments_without_device():
+    # argument checks that return before any device work, as in H:1700,1843,1914
+    L = api.host_lib()
+    assert L.ps_init(None, None) == api.PS_EINVAL
+    rc, ctx = api.ps_init(16, 4)  # smaller than the context itself
+    assert rc == api.PS_EINVAL and ctx is None
+    assert api.ps_destroy(None) == api.PS_EINVAL
+    arrs = api.make_arrs(None, None, None, None, None, None, None, None, 0)
+    rc, *_ = api.ps_prepare_particles(arrs, None)
+    assert rc == api.PS_EINVAL
+    assert L.ps_calc_forces(None, None, None, 0.0, 0.0, 0.0, 1.0) == api.PS_EINVAL
+
+
+def test_dropin_program_is_built():
+    assert os.access(os.path.join(api.LIBDIR, "dropin_tick"), os.X_OK)

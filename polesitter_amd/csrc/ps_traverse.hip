@@ -110,21 +110,30 @@ __global__ __launch_bounds__(kBlock) void trav_write_kernel(
     const float4 tg = t.geom[tn];
     const uint2 tl = t.link[tn];
     const bool final_p2p = c.z > 0;
+    // the MAC is re-evaluated with the same code as the count pass, so the
+    // outputs fill exactly the counted ranges; the bounds below only make
+    // that an invariant of the writes
     uint32_t w = p.x;
+    const uint32_t w_end = p.x + c.x;
     uint32_t q = p2p_base + p.z;
+    const uint32_t q_end = q + c.z;
     for (uint32_t k = 0; k < L; ++k) {
         const uint32_t s = list[o + k];
         const float4 sg = t.geom[s];
         if (ps_far(tg, sg, theta2)) continue;
         const uint2 sl = t.link[s];
         if (sl.y & PS_LEAF) {
-            if (final_p2p)
-                t.p2p_src[q++] = make_uint2(sl.x, t.nend[s]);
-            else
-                nlist[w++] = s;
+            if (final_p2p) {
+                if (q < q_end) t.p2p_src[q] = make_uint2(sl.x, t.nend[s]);
+                ++q;
+            } else {
+                if (w < w_end) nlist[w] = s;
+                ++w;
+            }
         } else {
             const uint32_t nc = (uint32_t)__popc(sl.y & 0xFFu);
-            for (uint32_t r = 0; r < nc; ++r) nlist[w++] = sl.x + r;
+            for (uint32_t r = 0; r < nc; ++r, ++w)
+                if (w < w_end) nlist[w] = sl.x + r;
         }
     }
     if (c.y) {

@@ -80,6 +80,26 @@ hipError_t grow(DevBuf& b, size_t need, hipStream_t s) {
     return hipSuccess;
 }
 
+// grow keeping the first `keep` bytes: for buffers that accumulate across
+// launches (the P2P source CSR fills over several traversal iterations)
+hipError_t grow_keep(DevBuf& b, size_t need, size_t keep, hipStream_t s) {
+    if (need <= b.bytes && b.p) return hipSuccess;
+    DevBuf nb;
+    hipError_t e = grow(nb, need, s);
+    if (e != hipSuccess) return e;
+    if (b.p && keep) {
+        e = hipMemcpyAsync(nb.p, b.p, std::min(keep, b.bytes), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            (void)hipFree(nb.p);
+            return e;
+        }
+    }
+    if (b.p) (void)hipFree(b.p);
+    b = nb;
+    return hipSuccess;
+}
+
 void release(DevBuf& b) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
@@ -297,6 +317,7 @@ int stage_traverse(ps_hip_engine* e) {
                             u32(e->ent[c0][2]), e->s);
 
     PsTravTree t;
+    t.n_nodes = e->n_nodes;
     t.geom = e->geom();
     t.mp = e->mp();
     t.link = e->link();
@@ -334,7 +355,8 @@ int stage_traverse(ps_hip_engine* e) {
         PS_CK(grow(e->lst[nxt], 4 * std::max<unsigned long long>(list_n, 1), e->s));
         for (int k = 0; k < 3; ++k)
             PS_CK(grow(e->ent[nxt][k], 4 * std::max<unsigned long long>(ent_n, 1), e->s));
-        PS_CK(grow(e->p2p_src_b, 8 * std::max<unsigned long long>(p2p_base + p2p_n, 1), e->s));
+        PS_CK(grow_keep(e->p2p_src_b, 8 * std::max<unsigned long long>(p2p_base + p2p_n, 1),
+                        8 * p2p_base, e->s));
         t.p2p_src = e->p2p_src();
         e->record(10);
         ps_launch_trav_write(t, theta2, u32(e->ent[cur][0]), u32(e->ent[cur][1]),

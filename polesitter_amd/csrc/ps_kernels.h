@@ -63,6 +63,7 @@ void ps_launch_l2l_level(uint32_t begin, uint32_t end, const uint32_t* parent, f
 
 // ---- ps_traverse.hip: dual-tree traversal, M2L, P2P lists -------------------
 struct PsTravTree {
+    uint32_t n_nodes;  // list entries are checked against it
     const float4* geom;
     const float4* mp;
     const uint2* link;

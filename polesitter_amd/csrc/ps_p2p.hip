@@ -38,8 +38,9 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
             const uint32_t L = p2p_len[lf];
             for (uint32_t k = 0; k < L; ++k) {
                 const uint2 r = p2p_src[o + k];
+                const uint32_t j_end = r.y < (uint32_t)n ? r.y : (uint32_t)n;
                 pairs += r.y - r.x;
-                for (uint32_t j = r.x; j < r.y; ++j) {
+                for (uint32_t j = r.x; j < j_end; ++j) {
                     const float4 q = pos4[j];
                     if (kKahan) {
                         float tx = 0.f, ty = 0.f, tz = 0.f;

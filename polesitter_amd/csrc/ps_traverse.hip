@@ -56,6 +56,7 @@ __global__ __launch_bounds__(kBlock) void trav_count_kernel(
     uint32_t nfar = 0, nleaf = 0, nchild = 0;
     for (uint32_t k = 0; k < L; ++k) {
         const uint32_t s = list[o + k];
+        if (s >= t.n_nodes) continue;  // never taken when counts and writes agree
         const float4 sg = t.geom[s];
         if (ps_far(tg, sg, theta2)) {
             if (kKahan) {
@@ -119,6 +120,7 @@ __global__ __launch_bounds__(kBlock) void trav_write_kernel(
     const uint32_t q_end = q + c.z;
     for (uint32_t k = 0; k < L; ++k) {
         const uint32_t s = list[o + k];
+        if (s >= t.n_nodes) continue;
         const float4 sg = t.geom[s];
         if (ps_far(tg, sg, theta2)) continue;
         const uint2 sl = t.link[s];

@@ -36,10 +36,12 @@ def test_morton_exhaustive_2_30():
 
 
 def op_m2l(t, s, mp, theta):
-    f = lambda v: np.asarray(v, np.float32)
+    # keep every array alive across the call: a temporary's .ctypes.data
+    # dangles as soon as the temporary is released
+    tg, sg, smp = (np.ascontiguousarray(v, np.float32) for v in (t, s, mp))
     out = np.zeros(4, np.float32)
-    api.check("op_m2l", api.hip_lib().ps_hip_op_m2l(f(t).ctypes.data, f(s).ctypes.data,
-                                                   f(mp).ctypes.data, theta, out.ctypes.data))
+    api.check("op_m2l", api.hip_lib().ps_hip_op_m2l(tg.ctypes.data, sg.ctypes.data,
+                                                   smp.ctypes.data, theta, out.ctypes.data))
     return out
 
 
@@ -165,8 +167,8 @@ def test_fmm_downward_l2l_chain():
     st = Staged([0, 0], [0, 0], [0, 0], [2.5, 1.0], [enc(1023, 1023, 1023)] * 2,
                 (0.0, 0.0, 0.0, 10.0))
     st.stage(2)
-    api.check("set_local", api.hip_lib().ps_hip_debug_set_local(
-        st.e, 0, np.asarray([5.0, -3.5, 42.0], np.float32).ctypes.data))
+    f3 = np.asarray([5.0, -3.5, 42.0], np.float32)
+    api.check("set_local", api.hip_lib().ps_hip_debug_set_local(st.e, 0, f3.ctypes.data))
     st.stage(5)
     t = st.tree()
     leaf = walk(t, 7)

@@ -57,30 +57,84 @@ __device__ __forceinline__ bool ps_far(float4 t, float4 s, float theta2) {
 
 // ---------------------------------------------------------------------------
 // M2L, H:1065-1092: first-order (monopole + dipole) softened field of the
-// source cell at the target centre, in the reference's operation order.
+// source cell at the target centre,
+//   E += m0 d / r^3 + p / r^3 - 3 (p.d) d / r^5,   r^2 = |d|^2 + eps^2,
+// with d = source - target and d2 = |d|^2 exactly as the MAC computed it.
+// The reference's sqrtf and two divisions become one v_rsq_f32 (1 ulp) and
+// products: the field is compared within a tolerance, never bit for bit.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void ps_m2l_d(float dx, float dy, float dz, float d2, float4 smp,
+                                         float& ex, float& ey, float& ez) {
+    const float ir = __builtin_amdgcn_rsqf(d2 + PS_SOFT2);
+    const float ir2 = ir * ir;
+    const float ir3 = ir2 * ir;
+    const float ir5 = ir3 * ir2;
+    const float mdr = smp.y * dx + smp.z * dy + smp.w * dz;
+    const float a = smp.x * ir3 - 3.0f * mdr * ir5;  // coefficient of d
+    ex += a * dx + smp.y * ir3;
+    ey += a * dy + smp.z * ir3;
+    ez += a * dz + smp.w * ir3;
+}
+
 __device__ __forceinline__ void ps_m2l(float4 t, float4 s, float4 smp, float& ex, float& ey,
                                        float& ez) {
+    float dx, dy, dz, d2;
+    {
 #pragma clang fp contract(off)
-    float dx = s.x - t.x;
-    float dy = s.y - t.y;
-    float dz = s.z - t.z;
-    float d2 = (dx * dx + dy * dy) + dz * dz;
-    float soft = d2 + PS_SOFT2;
-    float dist = sqrtf(soft);
-    float inv_r3 = 1.0f / (dist * soft);
-    float inv_r5 = inv_r3 / soft;
-    float fmx = smp.x * dx * inv_r3;
-    float fmy = smp.x * dy * inv_r3;
-    float fmz = smp.x * dz * inv_r3;
-    float mdr = (smp.y * dx + smp.z * dy) + smp.w * dz;
-    float coeff = 3.0f * mdr * inv_r5;
-    float fdx = smp.y * inv_r3 - dx * coeff;
-    float fdy = smp.z * inv_r3 - dy * coeff;
-    float fdz = smp.w * inv_r3 - dz * coeff;
-    ex += fmx + fdx;
-    ey += fmy + fdy;
-    ez += fmz + fdz;
+        dx = s.x - t.x;
+        dy = s.y - t.y;
+        dz = s.z - t.z;
+        d2 = (dx * dx + dy * dy) + dz * dz;
+    }
+    ps_m2l_d(dx, dy, dz, d2, smp, ex, ey, ez);
+}
+
+// MAC that also returns the separation it evaluated (for M2L on far pairs)
+__device__ __forceinline__ bool ps_far_d(float4 t, float4 s, float theta2, float& dx, float& dy,
+                                         float& dz, float& d2) {
+#pragma clang fp contract(off)
+    dx = s.x - t.x;
+    dy = s.y - t.y;
+    dz = s.z - t.z;
+    d2 = (dx * dx + dy * dy) + dz * dz;
+    const float hs = t.w + s.w;
+    return d2 > theta2 * (hs * hs);
+}
+
+// wave-wide sums (64 lanes), result valid in every lane
+__device__ __forceinline__ float ps_wave_sum(float v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t ps_wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// exclusive prefix over the wave; *total gets the wave-wide sum
+__device__ __forceinline__ uint32_t ps_wave_excl_scan(uint32_t v, uint32_t* total) {
+    const int lane = __lane_id();
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    *total = __shfl(inc, 63, 64);
+    return inc - v;
+}
+
+// merge two compensated sums (s, c): value = s - c
+__device__ __forceinline__ void ps_kahan_merge(float& s, float& c, float s2, float c2) {
+#pragma clang fp contract(off)
+    const float t = s + s2;
+    const float bp = t - s;
+    const float err = (s - (t - bp)) + (s2 - bp);  // exact: t + err == s + s2
+    s = t;
+    c = c + c2 - err;
 }
 
 // ---------------------------------------------------------------------------

@@ -15,15 +15,15 @@
 // and every target owns its accumulations: no atomics, deterministic output.
 //
 // One iteration = count (MAC + M2L, per-entry output sizes) -> scan ->
-// write (MAC again, emit next lists / next entries / P2P sources).  One
-// thread per entry; the siblings of a family are adjacent entries reading the
-// same list, so their source loads coincide.
+// write (MAC again, emit next lists / next entries / P2P sources), one
+// wavefront per entry in both kernels.
 #include "ps_device.h"
 #include "ps_kernels.h"
 
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
 
 __global__ __launch_bounds__(kBlock) void trav_init_kernel(uint32_t ls1, uint32_t ls2,
                                                            uint32_t ls3,
@@ -39,13 +39,18 @@ __global__ __launch_bounds__(kBlock) void trav_init_kernel(uint32_t ls1, uint32_
     len[k] = (n >= ls2 || (link[n].y & PS_LEAF)) ? 1u : 0u;
 }
 
+// One wavefront per entry: the 64 lanes take the entry's source list in
+// strides, so list loads are coalesced and the per-source work (MAC, M2L or
+// the near-field bookkeeping) is spread over the wave; sums are reduced
+// across lanes at the end.
 template <bool kKahan>
 __global__ __launch_bounds__(kBlock) void trav_count_kernel(
     PsTravTree t, float theta2, const uint32_t* __restrict__ tgt,
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ len,
     const uint32_t* __restrict__ list, uint32_t n_ent, uint4* __restrict__ cnt) {
-    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= n_ent) return;
+    const uint32_t e = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (e >= n_ent) return;  // uniform per wave
     const uint32_t tn = tgt[e];
     const uint32_t o = off[e];
     const uint32_t L = len[e];
@@ -54,19 +59,20 @@ __global__ __launch_bounds__(kBlock) void trav_count_kernel(
     float ex = 0.f, ey = 0.f, ez = 0.f;
     float cx = 0.f, cy = 0.f, cz = 0.f;
     uint32_t nfar = 0, nleaf = 0, nchild = 0;
-    for (uint32_t k = 0; k < L; ++k) {
+    for (uint32_t k = lane; k < L; k += 64) {
         const uint32_t s = list[o + k];
         if (s >= t.n_nodes) continue;  // never taken when counts and writes agree
         const float4 sg = t.geom[s];
-        if (ps_far(tg, sg, theta2)) {
+        float dx, dy, dz, d2;
+        if (ps_far_d(tg, sg, theta2, dx, dy, dz, d2)) {
             if (kKahan) {
                 float px = 0.f, py = 0.f, pz = 0.f;
-                ps_m2l(tg, sg, t.mp[s], px, py, pz);
+                ps_m2l_d(dx, dy, dz, d2, t.mp[s], px, py, pz);
                 ps_kahan_add(ex, cx, px);
                 ps_kahan_add(ey, cy, py);
                 ps_kahan_add(ez, cz, pz);
             } else {
-                ps_m2l(tg, sg, t.mp[s], ex, ey, ez);
+                ps_m2l_d(dx, dy, dz, d2, t.mp[s], ex, ey, ez);
             }
             ++nfar;
         } else {
@@ -77,6 +83,25 @@ __global__ __launch_bounds__(kBlock) void trav_count_kernel(
                 nchild += (uint32_t)__popc(sl.y & 0xFFu);
         }
     }
+    if (kKahan) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            ps_kahan_merge(ex, cx, __shfl_xor(ex, d, 64), __shfl_xor(cx, d, 64));
+            ps_kahan_merge(ey, cy, __shfl_xor(ey, d, 64), __shfl_xor(cy, d, 64));
+            ps_kahan_merge(ez, cz, __shfl_xor(ez, d, 64), __shfl_xor(cz, d, 64));
+        }
+        ex -= cx;
+        ey -= cy;
+        ez -= cz;
+    } else {
+        ex = ps_wave_sum(ex);
+        ey = ps_wave_sum(ey);
+        ez = ps_wave_sum(ez);
+    }
+    nfar = ps_wave_sum_u32(nfar);
+    nleaf = ps_wave_sum_u32(nleaf);
+    nchild = ps_wave_sum_u32(nchild);
+    if (lane != 0) return;
     if (nfar) {
         float4 l = t.loc[tn];
         l.y += ex;
@@ -94,13 +119,18 @@ __global__ __launch_bounds__(kBlock) void trav_count_kernel(
     cnt[e] = c;
 }
 
+// One wavefront per entry: each chunk of 64 sources re-runs the MAC, the
+// lanes' output counts (1 per near leaf, the child count per near internal
+// node) are prefix-summed across the wave, and the outputs land contiguously
+// in source order.
 __global__ __launch_bounds__(kBlock) void trav_write_kernel(
     PsTravTree t, float theta2, const uint32_t* __restrict__ tgt,
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ len,
     const uint32_t* __restrict__ list, uint32_t n_ent, const uint4* __restrict__ cnt,
     const uint4* __restrict__ pos, uint32_t p2p_base, uint32_t* __restrict__ ntgt,
     uint32_t* __restrict__ noff, uint32_t* __restrict__ nlen, uint32_t* __restrict__ nlist) {
-    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t e = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
     if (e >= n_ent) return;
     const uint4 c = cnt[e];
     if (c.x == 0 && c.z == 0) return;
@@ -112,46 +142,52 @@ __global__ __launch_bounds__(kBlock) void trav_write_kernel(
     const uint2 tl = t.link[tn];
     const bool final_p2p = c.z > 0;
     // the MAC is re-evaluated with the same code as the count pass, so the
-    // outputs fill exactly the counted ranges; the bounds below only make
-    // that an invariant of the writes
-    uint32_t w = p.x;
-    const uint32_t w_end = p.x + c.x;
-    uint32_t q = p2p_base + p.z;
-    const uint32_t q_end = q + c.z;
-    for (uint32_t k = 0; k < L; ++k) {
-        const uint32_t s = list[o + k];
-        if (s >= t.n_nodes) continue;
-        const float4 sg = t.geom[s];
-        if (ps_far(tg, sg, theta2)) continue;
-        const uint2 sl = t.link[s];
-        if (sl.y & PS_LEAF) {
-            if (final_p2p) {
-                if (q < q_end) t.p2p_src[q] = make_uint2(sl.x, t.nend[s]);
-                ++q;
-            } else {
-                if (w < w_end) nlist[w] = s;
-                ++w;
+    // outputs fill exactly the counted range; the bound only makes that an
+    // invariant of the stores
+    uint32_t base = final_p2p ? p2p_base + p.z : p.x;
+    const uint32_t limit = base + (final_p2p ? c.z : c.x);
+    for (uint32_t k0 = 0; k0 < L; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        uint32_t s = 0, nout = 0;
+        uint2 sl = make_uint2(0u, 0u);
+        if (k < L) {
+            s = list[o + k];
+            if (s < t.n_nodes && !ps_far(tg, t.geom[s], theta2)) {
+                sl = t.link[s];
+                nout = (sl.y & PS_LEAF) ? 1u : (uint32_t)__popc(sl.y & 0xFFu);
             }
-        } else {
-            const uint32_t nc = (uint32_t)__popc(sl.y & 0xFFu);
-            for (uint32_t r = 0; r < nc; ++r, ++w)
-                if (w < w_end) nlist[w] = sl.x + r;
         }
+        uint32_t total;
+        const uint32_t w = base + ps_wave_excl_scan(nout, &total);
+        if (nout) {
+            if (sl.y & PS_LEAF) {
+                if (w < limit) {
+                    if (final_p2p)
+                        t.p2p_src[w] = make_uint2(sl.x, t.nend[s]);
+                    else
+                        nlist[w] = s;
+                }
+            } else {
+                for (uint32_t r = 0; r < nout; ++r)
+                    if (w + r < limit) nlist[w + r] = sl.x + r;
+            }
+        }
+        base += total;
     }
     if (c.y) {
         if (tl.y & PS_LEAF) {
-            ntgt[p.y] = tn;
-            noff[p.y] = p.x;
-            nlen[p.y] = c.x;
-        } else {
-            for (uint32_t r = 0; r < c.y; ++r) {
-                ntgt[p.y + r] = tl.x + r;
-                noff[p.y + r] = p.x;
-                nlen[p.y + r] = c.x;
+            if (lane == 0) {
+                ntgt[p.y] = tn;
+                noff[p.y] = p.x;
+                nlen[p.y] = c.x;
             }
+        } else if (lane < c.y) {
+            ntgt[p.y + lane] = tl.x + lane;
+            noff[p.y + lane] = p.x;
+            nlen[p.y + lane] = c.x;
         }
     }
-    if (final_p2p) {
+    if (final_p2p && lane == 0) {
         t.p2p_off[tn] = p2p_base + p.z;
         t.p2p_len[tn] = c.z;
     }
@@ -170,7 +206,7 @@ void ps_launch_trav_count(PsTravTree t, float theta2, const uint32_t* tgt, const
                           const uint32_t* len, const uint32_t* list, uint32_t n_ent,
                           uint4* cnt, int kahan, hipStream_t s) {
     if (n_ent == 0) return;
-    uint32_t blocks = (n_ent + kBlock - 1) / kBlock;
+    uint32_t blocks = (n_ent + kWavesPerBlock - 1) / kWavesPerBlock;
     if (kahan)
         trav_count_kernel<true><<<blocks, kBlock, 0, s>>>(t, theta2, tgt, off, len, list, n_ent,
                                                           cnt);
@@ -185,7 +221,7 @@ void ps_launch_trav_write(PsTravTree t, float theta2, const uint32_t* tgt, const
                           uint32_t* ntgt, uint32_t* noff, uint32_t* nlen, uint32_t* nlist,
                           hipStream_t s) {
     if (n_ent == 0) return;
-    uint32_t blocks = (n_ent + kBlock - 1) / kBlock;
+    uint32_t blocks = (n_ent + kWavesPerBlock - 1) / kWavesPerBlock;
     trav_write_kernel<<<blocks, kBlock, 0, s>>>(t, theta2, tgt, off, len, list, n_ent, cnt, pos,
                                                 p2p_base, ntgt, noff, nlen, nlist);
 }

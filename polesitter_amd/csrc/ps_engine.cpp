@@ -138,13 +138,14 @@ struct ps_hip_engine {
     DevBuf sort_tmp, tree_tmp;
     DevBuf pos4_b, leaf_of_b;
     // nodes
-    DevBuf geom_b, mp_b, loc_b, link_b, nend_b, parent_b, p2p_off_b, p2p_len_b;
-    // traversal
-    DevBuf ent[2][3];
-    DevBuf lst[2];
-    DevBuf cnt_b, pos_b, scan_tmp, p2p_src_b;
+    DevBuf geom_b, mp_b, loc_b, link_b, nend_b, parent_b, p2p_off_b, p2p_len_b, p2p_moff_b;
+    // traversal: entries (tgt, off, len, moff) ping-pong; the list arena and
+    // the far-mask arena fill over the whole tick
+    DevBuf ent[2][4];
+    DevBuf arena_b, masks_b;
+    DevBuf cnt_b, pos_b, scan_tmp;
     // small device / pinned host scalars
-    DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs
+    DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs | u64 leaf pairs
     uint32_t* h_small = nullptr;  // pinned mirror
 
     // arrays of the current tick
@@ -165,6 +166,7 @@ struct ps_hip_engine {
         return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(small_b.p) + 64);
     }
     unsigned long long* pairs() { return totals() + 4; }
+    unsigned long long* leaf_pairs() { return totals() + 5; }
     float4* pos4() { return reinterpret_cast<float4*>(pos4_b.p); }
     uint32_t* leaf_of() { return reinterpret_cast<uint32_t*>(leaf_of_b.p); }
     float4* geom() { return reinterpret_cast<float4*>(geom_b.p); }
@@ -175,7 +177,17 @@ struct ps_hip_engine {
     uint32_t* parent() { return reinterpret_cast<uint32_t*>(parent_b.p); }
     uint32_t* p2p_off() { return reinterpret_cast<uint32_t*>(p2p_off_b.p); }
     uint32_t* p2p_len() { return reinterpret_cast<uint32_t*>(p2p_len_b.p); }
-    uint2* p2p_src() { return reinterpret_cast<uint2*>(p2p_src_b.p); }
+    uint32_t* p2p_moff() { return reinterpret_cast<uint32_t*>(p2p_moff_b.p); }
+    uint32_t* arena() { return reinterpret_cast<uint32_t*>(arena_b.p); }
+    unsigned long long* masks() { return reinterpret_cast<unsigned long long*>(masks_b.p); }
+    PsTravEntries entries(int k) {
+        PsTravEntries r;
+        r.tgt = reinterpret_cast<uint32_t*>(ent[k][0].p);
+        r.off = reinterpret_cast<uint32_t*>(ent[k][1].p);
+        r.len = reinterpret_cast<uint32_t*>(ent[k][2].p);
+        r.moff = reinterpret_cast<uint32_t*>(ent[k][3].p);
+        return r;
+    }
 
     PsSortArrays sort_set(int k) {
         char* base = reinterpret_cast<char*>(sort_t.p) + (size_t)k * 6 * max_p * 4;
@@ -202,8 +214,8 @@ struct ps_hip_engine {
         for (auto& b : stage) release(b);
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
-                         &p2p_len_b, &lst[0],  &lst[1],    &cnt_b,     &pos_b,     &scan_tmp,
-                         &p2p_src_b, &small_b};
+                         &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &cnt_b,     &pos_b,
+                         &scan_tmp,  &small_b};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
             for (auto& b : es) release(b);
@@ -236,6 +248,7 @@ hipError_t alloc_fixed(ps_hip_engine* e) {
     PS_AL(parent_b, 4 * M);
     PS_AL(p2p_off_b, 4 * M);
     PS_AL(p2p_len_b, 4 * M);
+    PS_AL(p2p_moff_b, 4 * M);
     PS_AL(small_b, 256);
 #undef PS_AL
     r = hipHostMalloc(reinterpret_cast<void**>(&e->h_small), 256, hipHostMallocDefault);
@@ -302,19 +315,17 @@ int stage_traverse(ps_hip_engine* e) {
     }
     const bool mt = e->trav == PS_HIP_TRAV_MT && e->n_nodes > 1;
     uint32_t n_ent = mt ? (e->ls[3] - e->ls[1]) : 1u;
-    const int c0 = 0;
-    PS_CK(grow(e->ent[c0][0], 4 * (size_t)n_ent, e->s));
-    PS_CK(grow(e->ent[c0][1], 4 * (size_t)n_ent, e->s));
-    PS_CK(grow(e->ent[c0][2], 4 * (size_t)n_ent, e->s));
-    PS_CK(grow(e->lst[c0], 4, e->s));
-    ps_fill_u32(reinterpret_cast<uint32_t*>(e->lst[c0].p), 0u, 1, e->s);  // {root}
-    auto u32 = [](DevBuf& b) { return reinterpret_cast<uint32_t*>(b.p); };
+    int cur = 0;
+    for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[cur][k], 4 * (size_t)n_ent, e->s));
+    // arena word 0 = the list {root}; one mask word per initial entry
+    unsigned long long arena_used = 1, mask_used = n_ent;
+    PS_CK(grow_keep(e->arena_b, 4 * arena_used, 0, e->s));
+    PS_CK(grow_keep(e->masks_b, 8 * mask_used, 0, e->s));
+    ps_fill_u32(e->arena(), 0u, 1, e->s);
     if (mt)
-        ps_launch_trav_init(e->ls[1], e->ls[3], e->ls[2], e->link(), u32(e->ent[c0][0]),
-                            u32(e->ent[c0][1]), u32(e->ent[c0][2]), e->s);
+        ps_launch_trav_init(e->ls[1], e->ls[3], e->ls[2], e->link(), e->entries(cur), e->s);
     else
-        ps_launch_trav_init(0u, 1u, 0u, e->link(), u32(e->ent[c0][0]), u32(e->ent[c0][1]),
-                            u32(e->ent[c0][2]), e->s);
+        ps_launch_trav_init(0u, 1u, 0u, e->link(), e->entries(cur), e->s);
 
     PsTravTree t;
     t.n_nodes = e->n_nodes;
@@ -325,10 +336,15 @@ int stage_traverse(ps_hip_engine* e) {
     t.loc = e->loc();
     t.p2p_off = e->p2p_off();
     t.p2p_len = e->p2p_len();
+    t.p2p_moff = e->p2p_moff();
+    t.p2p_leaf_pairs = nullptr;
+    if (e->stats_on) {
+        t.p2p_leaf_pairs = e->leaf_pairs();
+        ps_fill_u32(reinterpret_cast<uint32_t*>(t.p2p_leaf_pairs), 0u, 2, e->s);
+    }
 
-    unsigned long long p2p_base = 0, m2l = 0, lists = 0;
+    unsigned long long m2l = 0, lists = 0;
     uint32_t iters = 0;
-    int cur = 0;
     while (n_ent > 0) {
         const int nxt = cur ^ 1;
         PS_CK(grow(e->cnt_b, 16 * (size_t)n_ent, e->s));
@@ -337,8 +353,8 @@ int stage_traverse(ps_hip_engine* e) {
         uint4* cnt = reinterpret_cast<uint4*>(e->cnt_b.p);
         uint4* pos = reinterpret_cast<uint4*>(e->pos_b.p);
         e->record(8);
-        ps_launch_trav_count(t, theta2, u32(e->ent[cur][0]), u32(e->ent[cur][1]),
-                             u32(e->ent[cur][2]), u32(e->lst[cur]), n_ent, cnt, e->kahan, e->s);
+        ps_launch_trav_count(t, theta2, e->entries(cur), e->arena(), e->masks(), n_ent, cnt,
+                             e->kahan, e->s);
         e->record(9);
         PS_CK(ps_scan_u4(cnt, pos, n_ent, e->totals(), e->scan_tmp.p, e->s));
         PS_CK(hipMemcpyAsync(e->h_small, e->totals(), 32, hipMemcpyDeviceToHost, e->s));
@@ -346,25 +362,23 @@ int stage_traverse(ps_hip_engine* e) {
         e->st.ms_trav_count += e->elapsed(8, 9);
         if (iters > 0) e->st.ms_trav_write += e->elapsed(10, 11);
         const unsigned long long* tot = reinterpret_cast<unsigned long long*>(e->h_small);
-        const unsigned long long list_n = tot[0], ent_n = tot[1], p2p_n = tot[2];
+        const unsigned long long list_n = tot[0], ent_n = tot[1], chunk_n = tot[2];
         m2l += tot[3];
         lists += list_n;
-        if (list_n > 0xFFFFFFFFull || ent_n > 0xFFFFFFFFull ||
-            p2p_base + p2p_n > 0xFFFFFFFFull)
+        if (arena_used + list_n > 0xFFFFFFFFull || ent_n > 0xFFFFFFFFull ||
+            mask_used + chunk_n > 0xFFFFFFFFull)
             return PS_HIP_EOOM;
-        PS_CK(grow(e->lst[nxt], 4 * std::max<unsigned long long>(list_n, 1), e->s));
-        for (int k = 0; k < 3; ++k)
+        // the arenas keep what earlier iterations wrote: P2P reads final lists
+        PS_CK(grow_keep(e->arena_b, 4 * (arena_used + list_n), 4 * arena_used, e->s));
+        PS_CK(grow_keep(e->masks_b, 8 * (mask_used + chunk_n), 8 * mask_used, e->s));
+        for (int k = 0; k < 4; ++k)
             PS_CK(grow(e->ent[nxt][k], 4 * std::max<unsigned long long>(ent_n, 1), e->s));
-        PS_CK(grow_keep(e->p2p_src_b, 8 * std::max<unsigned long long>(p2p_base + p2p_n, 1),
-                        8 * p2p_base, e->s));
-        t.p2p_src = e->p2p_src();
         e->record(10);
-        ps_launch_trav_write(t, theta2, u32(e->ent[cur][0]), u32(e->ent[cur][1]),
-                             u32(e->ent[cur][2]), u32(e->lst[cur]), n_ent, cnt, pos,
-                             (uint32_t)p2p_base, u32(e->ent[nxt][0]), u32(e->ent[nxt][1]),
-                             u32(e->ent[nxt][2]), u32(e->lst[nxt]), e->s);
+        ps_launch_trav_write(t, e->entries(cur), e->arena(), e->masks(), n_ent, cnt, pos,
+                             (uint32_t)arena_used, (uint32_t)mask_used, e->entries(nxt), e->s);
         e->record(11);
-        p2p_base += p2p_n;
+        arena_used += list_n;
+        mask_used += chunk_n;
         n_ent = (uint32_t)ent_n;
         cur = nxt;
         ++iters;
@@ -374,10 +388,8 @@ int stage_traverse(ps_hip_engine* e) {
         e->st.ms_trav_write += e->elapsed(10, 11);
     }
     e->st.m2l_pairs = m2l;
-    e->st.p2p_leaf_pairs = p2p_base;
     e->st.list_entries = lists;
     e->st.iterations = iters;
-    if (!e->p2p_src_b.p) PS_CK(grow(e->p2p_src_b, 8, e->s));
     return map_err(hipGetLastError());
 }
 
@@ -393,10 +405,12 @@ int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
         pc = e->pairs();
         ps_fill_u32(reinterpret_cast<uint32_t*>(pc), 0u, 2, e->s);
     }
-    if (!e->p2p_src_b.p) PS_CK(grow(e->p2p_src_b, 8, e->s));
+    if (!e->arena_b.p) PS_CK(grow(e->arena_b, 256, e->s));
+    if (!e->masks_b.p) PS_CK(grow(e->masks_b, 256, e->s));
     if (ev_before >= 0) e->record(ev_before);
     ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
-                  e->p2p_src(), e->fx, e->fy, e->fz, mode, e->kahan, pc, e->s);
+                  e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
+                  e->fz, mode, e->kahan, pc, e->s);
     return map_err(hipGetLastError());
 }
 
@@ -592,7 +606,8 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
     for (int k = 0; k < 9; ++k)
         if (host[k] != dev[k])
             PS_CK(hipMemcpyAsync(host[k], dev[k], n * 4, hipMemcpyDeviceToHost, e->s));
-    if (e->stats_on) PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 8, hipMemcpyDeviceToHost, e->s));
+    if (e->stats_on)
+        PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 16, hipMemcpyDeviceToHost, e->s));
     PS_CK(hipEventRecord(e->ev[7], e->s));
     PS_CK(hipStreamSynchronize(e->s));
     e->st.n_particles = n;
@@ -601,6 +616,7 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
     std::memcpy(e->st.level_start, e->ls, sizeof(e->ls));
     if (e->stats_on) {
         e->st.p2p_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 32);
+        e->st.p2p_leaf_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 34);
         e->st.ms_sort = e->elapsed(1, 2);
         e->st.ms_build = e->elapsed(2, 3);
         e->st.ms_upward = e->elapsed(3, 4);

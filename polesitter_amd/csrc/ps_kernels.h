@@ -69,28 +69,41 @@ struct PsTravTree {
     const uint2* link;
     const uint32_t* nend;
     float4* loc;
-    uint32_t* p2p_off;
-    uint32_t* p2p_len;
-    uint2* p2p_src;
+    uint32_t* p2p_off;   // per leaf: arena offset of its final source list
+    uint32_t* p2p_len;   // per leaf: length of that list (0: no near field)
+    uint32_t* p2p_moff;  // per leaf: offset of the list's far masks
+    unsigned long long* p2p_leaf_pairs;  // stats counter (may be null)
+};
+// One traversal iteration's entries (target, list offset and length in the
+// list arena, offset of its far masks in the mask arena).
+struct PsTravEntries {
+    uint32_t* tgt;
+    uint32_t* off;
+    uint32_t* len;
+    uint32_t* moff;
 };
 // iteration-0 entries: depth-1 leaves and all depth-2 nodes (MT) with list {root}
 void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* link,
-                         uint32_t* tgt, uint32_t* off, uint32_t* len, hipStream_t s);
-void ps_launch_trav_count(PsTravTree t, float theta2, const uint32_t* tgt, const uint32_t* off,
-                          const uint32_t* len, const uint32_t* list, uint32_t n_ent,
-                          uint4* cnt, int kahan, hipStream_t s);
-void ps_launch_trav_write(PsTravTree t, float theta2, const uint32_t* tgt, const uint32_t* off,
-                          const uint32_t* len, const uint32_t* list, uint32_t n_ent,
-                          const uint4* cnt, const uint4* pos, uint32_t p2p_base,
-                          uint32_t* ntgt, uint32_t* noff, uint32_t* nlen, uint32_t* nlist,
+                         PsTravEntries ent, hipStream_t s);
+// MAC + M2L for every (entry target, listed source); far masks; per-entry
+// output sizes cnt = (next list length, next entries, next mask words, far).
+void ps_launch_trav_count(PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* arena,
+                          unsigned long long* masks, uint32_t n_ent, uint4* cnt, int kahan,
                           hipStream_t s);
+// next lists (appended to the arena at list_base) and next entries
+void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
+                          const unsigned long long* masks, uint32_t n_ent, const uint4* cnt,
+                          const uint4* pos, uint32_t list_base, uint32_t mask_base,
+                          PsTravEntries next, hipStream_t s);
 
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------
 // mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);
-// 2: f += m*sum (P2P only).
+// 2: f += m*sum (P2P only).  A leaf's near sources are the non-far entries of
+// its final traversal list.
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
-                   const uint32_t* p2p_off, const uint32_t* p2p_len, const uint2* p2p_src,
-                   float* fx, float* fy, float* fz, int mode, int kahan,
+                   const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
+                   const uint32_t* arena, const unsigned long long* masks, const uint2* link,
+                   const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
                    unsigned long long* pair_count, hipStream_t s);
 
 // ---- ps_direct.hip: FP64 direct sum (the error oracle on the device) -------

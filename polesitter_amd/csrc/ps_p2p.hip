@@ -4,9 +4,9 @@
 // One thread per target particle, particles in sorted order: the lanes of a
 // wave are spatially adjacent, and in dense leaves they share one leaf and so
 // the same source list -- every source load is the same address across the
-// wave.  The source list of a leaf is the CSR range the traversal wrote:
-// (first, end) particle ranges of its near leaves, self included (the self
-// term is zero: d = 0).  Output per particle, in one write:
+// wave.  A leaf's sources are the near (mask bit clear) entries of its final
+// traversal list, read in place from the list arena: near leaves only, self
+// included (the self term is zero: d = 0).  Output per particle, one write:
 //   f = m * field(leaf) + m * sum_j m_j d / (|d|^2 + eps^2)^(3/2)
 // FP32 with v_rsq_f32; optional Kahan-compensated accumulation.
 #include "ps_device.h"
@@ -20,9 +20,10 @@ template <int kMode, bool kKahan>
 __global__ __launch_bounds__(kBlock) void p2p_kernel(
     size_t n, const float4* __restrict__ pos4, const uint32_t* __restrict__ leaf_of,
     const float4* __restrict__ loc, const uint32_t* __restrict__ p2p_off,
-    const uint32_t* __restrict__ p2p_len, const uint2* __restrict__ p2p_src,
-    float* __restrict__ fx, float* __restrict__ fy, float* __restrict__ fz,
-    unsigned long long* pair_count) {
+    const uint32_t* __restrict__ p2p_len, const uint32_t* __restrict__ p2p_moff,
+    const uint32_t* __restrict__ arena, const unsigned long long* __restrict__ masks,
+    const uint2* __restrict__ link, const uint32_t* __restrict__ nend, float* __restrict__ fx,
+    float* __restrict__ fy, float* __restrict__ fz, unsigned long long* pair_count) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < n;
     float ax = 0.f, ay = 0.f, az = 0.f;
@@ -33,23 +34,32 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
     if (valid) {
         lf = leaf_of[i];
         p = pos4[i];
-        if (kMode != 1) {
+        const uint32_t L = (kMode != 1) ? p2p_len[lf] : 0u;
+        if (L) {
             const uint32_t o = p2p_off[lf];
-            const uint32_t L = p2p_len[lf];
-            for (uint32_t k = 0; k < L; ++k) {
-                const uint2 r = p2p_src[o + k];
-                const uint32_t j_end = r.y < (uint32_t)n ? r.y : (uint32_t)n;
-                pairs += r.y - r.x;
-                for (uint32_t j = r.x; j < j_end; ++j) {
-                    const float4 q = pos4[j];
-                    if (kKahan) {
-                        float tx = 0.f, ty = 0.f, tz = 0.f;
-                        ps_pair(p, q, tx, ty, tz);
-                        ps_kahan_add(ax, cx, tx);
-                        ps_kahan_add(ay, cy, ty);
-                        ps_kahan_add(az, cz, tz);
-                    } else {
-                        ps_pair(p, q, ax, ay, az);
+            const uint32_t mo = p2p_moff[lf];
+            for (uint32_t k0 = 0; k0 < L; k0 += 64) {
+                unsigned long long near = ~masks[mo + (k0 >> 6)];
+                if (L - k0 < 64) near &= (1ull << (L - k0)) - 1ull;
+                while (near) {
+                    const uint32_t b = (uint32_t)__ffsll((long long)near) - 1u;
+                    near &= near - 1ull;
+                    const uint32_t s = arena[o + k0 + b];
+                    const uint32_t j0 = link[s].x;
+                    const uint32_t j1e = nend[s];
+                    const uint32_t j1 = j1e < (uint32_t)n ? j1e : (uint32_t)n;
+                    pairs += j1e - j0;
+                    for (uint32_t j = j0; j < j1; ++j) {
+                        const float4 q = pos4[j];
+                        if (kKahan) {
+                            float tx = 0.f, ty = 0.f, tz = 0.f;
+                            ps_pair(p, q, tx, ty, tz);
+                            ps_kahan_add(ax, cx, tx);
+                            ps_kahan_add(ay, cy, ty);
+                            ps_kahan_add(az, cz, tz);
+                        } else {
+                            ps_pair(p, q, ax, ay, az);
+                        }
                     }
                 }
             }
@@ -84,32 +94,36 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
 
 template <int kMode>
 void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
-                 const uint32_t* p2p_off, const uint32_t* p2p_len, const uint2* p2p_src,
-                 float* fx, float* fy, float* fz, int kahan, unsigned long long* pc,
-                 hipStream_t s) {
+                 const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
+                 const uint32_t* arena, const unsigned long long* masks, const uint2* link,
+                 const uint32_t* nend, float* fx, float* fy, float* fz, int kahan,
+                 unsigned long long* pc, hipStream_t s) {
     unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     if (kahan)
         p2p_kernel<kMode, true><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
-                                                          p2p_len, p2p_src, fx, fy, fz, pc);
+                                                          p2p_len, p2p_moff, arena, masks, link,
+                                                          nend, fx, fy, fz, pc);
     else
         p2p_kernel<kMode, false><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
-                                                           p2p_len, p2p_src, fx, fy, fz, pc);
+                                                           p2p_len, p2p_moff, arena, masks, link,
+                                                           nend, fx, fy, fz, pc);
 }
 
 }  // namespace
 
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
-                   const uint32_t* p2p_off, const uint32_t* p2p_len, const uint2* p2p_src,
-                   float* fx, float* fy, float* fz, int mode, int kahan,
+                   const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
+                   const uint32_t* arena, const unsigned long long* masks, const uint2* link,
+                   const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
                    unsigned long long* pair_count, hipStream_t s) {
     if (n == 0) return;
     if (mode == 0)
-        launch_mode<0>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_src, fx, fy, fz, kahan,
-                       pair_count, s);
+        launch_mode<0>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
+                       nend, fx, fy, fz, kahan, pair_count, s);
     else if (mode == 1)
-        launch_mode<1>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_src, fx, fy, fz, kahan,
-                       pair_count, s);
+        launch_mode<1>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
+                       nend, fx, fy, fz, kahan, pair_count, s);
     else
-        launch_mode<2>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_src, fx, fy, fz, kahan,
-                       pair_count, s);
+        launch_mode<2>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
+                       nend, fx, fy, fz, kahan, pair_count, s);
 }

@@ -10,13 +10,20 @@
 //   near leaf -> kept; near internal -> replaced by its children.  The
 //   result C(t) is the list of EVERY child of an internal t (they all see the
 //   same sources), or the next list of a leaf t itself while C(t) still holds
-//   internal nodes; once it holds only leaves, those are t's P2P sources.
+//   internal nodes.  A leaf t whose list has no near internal node is final:
+//   the near entries of that list are its P2P sources.
 // This visits exactly the reference's pairs (the pair set, not the order),
-// and every target owns its accumulations: no atomics, deterministic output.
+// and every target owns its accumulations: no atomics on the results.
 //
-// One iteration = count (MAC + M2L, per-entry output sizes) -> scan ->
-// write (MAC again, emit next lists / next entries / P2P sources), one
-// wavefront per entry in both kernels.
+// One iteration = count -> scan -> write, one wavefront per entry:
+//   count  MAC for each listed source (the 64 lanes take the list in
+//          strides), M2L for far ones, a 64-bit far mask per 64-entry chunk,
+//          and the output sizes; a final leaf records (list, masks) for P2P.
+//   write  for continuing entries only: the mask picks the near sources,
+//          near leaves are copied and near internal nodes expanded to their
+//          children, compacted with a wave prefix sum into the list arena.
+// Lists are appended to one arena that lives for the whole tick, so the P2P
+// kernel reads each leaf's final list in place.
 #include "ps_device.h"
 #include "ps_kernels.h"
 
@@ -28,60 +35,68 @@ constexpr int kWavesPerBlock = kBlock / 64;
 __global__ __launch_bounds__(kBlock) void trav_init_kernel(uint32_t ls1, uint32_t ls2,
                                                            uint32_t ls3,
                                                            const uint2* __restrict__ link,
-                                                           uint32_t* tgt, uint32_t* off,
-                                                           uint32_t* len) {
+                                                           PsTravEntries ent) {
     uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     uint32_t n = ls1 + k;
     if (n >= ls3) return;
-    tgt[k] = n;
-    off[k] = 0;  // the list {root} at offset 0
+    ent.tgt[k] = n;
+    ent.off[k] = 0;  // the list {root} at arena offset 0
     // jobs: every node at depth 2 and every leaf above it (H:1780)
-    len[k] = (n >= ls2 || (link[n].y & PS_LEAF)) ? 1u : 0u;
+    ent.len[k] = (n >= ls2 || (link[n].y & PS_LEAF)) ? 1u : 0u;
+    ent.moff[k] = k;  // one mask word per entry
 }
 
-// One wavefront per entry: the 64 lanes take the entry's source list in
-// strides, so list loads are coalesced and the per-source work (MAC, M2L or
-// the near-field bookkeeping) is spread over the wave; sums are reduced
-// across lanes at the end.
+__device__ __forceinline__ uint32_t chunks_of(uint32_t len) { return (len + 63u) >> 6; }
+
 template <bool kKahan>
 __global__ __launch_bounds__(kBlock) void trav_count_kernel(
-    PsTravTree t, float theta2, const uint32_t* __restrict__ tgt,
-    const uint32_t* __restrict__ off, const uint32_t* __restrict__ len,
-    const uint32_t* __restrict__ list, uint32_t n_ent, uint4* __restrict__ cnt) {
+    PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* __restrict__ arena,
+    unsigned long long* __restrict__ masks, uint32_t n_ent, uint4* __restrict__ cnt) {
     const uint32_t e = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
     if (e >= n_ent) return;  // uniform per wave
-    const uint32_t tn = tgt[e];
-    const uint32_t o = off[e];
-    const uint32_t L = len[e];
+    const uint32_t tn = ent.tgt[e];
+    const uint32_t o = ent.off[e];
+    const uint32_t L = ent.len[e];
+    const uint32_t mo = ent.moff[e];
     const float4 tg = t.geom[tn];
     const uint2 tl = t.link[tn];
     float ex = 0.f, ey = 0.f, ez = 0.f;
     float cx = 0.f, cy = 0.f, cz = 0.f;
     uint32_t nfar = 0, nleaf = 0, nchild = 0;
-    for (uint32_t k = lane; k < L; k += 64) {
-        const uint32_t s = list[o + k];
-        if (s >= t.n_nodes) continue;  // never taken when counts and writes agree
-        const float4 sg = t.geom[s];
-        float dx, dy, dz, d2;
-        if (ps_far_d(tg, sg, theta2, dx, dy, dz, d2)) {
-            if (kKahan) {
-                float px = 0.f, py = 0.f, pz = 0.f;
-                ps_m2l_d(dx, dy, dz, d2, t.mp[s], px, py, pz);
-                ps_kahan_add(ex, cx, px);
-                ps_kahan_add(ey, cy, py);
-                ps_kahan_add(ez, cz, pz);
+    for (uint32_t k0 = 0; k0 < L; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        bool far = false;
+        if (k < L) {
+            const uint32_t s = arena[o + k];
+            if (s < t.n_nodes) {  // always true when counts and writes agree
+                const float4 sg = t.geom[s];
+                float dx, dy, dz, d2;
+                far = ps_far_d(tg, sg, theta2, dx, dy, dz, d2);
+                if (far) {
+                    if (kKahan) {
+                        float px = 0.f, py = 0.f, pz = 0.f;
+                        ps_m2l_d(dx, dy, dz, d2, t.mp[s], px, py, pz);
+                        ps_kahan_add(ex, cx, px);
+                        ps_kahan_add(ey, cy, py);
+                        ps_kahan_add(ez, cz, pz);
+                    } else {
+                        ps_m2l_d(dx, dy, dz, d2, t.mp[s], ex, ey, ez);
+                    }
+                    ++nfar;
+                } else {
+                    const uint2 sl = t.link[s];
+                    if (sl.y & PS_LEAF)
+                        ++nleaf;
+                    else
+                        nchild += (uint32_t)__popc(sl.y & 0xFFu);
+                }
             } else {
-                ps_m2l_d(dx, dy, dz, d2, t.mp[s], ex, ey, ez);
+                far = true;  // an invalid id is skipped by every consumer
             }
-            ++nfar;
-        } else {
-            const uint2 sl = t.link[s];
-            if (sl.y & PS_LEAF)
-                ++nleaf;
-            else
-                nchild += (uint32_t)__popc(sl.y & 0xFFu);
         }
+        const unsigned long long fm = __ballot(far);
+        if (lane == 0) masks[mo + (k0 >> 6)] = fm;
     }
     if (kKahan) {
 #pragma unroll
@@ -110,118 +125,110 @@ __global__ __launch_bounds__(kBlock) void trav_count_kernel(
         t.loc[tn] = l;
     }
     const uint32_t out = nleaf + nchild;
-    uint4 c;
+    uint4 c = make_uint4(0u, 0u, 0u, nfar);
     if (tl.y & PS_LEAF) {
-        c = (nchild == 0) ? make_uint4(0u, 0u, nleaf, nfar) : make_uint4(out, 1u, 0u, nfar);
-    } else {
-        c = out ? make_uint4(out, ps_child_count(tl), 0u, nfar) : make_uint4(0u, 0u, 0u, nfar);
+        if (nchild == 0) {  // final: the near entries of this list are the P2P sources
+            if (nleaf) {
+                t.p2p_off[tn] = o;
+                t.p2p_len[tn] = L;
+                t.p2p_moff[tn] = mo;
+                if (t.p2p_leaf_pairs) atomicAdd(t.p2p_leaf_pairs, (unsigned long long)nleaf);
+            }
+        } else {
+            c = make_uint4(out, 1u, chunks_of(out), nfar);
+        }
+    } else if (out) {
+        const uint32_t nk = ps_child_count(tl);
+        c = make_uint4(out, nk, nk * chunks_of(out), nfar);
     }
     cnt[e] = c;
 }
 
-// One wavefront per entry: each chunk of 64 sources re-runs the MAC, the
-// lanes' output counts (1 per near leaf, the child count per near internal
-// node) are prefix-summed across the wave, and the outputs land contiguously
-// in source order.
 __global__ __launch_bounds__(kBlock) void trav_write_kernel(
-    PsTravTree t, float theta2, const uint32_t* __restrict__ tgt,
-    const uint32_t* __restrict__ off, const uint32_t* __restrict__ len,
-    const uint32_t* __restrict__ list, uint32_t n_ent, const uint4* __restrict__ cnt,
-    const uint4* __restrict__ pos, uint32_t p2p_base, uint32_t* __restrict__ ntgt,
-    uint32_t* __restrict__ noff, uint32_t* __restrict__ nlen, uint32_t* __restrict__ nlist) {
+    PsTravTree t, PsTravEntries ent, uint32_t* __restrict__ arena,
+    const unsigned long long* __restrict__ masks, uint32_t n_ent, const uint4* __restrict__ cnt,
+    const uint4* __restrict__ pos, uint32_t list_base, uint32_t mask_base, PsTravEntries next) {
     const uint32_t e = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
     if (e >= n_ent) return;
     const uint4 c = cnt[e];
-    if (c.x == 0 && c.z == 0) return;
+    if (c.x == 0) return;  // final or empty: nothing to pass on (uniform)
     const uint4 p = pos[e];
-    const uint32_t tn = tgt[e];
-    const uint32_t o = off[e];
-    const uint32_t L = len[e];
-    const float4 tg = t.geom[tn];
+    const uint32_t tn = ent.tgt[e];
+    const uint32_t o = ent.off[e];
+    const uint32_t L = ent.len[e];
+    const uint32_t mo = ent.moff[e];
     const uint2 tl = t.link[tn];
-    const bool final_p2p = c.z > 0;
-    // the MAC is re-evaluated with the same code as the count pass, so the
-    // outputs fill exactly the counted range; the bound only makes that an
-    // invariant of the stores
-    uint32_t base = final_p2p ? p2p_base + p.z : p.x;
-    const uint32_t limit = base + (final_p2p ? c.z : c.x);
+    // the count pass's masks and link reads give exactly c.x outputs; the
+    // bound only makes that an invariant of the stores
+    uint32_t base = list_base + p.x;
+    const uint32_t limit = base + c.x;
     for (uint32_t k0 = 0; k0 < L; k0 += 64) {
         const uint32_t k = k0 + lane;
+        const unsigned long long fm = masks[mo + (k0 >> 6)];
         uint32_t s = 0, nout = 0;
         uint2 sl = make_uint2(0u, 0u);
-        if (k < L) {
-            s = list[o + k];
-            if (s < t.n_nodes && !ps_far(tg, t.geom[s], theta2)) {
-                sl = t.link[s];
-                nout = (sl.y & PS_LEAF) ? 1u : (uint32_t)__popc(sl.y & 0xFFu);
-            }
+        if (k < L && !((fm >> lane) & 1ull)) {
+            s = arena[o + k];
+            sl = t.link[s];
+            nout = (sl.y & PS_LEAF) ? 1u : (uint32_t)__popc(sl.y & 0xFFu);
         }
         uint32_t total;
         const uint32_t w = base + ps_wave_excl_scan(nout, &total);
         if (nout) {
             if (sl.y & PS_LEAF) {
-                if (w < limit) {
-                    if (final_p2p)
-                        t.p2p_src[w] = make_uint2(sl.x, t.nend[s]);
-                    else
-                        nlist[w] = s;
-                }
+                if (w < limit) arena[w] = s;
             } else {
                 for (uint32_t r = 0; r < nout; ++r)
-                    if (w + r < limit) nlist[w + r] = sl.x + r;
+                    if (w + r < limit) arena[w + r] = sl.x + r;
             }
         }
         base += total;
     }
-    if (c.y) {
-        if (tl.y & PS_LEAF) {
-            if (lane == 0) {
-                ntgt[p.y] = tn;
-                noff[p.y] = p.x;
-                nlen[p.y] = c.x;
-            }
-        } else if (lane < c.y) {
-            ntgt[p.y + lane] = tl.x + lane;
-            noff[p.y + lane] = p.x;
-            nlen[p.y + lane] = c.x;
+    const uint32_t nch = chunks_of(c.x);
+    if (tl.y & PS_LEAF) {
+        if (lane == 0) {
+            next.tgt[p.y] = tn;
+            next.off[p.y] = list_base + p.x;
+            next.len[p.y] = c.x;
+            next.moff[p.y] = mask_base + p.z;
         }
-    }
-    if (final_p2p && lane == 0) {
-        t.p2p_off[tn] = p2p_base + p.z;
-        t.p2p_len[tn] = c.z;
+    } else if (lane < c.y) {
+        next.tgt[p.y + lane] = tl.x + lane;
+        next.off[p.y + lane] = list_base + p.x;
+        next.len[p.y + lane] = c.x;
+        next.moff[p.y + lane] = mask_base + p.z + lane * nch;
     }
 }
 
 }  // namespace
 
 void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* link,
-                         uint32_t* tgt, uint32_t* off, uint32_t* len, hipStream_t s) {
+                         PsTravEntries ent, hipStream_t s) {
     if (ls3 <= ls1) return;
     uint32_t blocks = (ls3 - ls1 + kBlock - 1) / kBlock;
-    trav_init_kernel<<<blocks, kBlock, 0, s>>>(ls1, ls2, ls3, link, tgt, off, len);
+    trav_init_kernel<<<blocks, kBlock, 0, s>>>(ls1, ls2, ls3, link, ent);
 }
 
-void ps_launch_trav_count(PsTravTree t, float theta2, const uint32_t* tgt, const uint32_t* off,
-                          const uint32_t* len, const uint32_t* list, uint32_t n_ent,
-                          uint4* cnt, int kahan, hipStream_t s) {
-    if (n_ent == 0) return;
-    uint32_t blocks = (n_ent + kWavesPerBlock - 1) / kWavesPerBlock;
-    if (kahan)
-        trav_count_kernel<true><<<blocks, kBlock, 0, s>>>(t, theta2, tgt, off, len, list, n_ent,
-                                                          cnt);
-    else
-        trav_count_kernel<false><<<blocks, kBlock, 0, s>>>(t, theta2, tgt, off, len, list,
-                                                           n_ent, cnt);
-}
-
-void ps_launch_trav_write(PsTravTree t, float theta2, const uint32_t* tgt, const uint32_t* off,
-                          const uint32_t* len, const uint32_t* list, uint32_t n_ent,
-                          const uint4* cnt, const uint4* pos, uint32_t p2p_base,
-                          uint32_t* ntgt, uint32_t* noff, uint32_t* nlen, uint32_t* nlist,
+void ps_launch_trav_count(PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* arena,
+                          unsigned long long* masks, uint32_t n_ent, uint4* cnt, int kahan,
                           hipStream_t s) {
     if (n_ent == 0) return;
     uint32_t blocks = (n_ent + kWavesPerBlock - 1) / kWavesPerBlock;
-    trav_write_kernel<<<blocks, kBlock, 0, s>>>(t, theta2, tgt, off, len, list, n_ent, cnt, pos,
-                                                p2p_base, ntgt, noff, nlen, nlist);
+    if (kahan)
+        trav_count_kernel<true><<<blocks, kBlock, 0, s>>>(t, theta2, ent, arena, masks, n_ent,
+                                                          cnt);
+    else
+        trav_count_kernel<false><<<blocks, kBlock, 0, s>>>(t, theta2, ent, arena, masks, n_ent,
+                                                           cnt);
+}
+
+void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
+                          const unsigned long long* masks, uint32_t n_ent, const uint4* cnt,
+                          const uint4* pos, uint32_t list_base, uint32_t mask_base,
+                          PsTravEntries next, hipStream_t s) {
+    if (n_ent == 0) return;
+    uint32_t blocks = (n_ent + kWavesPerBlock - 1) / kWavesPerBlock;
+    trav_write_kernel<<<blocks, kBlock, 0, s>>>(t, ent, arena, masks, n_ent, cnt, pos, list_base,
+                                                mask_base, next);
 }

@@ -143,7 +143,7 @@ struct ps_hip_engine {
     // the far-mask arena fill over the whole tick
     DevBuf ent[2][4];
     DevBuf arena_b, masks_b;
-    DevBuf cnt_b, pos_b, scan_tmp;
+    DevBuf cnt_b, pos_b, scan_tmp, fin_b, fin_tmp, block_pairs_b;
     // small device / pinned host scalars
     DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs | u64 leaf pairs
     uint32_t* h_small = nullptr;  // pinned mirror
@@ -215,7 +215,7 @@ struct ps_hip_engine {
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
                          &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &cnt_b,     &pos_b,
-                         &scan_tmp,  &small_b};
+                         &scan_tmp,  &small_b, &fin_b, &fin_tmp, &block_pairs_b};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
             for (auto& b : es) release(b);
@@ -337,13 +337,9 @@ int stage_traverse(ps_hip_engine* e) {
     t.p2p_off = e->p2p_off();
     t.p2p_len = e->p2p_len();
     t.p2p_moff = e->p2p_moff();
-    t.p2p_leaf_pairs = nullptr;
-    if (e->stats_on) {
-        t.p2p_leaf_pairs = e->leaf_pairs();
-        ps_fill_u32(reinterpret_cast<uint32_t*>(t.p2p_leaf_pairs), 0u, 2, e->s);
-    }
+    t.fin_leaves = nullptr;
 
-    unsigned long long m2l = 0, lists = 0;
+    unsigned long long m2l = 0, lists = 0, fin_leaves = 0;
     uint32_t iters = 0;
     while (n_ent > 0) {
         const int nxt = cur ^ 1;
@@ -352,12 +348,20 @@ int stage_traverse(ps_hip_engine* e) {
         PS_CK(grow(e->scan_tmp, ps_scan_tmp_bytes_u4(n_ent), e->s));
         uint4* cnt = reinterpret_cast<uint4*>(e->cnt_b.p);
         uint4* pos = reinterpret_cast<uint4*>(e->pos_b.p);
+        if (e->stats_on) {
+            PS_CK(grow(e->fin_b, 4 * (size_t)n_ent, e->s));
+            PS_CK(grow(e->fin_tmp, ps_scan_tmp_bytes_u32(n_ent), e->s));
+            t.fin_leaves = reinterpret_cast<uint32_t*>(e->fin_b.p);
+        }
         e->record(8);
         ps_launch_trav_count(t, theta2, e->entries(cur), e->arena(), e->masks(), n_ent, cnt,
                              e->kahan, e->s);
         e->record(9);
         PS_CK(ps_scan_u4(cnt, pos, n_ent, e->totals(), e->scan_tmp.p, e->s));
-        PS_CK(hipMemcpyAsync(e->h_small, e->totals(), 32, hipMemcpyDeviceToHost, e->s));
+        if (e->stats_on)  // near leaves of the lists that became final (P2P leaf pairs)
+            PS_CK(ps_scan_u32(t.fin_leaves, t.fin_leaves, n_ent, e->leaf_pairs(), e->fin_tmp.p,
+                              e->s));
+        PS_CK(hipMemcpyAsync(e->h_small, e->totals(), 48, hipMemcpyDeviceToHost, e->s));
         PS_CK(hipStreamSynchronize(e->s));
         e->st.ms_trav_count += e->elapsed(8, 9);
         if (iters > 0) e->st.ms_trav_write += e->elapsed(10, 11);
@@ -365,6 +369,7 @@ int stage_traverse(ps_hip_engine* e) {
         const unsigned long long list_n = tot[0], ent_n = tot[1], chunk_n = tot[2];
         m2l += tot[3];
         lists += list_n;
+        if (e->stats_on) fin_leaves += tot[5];
         if (arena_used + list_n > 0xFFFFFFFFull || ent_n > 0xFFFFFFFFull ||
             mask_used + chunk_n > 0xFFFFFFFFull)
             return PS_HIP_EOOM;
@@ -388,6 +393,7 @@ int stage_traverse(ps_hip_engine* e) {
         e->st.ms_trav_write += e->elapsed(10, 11);
     }
     e->st.m2l_pairs = m2l;
+    e->st.p2p_leaf_pairs = fin_leaves;
     e->st.list_entries = lists;
     e->st.iterations = iters;
     return map_err(hipGetLastError());
@@ -403,14 +409,15 @@ int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
     unsigned long long* pc = nullptr;
     if (e->stats_on) {
         pc = e->pairs();
-        ps_fill_u32(reinterpret_cast<uint32_t*>(pc), 0u, 2, e->s);
+        PS_CK(grow(e->block_pairs_b, 8 * (size_t)ps_p2p_blocks(e->n), e->s));
     }
     if (!e->arena_b.p) PS_CK(grow(e->arena_b, 256, e->s));
     if (!e->masks_b.p) PS_CK(grow(e->masks_b, 256, e->s));
     if (ev_before >= 0) e->record(ev_before);
     ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
                   e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
-                  e->fz, mode, e->kahan, pc, e->s);
+                  e->fz, mode, e->kahan,
+                  reinterpret_cast<unsigned long long*>(e->block_pairs_b.p), pc, e->s);
     return map_err(hipGetLastError());
 }
 
@@ -607,7 +614,7 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
         if (host[k] != dev[k])
             PS_CK(hipMemcpyAsync(host[k], dev[k], n * 4, hipMemcpyDeviceToHost, e->s));
     if (e->stats_on)
-        PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 16, hipMemcpyDeviceToHost, e->s));
+        PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 8, hipMemcpyDeviceToHost, e->s));
     PS_CK(hipEventRecord(e->ev[7], e->s));
     PS_CK(hipStreamSynchronize(e->s));
     e->st.n_particles = n;
@@ -616,7 +623,6 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
     std::memcpy(e->st.level_start, e->ls, sizeof(e->ls));
     if (e->stats_on) {
         e->st.p2p_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 32);
-        e->st.p2p_leaf_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 34);
         e->st.ms_sort = e->elapsed(1, 2);
         e->st.ms_build = e->elapsed(2, 3);
         e->st.ms_upward = e->elapsed(3, 4);

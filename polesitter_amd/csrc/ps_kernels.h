@@ -18,6 +18,9 @@ hipError_t ps_scan_u4(const uint4* in, uint4* out, size_t n, unsigned long long*
                       void* tmp, hipStream_t s);
 // graph-safe fills (no memset nodes)
 void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// *out = sum of in[0..n) (one block, no atomics)
+void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* out,
+                   hipStream_t s);
 
 // ---- ps_prepare.hip: bounding cube + Morton codes, H:1911-2170 --------------
 // prm (device, 4 floats): min, max, range, 1023/range.  part: >= 1024 float2.
@@ -72,7 +75,7 @@ struct PsTravTree {
     uint32_t* p2p_off;   // per leaf: arena offset of its final source list
     uint32_t* p2p_len;   // per leaf: length of that list (0: no near field)
     uint32_t* p2p_moff;  // per leaf: offset of the list's far masks
-    unsigned long long* p2p_leaf_pairs;  // stats counter (may be null)
+    uint32_t* fin_leaves;  // stats: per entry, near leaves of a final list (may be null)
 };
 // One traversal iteration's entries (target, list offset and length in the
 // list arena, offset of its far masks in the mask arena).
@@ -100,11 +103,15 @@ void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
 // mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);
 // 2: f += m*sum (P2P only).  A leaf's near sources are the non-far entries of
 // its final traversal list.
+// pair_count (stats, may be null): per-block partial sums in block_pairs
+// (>= ps_p2p_blocks(n) entries), reduced into *pair_count.
+uint32_t ps_p2p_blocks(size_t n);
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                    const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
-                   unsigned long long* pair_count, hipStream_t s);
+                   unsigned long long* block_pairs, unsigned long long* pair_count,
+                   hipStream_t s);
 
 // ---- ps_direct.hip: FP64 direct sum (the error oracle on the device) -------
 void ps_launch_direct_f64(const float4* pos4, size_t n, const uint32_t* targets, size_t nt,

@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
     const uint32_t* __restrict__ p2p_len, const uint32_t* __restrict__ p2p_moff,
     const uint32_t* __restrict__ arena, const unsigned long long* __restrict__ masks,
     const uint2* __restrict__ link, const uint32_t* __restrict__ nend, float* __restrict__ fx,
-    float* __restrict__ fy, float* __restrict__ fz, unsigned long long* pair_count) {
+    float* __restrict__ fy, float* __restrict__ fz, unsigned long long* block_pairs) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < n;
     float ax = 0.f, ay = 0.f, az = 0.f;
@@ -65,11 +65,18 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
             }
         }
     }
-    if (pair_count) {
+    if (block_pairs) {  // per-block partial sums, reduced by another kernel
+        __shared__ unsigned long long wsum[kBlock / 64];
         unsigned long long v = pairs;
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(pair_count, v);
+        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long b = 0;
+            for (int w = 0; w < kBlock / 64; ++w) b += wsum[w];
+            block_pairs[blockIdx.x] = b;
+        }
     }
     if (!valid) return;
     {
@@ -97,33 +104,38 @@ void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const fl
                  const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                  const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                  const uint32_t* nend, float* fx, float* fy, float* fz, int kahan,
-                 unsigned long long* pc, hipStream_t s) {
+                 unsigned long long* bp, hipStream_t s) {
     unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     if (kahan)
         p2p_kernel<kMode, true><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
                                                           p2p_len, p2p_moff, arena, masks, link,
-                                                          nend, fx, fy, fz, pc);
+                                                          nend, fx, fy, fz, bp);
     else
         p2p_kernel<kMode, false><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
                                                            p2p_len, p2p_moff, arena, masks, link,
-                                                           nend, fx, fy, fz, pc);
+                                                           nend, fx, fy, fz, bp);
 }
 
 }  // namespace
+
+uint32_t ps_p2p_blocks(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                    const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
-                   unsigned long long* pair_count, hipStream_t s) {
+                   unsigned long long* block_pairs, unsigned long long* pair_count,
+                   hipStream_t s) {
     if (n == 0) return;
+    unsigned long long* bp = pair_count ? block_pairs : nullptr;
     if (mode == 0)
         launch_mode<0>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
-                       nend, fx, fy, fz, kahan, pair_count, s);
+                       nend, fx, fy, fz, kahan, bp, s);
     else if (mode == 1)
         launch_mode<1>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
-                       nend, fx, fy, fz, kahan, pair_count, s);
+                       nend, fx, fy, fz, kahan, bp, s);
     else
         launch_mode<2>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
-                       nend, fx, fy, fz, kahan, pair_count, s);
+                       nend, fx, fy, fz, kahan, bp, s);
+    if (bp) ps_reduce_u64(bp, ps_p2p_blocks(n), pair_count, s);
 }

@@ -130,6 +130,22 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+__global__ __launch_bounds__(kBlock) void reduce_u64_kernel(const unsigned long long* in,
+                                                             size_t n, unsigned long long* out) {
+    __shared__ unsigned long long ws[kWaves];
+    unsigned long long v = 0;
+    for (size_t i = threadIdx.x; i < n; i += kBlock) v += in[i];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int w = 0; w < kWaves; ++w) s += ws[w];
+        *out = s;
+    }
+}
+
 __global__ void fill_u64_kernel(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
@@ -192,4 +208,9 @@ void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     fill_u32_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, v, n);
+}
+
+void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* out,
+                   hipStream_t s) {
+    reduce_u64_kernel<<<1, kBlock, 0, s>>>(in, n, out);
 }

@@ -132,14 +132,18 @@ __global__ __launch_bounds__(kBlock) void trav_count_kernel(
                 t.p2p_off[tn] = o;
                 t.p2p_len[tn] = L;
                 t.p2p_moff[tn] = mo;
-                if (t.p2p_leaf_pairs) atomicAdd(t.p2p_leaf_pairs, (unsigned long long)nleaf);
             }
+            if (t.fin_leaves) t.fin_leaves[e] = nleaf;
         } else {
+            if (t.fin_leaves) t.fin_leaves[e] = 0u;
             c = make_uint4(out, 1u, chunks_of(out), nfar);
         }
-    } else if (out) {
-        const uint32_t nk = ps_child_count(tl);
-        c = make_uint4(out, nk, nk * chunks_of(out), nfar);
+    } else {
+        if (t.fin_leaves) t.fin_leaves[e] = 0u;
+        if (out) {
+            const uint32_t nk = ps_child_count(tl);
+            c = make_uint4(out, nk, nk * chunks_of(out), nfar);
+        }
     }
     cnt[e] = c;
 }

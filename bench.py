@@ -121,6 +121,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="plummer1m")
     ap.add_argument("--theta", type=float, default=2.0)
     ap.add_argument("--kahan", action="store_true")
+    ap.add_argument("--trav-lanes", type=int, default=32, choices=(16, 32, 64))
     ap.add_argument("--cpu-ticks", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
@@ -136,6 +137,7 @@ def main(argv=None):
     log(f"bench: {api.device_name()} workload {a.workload}: {desc}")
     L = api.hip_lib()
     s = api.Solver(n, theta=a.theta, kahan=a.kahan, stats=True)
+    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_LANES, a.trav_lanes))
     s.load(x, y, z, m)
     vel = [api.DeviceArray.from_numpy(vv) for vv in v]
     tmp = api.DeviceArray(3 * n, np.float32)
@@ -196,7 +198,8 @@ def main(argv=None):
                 config=dict(workload=a.workload, description=desc, n_particles=n,
                             theta=a.theta, traversal="MT depth-2 jobs (PS_MULTITHREADING)",
                             step="tick + id-gather + semi-implicit Euler dt=1e-3",
-                            kahan=bool(a.kahan), pointers="device-resident"),
+                            kahan=bool(a.kahan), trav_lanes=a.trav_lanes,
+                            pointers="device-resident"),
                 roofline=roof,
                 stages_ms={k: round(mean("ms_" + k), 4) for k in
                            ("sort", "build", "upward", "traverse", "l2l", "p2p", "total",

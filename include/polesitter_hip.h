@@ -47,6 +47,7 @@ extern "C" {
 #define PS_HIP_OPT_KAHAN     2 /* 1: compensated far-field and P2P sums */
 #define PS_HIP_OPT_THETA     3 /* MAC theta, like ps_config_t.theta */
 #define PS_HIP_OPT_STATS     4 /* 1: count P2P pairs and time each stage */
+#define PS_HIP_OPT_TRAV_LANES 5 /* lanes per traversal entry: 16, 32 (default), 64 */
 
 typedef struct ps_hip_engine ps_hip_engine_t;
 

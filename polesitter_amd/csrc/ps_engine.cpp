@@ -130,6 +130,7 @@ struct ps_hip_engine {
     int trav = PS_HIP_TRAV_MT;
     int kahan = 0;
     int stats_on = 0;
+    int trav_lanes = 32;  // lanes per traversal entry
 
     // host-pointer staging, one buffer per array (allocated on first use)
     DevBuf stage[9];
@@ -355,7 +356,7 @@ int stage_traverse(ps_hip_engine* e) {
         }
         e->record(8);
         ps_launch_trav_count(t, theta2, e->entries(cur), e->arena(), e->masks(), n_ent, cnt,
-                             e->kahan, e->s);
+                             e->kahan, e->trav_lanes, e->s);
         e->record(9);
         PS_CK(ps_scan_u4(cnt, pos, n_ent, e->totals(), e->scan_tmp.p, e->s));
         if (e->stats_on)  // near leaves of the lists that became final (P2P leaf pairs)
@@ -380,7 +381,8 @@ int stage_traverse(ps_hip_engine* e) {
             PS_CK(grow(e->ent[nxt][k], 4 * std::max<unsigned long long>(ent_n, 1), e->s));
         e->record(10);
         ps_launch_trav_write(t, e->entries(cur), e->arena(), e->masks(), n_ent, cnt, pos,
-                             (uint32_t)arena_used, (uint32_t)mask_used, e->entries(nxt), e->s);
+                             (uint32_t)arena_used, (uint32_t)mask_used, e->entries(nxt),
+                             e->trav_lanes, e->s);
         e->record(11);
         arena_used += list_n;
         mask_used += chunk_n;
@@ -523,6 +525,10 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
             return PS_HIP_OK;
         case PS_HIP_OPT_STATS:
             e->stats_on = value != 0.0;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_TRAV_LANES:
+            if (value != 16 && value != 32 && value != 64) return PS_HIP_EINVAL;
+            e->trav_lanes = (int)value;
             return PS_HIP_OK;
         default:
             return PS_HIP_EINVAL;

@@ -90,14 +90,15 @@ void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* 
                          PsTravEntries ent, hipStream_t s);
 // MAC + M2L for every (entry target, listed source); far masks; per-entry
 // output sizes cnt = (next list length, next entries, next mask words, far).
+// lanes: lanes per entry (16, 32 or 64)
 void ps_launch_trav_count(PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* arena,
                           unsigned long long* masks, uint32_t n_ent, uint4* cnt, int kahan,
-                          hipStream_t s);
+                          int lanes, hipStream_t s);
 // next lists (appended to the arena at list_base) and next entries
 void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
                           const unsigned long long* masks, uint32_t n_ent, const uint4* cnt,
                           const uint4* pos, uint32_t list_base, uint32_t mask_base,
-                          PsTravEntries next, hipStream_t s);
+                          PsTravEntries next, int lanes, hipStream_t s);
 
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------
 // mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);

@@ -22,7 +22,7 @@ def log(msg):
 
 
 def gpu_tick(x, y, z, m, theta=2.0, trav=api.TRAV_MT, mode=api.MORTON_TRUNC, host=False,
-             kahan=False, ids=None, tree=True):
+             kahan=False, ids=None, tree=True, lanes=None):
     """ps_prepare_particles + ps_calc_forces on the device, root = min +
     range/2 (tests/accuracy.c:493-503).  host=True passes numpy (host)
     arrays, otherwise device arrays."""
@@ -31,6 +31,8 @@ def gpu_tick(x, y, z, m, theta=2.0, trav=api.TRAV_MT, mode=api.MORTON_TRUNC, hos
     x, y, z, m = (np.ascontiguousarray(a, f32).copy() for a in (x, y, z, m))
     ids = np.arange(n, dtype=np.uint32) if ids is None else np.asarray(ids, np.uint32).copy()
     with api.Solver(n, theta=theta, traversal=trav, kahan=kahan, stats=True) as s:
+        if lanes is not None:
+            api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_LANES, lanes))
         if host:
             fx, fy, fz = (np.zeros(n, f32) for _ in range(3))
             codes = np.zeros(n, np.uint32)

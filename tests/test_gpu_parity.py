@@ -72,6 +72,15 @@ def test_tick_matches_oracle(oracle, case, trav):
     log(f"  force rms {f:.2e} locals {l:.2e} m2l {g['stats']['m2l_pairs']}")
 
 
+@pytest.mark.parametrize("lanes", [16, 32, 64])
+def test_traversal_lane_groups(oracle, lanes):
+    # every lanes-per-entry variant of the traversal kernels visits the same pairs
+    x, y, z, m = W.two_plummers(80_000, seed=34)[:4]
+    o = oracle.tick(x, y, z, m, theta=2.0)
+    g = gpu_tick(x, y, z, m, theta=2.0, lanes=lanes)
+    assert_tick_parity(g, o)
+
+
 @pytest.mark.parametrize("theta", [0.5, 1.0, 3.0, 6.0])
 def test_other_theta(oracle, theta):
     # theta < ~1.04 reproduces the reference's depth-2 defect (a target can take

@@ -121,7 +121,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="plummer1m")
     ap.add_argument("--theta", type=float, default=2.0)
     ap.add_argument("--kahan", action="store_true")
-    ap.add_argument("--trav-lanes", type=int, default=32, choices=(16, 32, 64))
+    ap.add_argument("--trav-lanes", type=int, default=16, choices=(8, 16, 32, 64))
     ap.add_argument("--cpu-ticks", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")

@@ -130,7 +130,7 @@ struct ps_hip_engine {
     int trav = PS_HIP_TRAV_MT;
     int kahan = 0;
     int stats_on = 0;
-    int trav_lanes = 32;  // lanes per traversal entry
+    int trav_lanes = 16;  // lanes per traversal entry
 
     // host-pointer staging, one buffer per array (allocated on first use)
     DevBuf stage[9];
@@ -144,7 +144,7 @@ struct ps_hip_engine {
     // the far-mask arena fill over the whole tick
     DevBuf ent[2][4];
     DevBuf arena_b, masks_b;
-    DevBuf cnt_b, pos_b, scan_tmp, fin_b, fin_tmp, block_pairs_b;
+    DevBuf cnt_b, pos_b, scan_tmp, fin_b, fin_tmp, block_pairs_b, tstate_b;
     // small device / pinned host scalars
     DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs | u64 leaf pairs
     uint32_t* h_small = nullptr;  // pinned mirror
@@ -161,6 +161,7 @@ struct ps_hip_engine {
     uint32_t n_nodes = 0, max_depth = 0;
     ps_hip_stats_t st;
     hipEvent_t ev[12] = {nullptr};
+    hipEvent_t ev_it[4 * 16] = {nullptr};  // per traversal iteration: count and write
 
     uint32_t* small_u32() { return reinterpret_cast<uint32_t*>(small_b.p); }
     unsigned long long* totals() {
@@ -216,13 +217,15 @@ struct ps_hip_engine {
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
                          &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &cnt_b,     &pos_b,
-                         &scan_tmp,  &small_b, &fin_b, &fin_tmp, &block_pairs_b};
+                         &scan_tmp,  &small_b, &fin_b, &fin_tmp, &block_pairs_b, &tstate_b};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
             for (auto& b : es) release(b);
         if (h_small) (void)hipHostFree(h_small);
         h_small = nullptr;
         for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : ev_it)
             if (e) (void)hipEventDestroy(e);
         if (s) (void)hipStreamDestroy(s);
         s = nullptr;
@@ -308,28 +311,33 @@ int stage_upward(ps_hip_engine* e) {
     return map_err(hipGetLastError());
 }
 
-int stage_traverse(ps_hip_engine* e) {
-    float theta2;
-    {
-        volatile float t = e->theta;  // fl(theta * theta), H:1064
-        theta2 = t * t;
+// Upper bound on the entries of iteration k: MT targets at iteration k lie
+// at depths 1..k+2, ST targets at depths 0..k.
+uint32_t entries_bound(const ps_hip_engine* e, bool mt, int k) {
+    if (mt) {
+        const int d = std::min(k + 3, 11);
+        return e->ls[d] - e->ls[1];
     }
-    const bool mt = e->trav == PS_HIP_TRAV_MT && e->n_nodes > 1;
-    uint32_t n_ent = mt ? (e->ls[3] - e->ls[1]) : 1u;
-    int cur = 0;
-    for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[cur][k], 4 * (size_t)n_ent, e->s));
-    // arena word 0 = the list {root}; one mask word per initial entry
-    unsigned long long arena_used = 1, mask_used = n_ent;
-    PS_CK(grow_keep(e->arena_b, 4 * arena_used, 0, e->s));
-    PS_CK(grow_keep(e->masks_b, 8 * mask_used, 0, e->s));
-    ps_fill_u32(e->arena(), 0u, 1, e->s);
-    if (mt)
-        ps_launch_trav_init(e->ls[1], e->ls[3], e->ls[2], e->link(), e->entries(cur), e->s);
-    else
-        ps_launch_trav_init(0u, 1u, 0u, e->link(), e->entries(cur), e->s);
+    return e->ls[std::min(k + 1, 11)];
+}
 
+// Launch the whole traversal: every iteration is count -> scan -> plan ->
+// write with sizes in device memory, no host synchronisation in between.
+// An internal source at iteration k sits at depth k (each iteration expands
+// every near internal source), so a tree of depth D needs D + 1 iterations.
+int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
+    const uint32_t n_nodes = e->n_nodes;
+    PsTravState* st = reinterpret_cast<PsTravState*>(e->tstate_b.p);
+    const uint32_t arena_cap = (uint32_t)std::min<size_t>(e->arena_b.bytes / 4, 0xFFFFFFFFull);
+    const uint32_t mask_cap = (uint32_t)std::min<size_t>(e->masks_b.bytes / 8, 0xFFFFFFFFull);
+    if (mt)
+        ps_launch_trav_init(e->ls[1], e->ls[3], e->ls[2], e->link(), e->entries(0), st,
+                            e->arena(), arena_cap, mask_cap, e->s);
+    else
+        ps_launch_trav_init(0u, 1u, 0u, e->link(), e->entries(0), st, e->arena(), arena_cap,
+                            mask_cap, e->s);
     PsTravTree t;
-    t.n_nodes = e->n_nodes;
+    t.n_nodes = n_nodes;
     t.geom = e->geom();
     t.mp = e->mp();
     t.link = e->link();
@@ -338,67 +346,85 @@ int stage_traverse(ps_hip_engine* e) {
     t.p2p_off = e->p2p_off();
     t.p2p_len = e->p2p_len();
     t.p2p_moff = e->p2p_moff();
-    t.fin_leaves = nullptr;
-
-    unsigned long long m2l = 0, lists = 0, fin_leaves = 0;
-    uint32_t iters = 0;
-    while (n_ent > 0) {
-        const int nxt = cur ^ 1;
-        PS_CK(grow(e->cnt_b, 16 * (size_t)n_ent, e->s));
-        PS_CK(grow(e->pos_b, 16 * (size_t)n_ent, e->s));
-        PS_CK(grow(e->scan_tmp, ps_scan_tmp_bytes_u4(n_ent), e->s));
-        uint4* cnt = reinterpret_cast<uint4*>(e->cnt_b.p);
-        uint4* pos = reinterpret_cast<uint4*>(e->pos_b.p);
-        if (e->stats_on) {
-            PS_CK(grow(e->fin_b, 4 * (size_t)n_ent, e->s));
-            PS_CK(grow(e->fin_tmp, ps_scan_tmp_bytes_u32(n_ent), e->s));
-            t.fin_leaves = reinterpret_cast<uint32_t*>(e->fin_b.p);
-        }
-        e->record(8);
-        ps_launch_trav_count(t, theta2, e->entries(cur), e->arena(), e->masks(), n_ent, cnt,
+    t.fin_leaves = e->stats_on ? reinterpret_cast<uint32_t*>(e->fin_b.p) : nullptr;
+    uint4* cnt = reinterpret_cast<uint4*>(e->cnt_b.p);
+    uint4* pos = reinterpret_cast<uint4*>(e->pos_b.p);
+    for (int k = 0; k < iters; ++k) {
+        const int cur = k & 1, nxt = cur ^ 1;
+        const uint32_t ub = std::min(entries_bound(e, mt, k), n_nodes);
+        const uint32_t* n_dev = &st->n_ent[k];
+        if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 0], e->s);
+        ps_launch_trav_count(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub, cnt,
                              e->kahan, e->trav_lanes, e->s);
-        e->record(9);
-        PS_CK(ps_scan_u4(cnt, pos, n_ent, e->totals(), e->scan_tmp.p, e->s));
+        if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 1], e->s);
+        PS_CK(ps_scan_dev_u4(cnt, pos, n_dev, ub, st->totals, e->scan_tmp.p, e->s));
         if (e->stats_on)  // near leaves of the lists that became final (P2P leaf pairs)
-            PS_CK(ps_scan_u32(t.fin_leaves, t.fin_leaves, n_ent, e->leaf_pairs(), e->fin_tmp.p,
-                              e->s));
-        PS_CK(hipMemcpyAsync(e->h_small, e->totals(), 48, hipMemcpyDeviceToHost, e->s));
-        PS_CK(hipStreamSynchronize(e->s));
-        e->st.ms_trav_count += e->elapsed(8, 9);
-        if (iters > 0) e->st.ms_trav_write += e->elapsed(10, 11);
-        const unsigned long long* tot = reinterpret_cast<unsigned long long*>(e->h_small);
-        const unsigned long long list_n = tot[0], ent_n = tot[1], chunk_n = tot[2];
-        m2l += tot[3];
-        lists += list_n;
-        if (e->stats_on) fin_leaves += tot[5];
-        if (arena_used + list_n > 0xFFFFFFFFull || ent_n > 0xFFFFFFFFull ||
-            mask_used + chunk_n > 0xFFFFFFFFull)
-            return PS_HIP_EOOM;
-        // the arenas keep what earlier iterations wrote: P2P reads final lists
-        PS_CK(grow_keep(e->arena_b, 4 * (arena_used + list_n), 4 * arena_used, e->s));
-        PS_CK(grow_keep(e->masks_b, 8 * (mask_used + chunk_n), 8 * mask_used, e->s));
-        for (int k = 0; k < 4; ++k)
-            PS_CK(grow(e->ent[nxt][k], 4 * std::max<unsigned long long>(ent_n, 1), e->s));
-        e->record(10);
-        ps_launch_trav_write(t, e->entries(cur), e->arena(), e->masks(), n_ent, cnt, pos,
-                             (uint32_t)arena_used, (uint32_t)mask_used, e->entries(nxt),
-                             e->trav_lanes, e->s);
-        e->record(11);
-        arena_used += list_n;
-        mask_used += chunk_n;
-        n_ent = (uint32_t)ent_n;
-        cur = nxt;
-        ++iters;
+            PS_CK(ps_scan_dev_u32(t.fin_leaves, t.fin_leaves, n_dev, ub, &st->fin_total,
+                                  e->fin_tmp.p, e->s));
+        ps_launch_trav_plan(st, k, e->s);
+        if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 2], e->s);
+        ps_launch_trav_write(t, e->entries(cur), e->arena(), e->masks(), st, k, ub, cnt, pos,
+                             e->entries(nxt), e->trav_lanes, e->s);
+        if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 3], e->s);
     }
-    if (iters > 0 && e->stats_on) {
-        PS_CK(hipEventSynchronize(e->ev[11]));
-        e->st.ms_trav_write += e->elapsed(10, 11);
-    }
-    e->st.m2l_pairs = m2l;
-    e->st.p2p_leaf_pairs = fin_leaves;
-    e->st.list_entries = lists;
-    e->st.iterations = iters;
     return map_err(hipGetLastError());
+}
+
+int stage_traverse(ps_hip_engine* e) {
+    float theta2;
+    {
+        volatile float t = e->theta;  // fl(theta * theta), H:1064
+        theta2 = t * t;
+    }
+    const bool mt = e->trav == PS_HIP_TRAV_MT && e->n_nodes > 1;
+    const int iters = (int)e->max_depth + 1;
+    const size_t M = e->n_nodes;
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[b][k], 4 * M, e->s));
+    PS_CK(grow(e->cnt_b, 16 * M, e->s));
+    PS_CK(grow(e->pos_b, 16 * M, e->s));
+    PS_CK(grow(e->scan_tmp, ps_scan_dev_tmp_bytes_u4(M), e->s));
+    PS_CK(grow(e->tstate_b, sizeof(PsTravState), e->s));
+    if (e->stats_on) {
+        PS_CK(grow(e->fin_b, 4 * M, e->s));
+        PS_CK(grow(e->fin_tmp, ps_scan_dev_tmp_bytes_u32(M), e->s));
+    }
+    // first use: room for ~96 list words per particle; a larger need is met
+    // by growing and running the traversal again
+    const size_t n0 = std::max<size_t>(e->ls[3] - e->ls[1], 1) + 16;
+    PS_CK(grow(e->arena_b, 4 * std::max<size_t>(96 * e->n + 4096, 1 << 16), e->s));
+    PS_CK(grow(e->masks_b, 8 * std::max<size_t>(4 * M + n0 + 4096, n0), e->s));
+
+    PsTravState hst;
+    for (int attempt = 0;; ++attempt) {
+        PS_RC(launch_traverse(e, theta2, mt, iters));
+        PS_CK(hipMemcpyAsync(&hst, e->tstate_b.p, sizeof(hst), hipMemcpyDeviceToHost, e->s));
+        PS_CK(hipStreamSynchronize(e->s));
+        if (!hst.overflow) break;
+        if (attempt >= 4 || hst.arena_need > 0xFFFFFFFFull || hst.mask_need > 0xFFFFFFFFull)
+            return PS_HIP_EOOM;
+        // grow past what was asked for, clear the partial results, run again
+        PS_CK(grow(e->arena_b, std::max<size_t>(2 * e->arena_b.bytes, 5 * hst.arena_need), e->s));
+        PS_CK(grow(e->masks_b, std::max<size_t>(2 * e->masks_b.bytes, 10 * hst.mask_need), e->s));
+        ps_fill_u32(reinterpret_cast<uint32_t*>(e->loc()), 0u, 4 * (size_t)e->n_nodes, e->s);
+        ps_fill_u32(e->p2p_len(), 0u, e->n_nodes, e->s);
+    }
+    e->st.m2l_pairs = hst.m2l_pairs;
+    e->st.p2p_leaf_pairs = hst.fin_leaves;
+    e->st.list_entries = hst.list_entries;
+    e->st.iterations = iters;
+    if (e->stats_on) {
+        float c = 0.f, w = 0.f;
+        for (int k = 0; k < iters; ++k) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e->ev_it[4 * k], e->ev_it[4 * k + 1]) == hipSuccess) c += ms;
+            if (hipEventElapsedTime(&ms, e->ev_it[4 * k + 2], e->ev_it[4 * k + 3]) == hipSuccess)
+                w += ms;
+        }
+        e->st.ms_trav_count = c;
+        e->st.ms_trav_write = w;
+    }
+    return PS_HIP_OK;
 }
 
 int stage_l2l(ps_hip_engine* e) {
@@ -491,6 +517,7 @@ int ps_hip_engine_create(ps_hip_engine_t** out, size_t max_particles, size_t max
     hipError_t r = hipGetDevice(&e->device);
     if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking);
     for (int k = 0; k < 12 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev[k]);
+    for (int k = 0; k < 64 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev_it[k]);
     if (r == hipSuccess) r = alloc_fixed(e);
     if (r != hipSuccess) {
         (void)hipGetLastError();
@@ -527,7 +554,7 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
             e->stats_on = value != 0.0;
             return PS_HIP_OK;
         case PS_HIP_OPT_TRAV_LANES:
-            if (value != 16 && value != 32 && value != 64) return PS_HIP_EINVAL;
+            if (value != 8 && value != 16 && value != 32 && value != 64) return PS_HIP_EINVAL;
             e->trav_lanes = (int)value;
             return PS_HIP_OK;
         default:

@@ -16,6 +16,14 @@ hipError_t ps_scan_u32(const uint32_t* in, uint32_t* out, size_t n,
                        unsigned long long* total64, void* tmp, hipStream_t s);
 hipError_t ps_scan_u4(const uint4* in, uint4* out, size_t n, unsigned long long* total64,
                       void* tmp, hipStream_t s);
+// the same with the element count read from device memory at run time
+// (n_max: host-side upper bound that sizes the launch and tmp)
+size_t ps_scan_dev_tmp_bytes_u4(size_t n_max);
+size_t ps_scan_dev_tmp_bytes_u32(size_t n_max);
+hipError_t ps_scan_dev_u4(const uint4* in, uint4* out, const uint32_t* n_dev, size_t n_max,
+                          unsigned long long* total64, void* tmp, hipStream_t s);
+hipError_t ps_scan_dev_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev,
+                           size_t n_max, unsigned long long* total64, void* tmp, hipStream_t s);
 // graph-safe fills (no memset nodes)
 void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 // *out = sum of in[0..n) (one block, no atomics)
@@ -85,19 +93,41 @@ struct PsTravEntries {
     uint32_t* len;
     uint32_t* moff;
 };
-// iteration-0 entries: depth-1 leaves and all depth-2 nodes (MT) with list {root}
+// Device-resident bookkeeping of one traversal: every iteration is launched
+// without host synchronisation; sizes live here.
+struct PsTravState {
+    uint32_t n_ent[16];      // entries of iteration k (n_ent[0] from the init)
+    uint32_t list_base;      // arena offset where this iteration's write appends
+    uint32_t mask_base;      // mask offset of the next iteration's entries
+    uint32_t arena_used;     // arena words in use
+    uint32_t mask_used;      // mask words in use
+    uint32_t arena_cap;      // capacities of the arenas
+    uint32_t mask_cap;
+    uint32_t overflow;       // 1: a capacity was exceeded, the traversal is redone
+    uint32_t pad;
+    unsigned long long arena_need, mask_need;  // sizes asked for at the overflow
+    unsigned long long m2l_pairs, fin_leaves, list_entries;
+    unsigned long long totals[4];  // scan totals of the current iteration
+    unsigned long long fin_total;  // scan total of the final near-leaf counts
+};
+// iteration-0 entries: depth-1 leaves and all depth-2 nodes (MT) with list
+// {root}; resets the state (arena word 0 = root, one mask word per entry)
 void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* link,
-                         PsTravEntries ent, hipStream_t s);
-// MAC + M2L for every (entry target, listed source); far masks; per-entry
-// output sizes cnt = (next list length, next entries, next mask words, far).
-// lanes: lanes per entry (16, 32 or 64)
+                         PsTravEntries ent, PsTravState* st, uint32_t* arena,
+                         uint32_t arena_cap, uint32_t mask_cap, hipStream_t s);
+// MAC + M2L for every (entry target, listed source) of iteration k; far
+// masks; per-entry output sizes cnt = (next list length, next entries, next
+// mask words, far).  lanes: lanes per entry (8, 16, 32 or 64); max_ent bounds
+// the launch (the count itself is st->n_ent[k]).
 void ps_launch_trav_count(PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* arena,
-                          unsigned long long* masks, uint32_t n_ent, uint4* cnt, int kahan,
-                          int lanes, hipStream_t s);
-// next lists (appended to the arena at list_base) and next entries
+                          unsigned long long* masks, const PsTravState* st, int k,
+                          uint32_t max_ent, uint4* cnt, int kahan, int lanes, hipStream_t s);
+// after the scan of iteration k: next sizes, arena bases, capacity check
+void ps_launch_trav_plan(PsTravState* st, int k, hipStream_t s);
+// next lists (appended to the arena) and next entries
 void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
-                          const unsigned long long* masks, uint32_t n_ent, const uint4* cnt,
-                          const uint4* pos, uint32_t list_base, uint32_t mask_base,
+                          const unsigned long long* masks, const PsTravState* st, int k,
+                          uint32_t max_ent, const uint4* cnt, const uint4* pos,
                           PsTravEntries next, int lanes, hipStream_t s);
 
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------

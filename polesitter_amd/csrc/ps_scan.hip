@@ -74,6 +74,40 @@ __global__ __launch_bounds__(kBlock) void scan_reduce_kernel(const typename Ops:
     }
 }
 
+// device-side element count: tiles at or beyond *n_dev contribute zeros
+template <class Ops>
+__global__ __launch_bounds__(kBlock) void scan_reduce_dev_kernel(
+    const typename Ops::T* __restrict__ in, const uint32_t* __restrict__ n_dev,
+    typename Ops::T* bsum, unsigned long long* total) {
+    using T = typename Ops::T;
+    __shared__ T wsum[kWaves];
+    const size_t n = *n_dev;
+    const size_t base = (size_t)blockIdx.x * kTile;
+    if (base >= n) {
+        if (threadIdx.x == 0) bsum[blockIdx.x] = Ops::zero();
+        return;
+    }
+    T acc = Ops::zero();
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        size_t i = base + (size_t)k * kBlock + threadIdx.x;
+        if (i < n) acc = Ops::add(acc, in[i]);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) acc = Ops::add(acc, Ops::shfl_down(acc, d));
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wsum[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T s = Ops::zero();
+        for (int w = 0; w < kWaves; ++w) s = Ops::add(s, wsum[w]);
+        bsum[blockIdx.x] = s;
+        if (total) {
+            for (int c = 0; c < Ops::C; ++c) atomicAdd(&total[c], Ops::comp(s, c));
+        }
+    }
+}
+
 // rescan each tile from its offset
 template <class Ops>
 __global__ __launch_bounds__(kBlock) void scan_down_kernel(const typename Ops::T* in,
@@ -97,6 +131,57 @@ __global__ __launch_bounds__(kBlock) void scan_down_kernel(const typename Ops::T
         s = Ops::add(s, local[k]);
     }
     // inclusive wave scan of the per-thread sums
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    T inc = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = Ops::shfl_up(inc, d);
+        if (lane >= d) inc = Ops::add(inc, o);
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    T run = boff ? boff[blockIdx.x] : Ops::zero();
+    for (int w = 0; w < wave; ++w) run = Ops::add(run, wsum[w]);
+    T excl = Ops::shfl_up(inc, 1);
+    if (lane > 0) run = Ops::add(run, excl);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        tile[threadIdx.x * kItems + k] = run;
+        run = Ops::add(run, local[k]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        size_t i = base + (size_t)k * kBlock + threadIdx.x;
+        if (i < n) out[i] = tile[k * kBlock + threadIdx.x];
+    }
+}
+
+template <class Ops>
+__global__ __launch_bounds__(kBlock) void scan_down_dev_kernel(const typename Ops::T* in,
+                                                               typename Ops::T* out,
+                                                               const uint32_t* __restrict__ n_dev,
+                                                               const typename Ops::T* boff) {
+    using T = typename Ops::T;
+    __shared__ T tile[kTile];
+    __shared__ T wsum[kWaves];
+    const size_t n = *n_dev;
+    const size_t base = (size_t)blockIdx.x * kTile;
+    if (base >= n) return;  // block-uniform
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        size_t i = base + (size_t)k * kBlock + threadIdx.x;
+        tile[k * kBlock + threadIdx.x] = (i < n) ? in[i] : Ops::zero();
+    }
+    __syncthreads();
+    T local[kItems];
+    T s = Ops::zero();
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        local[k] = tile[threadIdx.x * kItems + k];
+        s = Ops::add(s, local[k]);
+    }
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     T inc = s;
@@ -188,7 +273,45 @@ hipError_t scan_impl(const typename Ops::T* in, typename Ops::T* out, size_t n,
     return hipGetLastError();
 }
 
+template <class Ops>
+hipError_t scan_dev_impl(const typename Ops::T* in, typename Ops::T* out, const uint32_t* n_dev,
+                         size_t n_max, unsigned long long* total, void* tmp, hipStream_t s) {
+    using T = typename Ops::T;
+    if (total) fill_u64_kernel<<<1, 64, 0, s>>>(total, Ops::C);
+    if (n_max == 0) return hipGetLastError();
+    size_t nb = (n_max + kTile - 1) / kTile;
+    T* bsum = reinterpret_cast<T*>(tmp);
+    T* boff = reinterpret_cast<T*>(reinterpret_cast<char*>(tmp) + align256(nb * sizeof(T)));
+    void* next = reinterpret_cast<char*>(tmp) + 2 * align256(nb * sizeof(T));
+    scan_reduce_dev_kernel<Ops><<<(unsigned)nb, kBlock, 0, s>>>(in, n_dev, bsum, total);
+    if (nb > 1) {
+        hipError_t e = scan_impl<Ops>(bsum, boff, nb, nullptr, next, s);
+        if (e != hipSuccess) return e;
+    }
+    scan_down_dev_kernel<Ops><<<(unsigned)nb, kBlock, 0, s>>>(in, out, n_dev,
+                                                               nb > 1 ? boff : nullptr);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+size_t ps_scan_dev_tmp_bytes_u4(size_t n_max) {
+    size_t nb = (n_max + kTile - 1) / kTile;
+    return 2 * align256(nb * sizeof(uint4)) + scan_tmp_bytes<OpsU4>(nb) + 256;
+}
+size_t ps_scan_dev_tmp_bytes_u32(size_t n_max) {
+    size_t nb = (n_max + kTile - 1) / kTile;
+    return 2 * align256(nb * sizeof(uint32_t)) + scan_tmp_bytes<OpsU32>(nb) + 256;
+}
+
+hipError_t ps_scan_dev_u4(const uint4* in, uint4* out, const uint32_t* n_dev, size_t n_max,
+                          unsigned long long* total64, void* tmp, hipStream_t s) {
+    return scan_dev_impl<OpsU4>(in, out, n_dev, n_max, total64, tmp, s);
+}
+hipError_t ps_scan_dev_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev,
+                           size_t n_max, unsigned long long* total64, void* tmp, hipStream_t s) {
+    return scan_dev_impl<OpsU32>(in, out, n_dev, n_max, total64, tmp, s);
+}
 
 size_t ps_scan_tmp_bytes_u32(size_t n) { return scan_tmp_bytes<OpsU32>(n); }
 size_t ps_scan_tmp_bytes_u4(size_t n) { return scan_tmp_bytes<OpsU4>(n); }

@@ -36,15 +36,54 @@ constexpr int kWavesPerBlock = kBlock / 64;
 __global__ __launch_bounds__(kBlock) void trav_init_kernel(uint32_t ls1, uint32_t ls2,
                                                            uint32_t ls3,
                                                            const uint2* __restrict__ link,
-                                                           PsTravEntries ent) {
+                                                           PsTravEntries ent, PsTravState* st,
+                                                           uint32_t* arena, uint32_t arena_cap,
+                                                           uint32_t mask_cap) {
     uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     uint32_t n = ls1 + k;
+    if (k == 0) {
+        PsTravState z = {};
+        z.n_ent[0] = ls3 - ls1;
+        z.arena_used = 1;  // arena word 0 holds the list {root}
+        z.mask_used = ls3 - ls1;
+        z.arena_cap = arena_cap;
+        z.mask_cap = mask_cap;
+        *st = z;
+        arena[0] = 0;
+    }
     if (n >= ls3) return;
     ent.tgt[k] = n;
-    ent.off[k] = 0;  // the list {root} at arena offset 0
+    ent.off[k] = 0;
     // jobs: every node at depth 2 and every leaf above it (H:1780)
     ent.len[k] = (n >= ls2 || (link[n].y & PS_LEAF)) ? 1u : 0u;
     ent.moff[k] = k;  // one mask word per entry
+}
+
+__global__ void trav_plan_kernel(PsTravState* st, int k) {
+    PsTravState& S = *st;
+    const unsigned long long list_n = S.totals[0], ent_n = S.totals[1], chunk_n = S.totals[2];
+    S.m2l_pairs += S.totals[3];
+    S.list_entries += list_n;
+    S.fin_leaves += S.fin_total;
+    S.fin_total = 0;
+    if (S.overflow) {
+        S.n_ent[k + 1] = 0;
+        return;
+    }
+    const unsigned long long na = (unsigned long long)S.arena_used + list_n;
+    const unsigned long long nm = (unsigned long long)S.mask_used + chunk_n;
+    if (na > S.arena_cap || nm > S.mask_cap) {
+        S.overflow = 1;
+        S.arena_need = na;
+        S.mask_need = nm;
+        S.n_ent[k + 1] = 0;
+        return;
+    }
+    S.list_base = S.arena_used;
+    S.mask_base = S.mask_used;
+    S.arena_used = (uint32_t)na;
+    S.mask_used = (uint32_t)nm;
+    S.n_ent[k + 1] = (uint32_t)ent_n;
 }
 
 __device__ __forceinline__ uint32_t chunks_of(uint32_t len) { return (len + 63u) >> 6; }
@@ -70,12 +109,14 @@ __device__ __forceinline__ unsigned long long group_bits(bool pred, uint32_t gsh
 template <bool kKahan, int G>
 __global__ __launch_bounds__(kBlock) void trav_count_kernel(
     PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* __restrict__ arena,
-    unsigned long long* __restrict__ masks, uint32_t n_ent, uint4* __restrict__ cnt) {
+    unsigned long long* __restrict__ masks, const PsTravState* __restrict__ st, int it,
+    uint4* __restrict__ cnt) {
     constexpr int kGroups = kBlock / G;
     const uint32_t e = blockIdx.x * kGroups + threadIdx.x / G;
     const uint32_t gl = threadIdx.x & (G - 1);
     const uint32_t gshift = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
-    if (e >= n_ent) return;  // uniform per group
+    if (e >= st->n_ent[it] || st->overflow) return;  // uniform per group
+    const uint32_t mask_cap = st->mask_cap;
     const uint32_t tn = ent.tgt[e];
     const uint32_t o = ent.off[e];
     const uint32_t L = ent.len[e];
@@ -126,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void trav_count_kernel(
             nchild += (uint32_t)__popcll(group_bits<G>((cc >> b) & 1u, gshift)) << b;
         word |= fb << (k0 & 63u);
         if (((k0 + G) & 63u) == 0 || k0 + G >= L) {
-            if (gl == 0) masks[mo + (k0 >> 6)] = word;
+            if (gl == 0 && mo + (k0 >> 6) < mask_cap) masks[mo + (k0 >> 6)] = word;
             word = 0;
         }
     }
@@ -178,12 +219,14 @@ __global__ __launch_bounds__(kBlock) void trav_count_kernel(
 template <int G>
 __global__ __launch_bounds__(kBlock) void trav_write_kernel(
     PsTravTree t, PsTravEntries ent, uint32_t* __restrict__ arena,
-    const unsigned long long* __restrict__ masks, uint32_t n_ent, const uint4* __restrict__ cnt,
-    const uint4* __restrict__ pos, uint32_t list_base, uint32_t mask_base, PsTravEntries next) {
+    const unsigned long long* __restrict__ masks, const PsTravState* __restrict__ st, int it,
+    const uint4* __restrict__ cnt, const uint4* __restrict__ pos, PsTravEntries next) {
     constexpr int kGroups = kBlock / G;
     const uint32_t e = blockIdx.x * kGroups + threadIdx.x / G;
     const uint32_t gl = threadIdx.x & (G - 1);
-    if (e >= n_ent) return;
+    if (e >= st->n_ent[it] || st->overflow) return;  // uniform per group
+    const uint32_t list_base = st->list_base;
+    const uint32_t mask_base = st->mask_base;
     const uint4 c = cnt[e];
     if (c.x == 0) return;  // final or empty: nothing to pass on (uniform per group)
     const uint4 p = pos[e];
@@ -245,41 +288,50 @@ __global__ __launch_bounds__(kBlock) void trav_write_kernel(
 }  // namespace
 
 void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* link,
-                         PsTravEntries ent, hipStream_t s) {
-    if (ls3 <= ls1) return;
-    uint32_t blocks = (ls3 - ls1 + kBlock - 1) / kBlock;
-    trav_init_kernel<<<blocks, kBlock, 0, s>>>(ls1, ls2, ls3, link, ent);
+                         PsTravEntries ent, PsTravState* st, uint32_t* arena,
+                         uint32_t arena_cap, uint32_t mask_cap, hipStream_t s) {
+    uint32_t n = ls3 > ls1 ? ls3 - ls1 : 1u;
+    uint32_t blocks = (n + kBlock - 1) / kBlock;
+    trav_init_kernel<<<blocks, kBlock, 0, s>>>(ls1, ls2, ls3, link, ent, st, arena, arena_cap,
+                                               mask_cap);
+}
+
+void ps_launch_trav_plan(PsTravState* st, int k, hipStream_t s) {
+    trav_plan_kernel<<<1, 1, 0, s>>>(st, k);
 }
 
 void ps_launch_trav_count(PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* arena,
-                          unsigned long long* masks, uint32_t n_ent, uint4* cnt, int kahan,
-                          int lanes, hipStream_t s) {
-    if (n_ent == 0) return;
+                          unsigned long long* masks, const PsTravState* st, int k,
+                          uint32_t max_ent, uint4* cnt, int kahan, int lanes, hipStream_t s) {
+    if (max_ent == 0) return;
 #define PS_COUNT(KH, G)                                                                       \
-    trav_count_kernel<KH, G><<<(n_ent + kBlock / G - 1) / (kBlock / G), kBlock, 0, s>>>(       \
-        t, theta2, ent, arena, masks, n_ent, cnt)
+    trav_count_kernel<KH, G><<<(max_ent + kBlock / G - 1) / (kBlock / G), kBlock, 0, s>>>(     \
+        t, theta2, ent, arena, masks, st, k, cnt)
     if (kahan) {
-        if (lanes == 16) PS_COUNT(true, 16);
+        if (lanes == 8) PS_COUNT(true, 8);
+        else if (lanes == 32) PS_COUNT(true, 32);
         else if (lanes == 64) PS_COUNT(true, 64);
-        else PS_COUNT(true, 32);
+        else PS_COUNT(true, 16);
     } else {
-        if (lanes == 16) PS_COUNT(false, 16);
+        if (lanes == 8) PS_COUNT(false, 8);
+        else if (lanes == 32) PS_COUNT(false, 32);
         else if (lanes == 64) PS_COUNT(false, 64);
-        else PS_COUNT(false, 32);
+        else PS_COUNT(false, 16);
     }
 #undef PS_COUNT
 }
 
 void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
-                          const unsigned long long* masks, uint32_t n_ent, const uint4* cnt,
-                          const uint4* pos, uint32_t list_base, uint32_t mask_base,
+                          const unsigned long long* masks, const PsTravState* st, int k,
+                          uint32_t max_ent, const uint4* cnt, const uint4* pos,
                           PsTravEntries next, int lanes, hipStream_t s) {
-    if (n_ent == 0) return;
+    if (max_ent == 0) return;
 #define PS_WRITE(G)                                                                           \
-    trav_write_kernel<G><<<(n_ent + kBlock / G - 1) / (kBlock / G), kBlock, 0, s>>>(           \
-        t, ent, arena, masks, n_ent, cnt, pos, list_base, mask_base, next)
-    if (lanes == 16) PS_WRITE(16);
+    trav_write_kernel<G><<<(max_ent + kBlock / G - 1) / (kBlock / G), kBlock, 0, s>>>(         \
+        t, ent, arena, masks, st, k, cnt, pos, next)
+    if (lanes == 8) PS_WRITE(8);
+    else if (lanes == 32) PS_WRITE(32);
     else if (lanes == 64) PS_WRITE(64);
-    else PS_WRITE(32);
+    else PS_WRITE(16);
 #undef PS_WRITE
 }

@@ -144,7 +144,7 @@ struct ps_hip_engine {
     // the far-mask arena fill over the whole tick
     DevBuf ent[2][4];
     DevBuf arena_b, masks_b;
-    DevBuf cnt_b, pos_b, scan_tmp, fin_b, fin_tmp, block_pairs_b, tstate_b;
+    DevBuf cnt_b, bsum_b, block_pairs_b, tstate_b;
     // small device / pinned host scalars
     DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs | u64 leaf pairs
     uint32_t* h_small = nullptr;  // pinned mirror
@@ -216,8 +216,8 @@ struct ps_hip_engine {
         for (auto& b : stage) release(b);
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
-                         &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &cnt_b,     &pos_b,
-                         &scan_tmp,  &small_b, &fin_b, &fin_tmp, &block_pairs_b, &tstate_b};
+                         &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &cnt_b,     &bsum_b,
+                         &small_b,   &block_pairs_b, &tstate_b};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
             for (auto& b : es) release(b);
@@ -321,8 +321,8 @@ uint32_t entries_bound(const ps_hip_engine* e, bool mt, int k) {
     return e->ls[std::min(k + 1, 11)];
 }
 
-// Launch the whole traversal: every iteration is count -> scan -> plan ->
-// write with sizes in device memory, no host synchronisation in between.
+// Launch the whole traversal: every iteration is count -> plan -> write
+// with sizes in device memory, no host synchronisation in between.
 // An internal source at iteration k sits at depth k (each iteration expands
 // every near internal source), so a tree of depth D needs D + 1 iterations.
 int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
@@ -346,24 +346,18 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     t.p2p_off = e->p2p_off();
     t.p2p_len = e->p2p_len();
     t.p2p_moff = e->p2p_moff();
-    t.fin_leaves = e->stats_on ? reinterpret_cast<uint32_t*>(e->fin_b.p) : nullptr;
     uint4* cnt = reinterpret_cast<uint4*>(e->cnt_b.p);
-    uint4* pos = reinterpret_cast<uint4*>(e->pos_b.p);
+    uint4* bsum = reinterpret_cast<uint4*>(e->bsum_b.p);
     for (int k = 0; k < iters; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
         const uint32_t ub = std::min(entries_bound(e, mt, k), n_nodes);
-        const uint32_t* n_dev = &st->n_ent[k];
         if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 0], e->s);
         ps_launch_trav_count(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub, cnt,
-                             e->kahan, e->trav_lanes, e->s);
+                             bsum, e->kahan, e->trav_lanes, e->s);
         if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 1], e->s);
-        PS_CK(ps_scan_dev_u4(cnt, pos, n_dev, ub, st->totals, e->scan_tmp.p, e->s));
-        if (e->stats_on)  // near leaves of the lists that became final (P2P leaf pairs)
-            PS_CK(ps_scan_dev_u32(t.fin_leaves, t.fin_leaves, n_dev, ub, &st->fin_total,
-                                  e->fin_tmp.p, e->s));
-        ps_launch_trav_plan(st, k, e->s);
+        ps_launch_trav_plan(st, k, bsum, e->trav_lanes, e->s);
         if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 2], e->s);
-        ps_launch_trav_write(t, e->entries(cur), e->arena(), e->masks(), st, k, ub, cnt, pos,
+        ps_launch_trav_write(t, e->entries(cur), e->arena(), e->masks(), st, k, ub, cnt, bsum,
                              e->entries(nxt), e->trav_lanes, e->s);
         if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 3], e->s);
     }
@@ -382,13 +376,8 @@ int stage_traverse(ps_hip_engine* e) {
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[b][k], 4 * M, e->s));
     PS_CK(grow(e->cnt_b, 16 * M, e->s));
-    PS_CK(grow(e->pos_b, 16 * M, e->s));
-    PS_CK(grow(e->scan_tmp, ps_scan_dev_tmp_bytes_u4(M), e->s));
+    PS_CK(grow(e->bsum_b, 16 * ((size_t)ps_trav_blocks((uint32_t)M, 64) + 1), e->s));
     PS_CK(grow(e->tstate_b, sizeof(PsTravState), e->s));
-    if (e->stats_on) {
-        PS_CK(grow(e->fin_b, 4 * M, e->s));
-        PS_CK(grow(e->fin_tmp, ps_scan_dev_tmp_bytes_u32(M), e->s));
-    }
     // first use: room for ~96 list words per particle; a larger need is met
     // by growing and running the traversal again
     const size_t n0 = std::max<size_t>(e->ls[3] - e->ls[1], 1) + 16;
@@ -410,7 +399,6 @@ int stage_traverse(ps_hip_engine* e) {
         ps_fill_u32(e->p2p_len(), 0u, e->n_nodes, e->s);
     }
     e->st.m2l_pairs = hst.m2l_pairs;
-    e->st.p2p_leaf_pairs = hst.fin_leaves;
     e->st.list_entries = hst.list_entries;
     e->st.iterations = iters;
     if (e->stats_on) {
@@ -437,7 +425,7 @@ int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
     unsigned long long* pc = nullptr;
     if (e->stats_on) {
         pc = e->pairs();
-        PS_CK(grow(e->block_pairs_b, 8 * (size_t)ps_p2p_blocks(e->n), e->s));
+        PS_CK(grow(e->block_pairs_b, 16 * (size_t)ps_p2p_blocks(e->n), e->s));
     }
     if (!e->arena_b.p) PS_CK(grow(e->arena_b, 256, e->s));
     if (!e->masks_b.p) PS_CK(grow(e->masks_b, 256, e->s));
@@ -647,7 +635,7 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
         if (host[k] != dev[k])
             PS_CK(hipMemcpyAsync(host[k], dev[k], n * 4, hipMemcpyDeviceToHost, e->s));
     if (e->stats_on)
-        PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 8, hipMemcpyDeviceToHost, e->s));
+        PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 16, hipMemcpyDeviceToHost, e->s));
     PS_CK(hipEventRecord(e->ev[7], e->s));
     PS_CK(hipStreamSynchronize(e->s));
     e->st.n_particles = n;
@@ -656,6 +644,7 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
     std::memcpy(e->st.level_start, e->ls, sizeof(e->ls));
     if (e->stats_on) {
         e->st.p2p_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 32);
+        e->st.p2p_leaf_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 34);
         e->st.ms_sort = e->elapsed(1, 2);
         e->st.ms_build = e->elapsed(2, 3);
         e->st.ms_upward = e->elapsed(3, 4);

@@ -83,7 +83,6 @@ struct PsTravTree {
     uint32_t* p2p_off;   // per leaf: arena offset of its final source list
     uint32_t* p2p_len;   // per leaf: length of that list (0: no near field)
     uint32_t* p2p_moff;  // per leaf: offset of the list's far masks
-    uint32_t* fin_leaves;  // stats: per entry, near leaves of a final list (may be null)
 };
 // One traversal iteration's entries (target, list offset and length in the
 // list arena, offset of its far masks in the mask arena).
@@ -106,9 +105,7 @@ struct PsTravState {
     uint32_t overflow;       // 1: a capacity was exceeded, the traversal is redone
     uint32_t pad;
     unsigned long long arena_need, mask_need;  // sizes asked for at the overflow
-    unsigned long long m2l_pairs, fin_leaves, list_entries;
-    unsigned long long totals[4];  // scan totals of the current iteration
-    unsigned long long fin_total;  // scan total of the final near-leaf counts
+    unsigned long long m2l_pairs, list_entries;
 };
 // iteration-0 entries: depth-1 leaves and all depth-2 nodes (MT) with list
 // {root}; resets the state (arena word 0 = root, one mask word per entry)
@@ -117,17 +114,21 @@ void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* 
                          uint32_t arena_cap, uint32_t mask_cap, hipStream_t s);
 // MAC + M2L for every (entry target, listed source) of iteration k; far
 // masks; per-entry output sizes cnt = (next list length, next entries, next
-// mask words, far).  lanes: lanes per entry (8, 16, 32 or 64); max_ent bounds
-// the launch (the count itself is st->n_ent[k]).
+// mask words, far) and their per-block sums.  lanes: lanes per entry (8, 16,
+// 32 or 64); max_ent bounds the launch (the count itself is st->n_ent[k]).
 void ps_launch_trav_count(PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* arena,
                           unsigned long long* masks, const PsTravState* st, int k,
-                          uint32_t max_ent, uint4* cnt, int kahan, int lanes, hipStream_t s);
-// after the scan of iteration k: next sizes, arena bases, capacity check
-void ps_launch_trav_plan(PsTravState* st, int k, hipStream_t s);
+                          uint32_t max_ent, uint4* cnt, uint4* bsum, int kahan, int lanes,
+                          hipStream_t s);
+// blocks of the count / write launches for max_ent entries (bsum length)
+uint32_t ps_trav_blocks(uint32_t max_ent, int lanes);
+// after the count of iteration k (same lanes): scan of the block sums that
+// held entries, next sizes, arena bases, capacity check (one block)
+void ps_launch_trav_plan(PsTravState* st, int k, uint4* bsum, int lanes, hipStream_t s);
 // next lists (appended to the arena) and next entries
 void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
                           const unsigned long long* masks, const PsTravState* st, int k,
-                          uint32_t max_ent, const uint4* cnt, const uint4* pos,
+                          uint32_t max_ent, const uint4* cnt, const uint4* bsum,
                           PsTravEntries next, int lanes, hipStream_t s);
 
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------
@@ -135,7 +136,8 @@ void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
 // 2: f += m*sum (P2P only).  A leaf's near sources are the non-far entries of
 // its final traversal list.
 // pair_count (stats, may be null): per-block partial sums in block_pairs
-// (>= ps_p2p_blocks(n) entries), reduced into *pair_count.
+// (>= 2 * ps_p2p_blocks(n) entries), reduced into pair_count[0] (particle
+// pairs) and pair_count[1] (target leaf x source leaf pairs).
 uint32_t ps_p2p_blocks(size_t n);
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,

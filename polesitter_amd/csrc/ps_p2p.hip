@@ -23,12 +23,13 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
     const uint32_t* __restrict__ p2p_len, const uint32_t* __restrict__ p2p_moff,
     const uint32_t* __restrict__ arena, const unsigned long long* __restrict__ masks,
     const uint2* __restrict__ link, const uint32_t* __restrict__ nend, float* __restrict__ fx,
-    float* __restrict__ fy, float* __restrict__ fz, unsigned long long* block_pairs) {
+    float* __restrict__ fy, float* __restrict__ fz, unsigned long long* block_pairs,
+    uint32_t nblocks) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < n;
     float ax = 0.f, ay = 0.f, az = 0.f;
     float cx = 0.f, cy = 0.f, cz = 0.f;
-    uint32_t pairs = 0;
+    uint32_t pairs = 0, nsrc = 0;  // particle pairs; near source leaves
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t lf = 0;
     if (valid) {
@@ -44,6 +45,7 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
                 while (near) {
                     const uint32_t b = (uint32_t)__ffsll((long long)near) - 1u;
                     near &= near - 1ull;
+                    ++nsrc;
                     const uint32_t s = arena[o + k0 + b];
                     const uint32_t j0 = link[s].x;
                     const uint32_t j1e = nend[s];
@@ -66,16 +68,28 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
         }
     }
     if (block_pairs) {  // per-block partial sums, reduced by another kernel
-        __shared__ unsigned long long wsum[kBlock / 64];
-        unsigned long long v = pairs;
+        // leaf pairs: the first particle of each leaf counts its near sources
+        if (valid && i != link[lf].x) nsrc = 0;
+        __shared__ unsigned long long wsum[2][kBlock / 64];
+        unsigned long long v = pairs, u = nsrc;
 #pragma unroll
-        for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
-        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+        for (int d = 32; d > 0; d >>= 1) {
+            v += __shfl_down(v, d, 64);
+            u += __shfl_down(u, d, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            wsum[0][threadIdx.x >> 6] = v;
+            wsum[1][threadIdx.x >> 6] = u;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            unsigned long long b = 0;
-            for (int w = 0; w < kBlock / 64; ++w) b += wsum[w];
+            unsigned long long b = 0, c = 0;
+            for (int w = 0; w < kBlock / 64; ++w) {
+                b += wsum[0][w];
+                c += wsum[1][w];
+            }
             block_pairs[blockIdx.x] = b;
+            block_pairs[nblocks + blockIdx.x] = c;
         }
     }
     if (!valid) return;
@@ -109,11 +123,11 @@ void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const fl
     if (kahan)
         p2p_kernel<kMode, true><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
                                                           p2p_len, p2p_moff, arena, masks, link,
-                                                          nend, fx, fy, fz, bp);
+                                                          nend, fx, fy, fz, bp, blocks);
     else
         p2p_kernel<kMode, false><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
                                                            p2p_len, p2p_moff, arena, masks, link,
-                                                           nend, fx, fy, fz, bp);
+                                                           nend, fx, fy, fz, bp, blocks);
 }
 
 }  // namespace
@@ -137,5 +151,9 @@ void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const 
     else
         launch_mode<2>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
                        nend, fx, fy, fz, kahan, bp, s);
-    if (bp) ps_reduce_u64(bp, ps_p2p_blocks(n), pair_count, s);
+    if (bp) {
+        const uint32_t nb = ps_p2p_blocks(n);
+        ps_reduce_u64(bp, nb, pair_count, s);
+        ps_reduce_u64(bp + nb, nb, pair_count + 1, s);
+    }
 }

@@ -47,7 +47,7 @@ extern "C" {
 #define PS_HIP_OPT_KAHAN     2 /* 1: compensated far-field and P2P sums */
 #define PS_HIP_OPT_THETA     3 /* MAC theta, like ps_config_t.theta */
 #define PS_HIP_OPT_STATS     4 /* 1: count P2P pairs and time each stage */
-#define PS_HIP_OPT_TRAV_LANES 5 /* lanes per traversal entry: 16, 32 (default), 64 */
+#define PS_HIP_OPT_TRAV_LANES 5 /* lanes per traversal entry: 8 (default), 16, 32, 64 */
 
 typedef struct ps_hip_engine ps_hip_engine_t;
 
@@ -64,10 +64,10 @@ typedef struct {
     uint64_t list_entries;    /* traversal list entries written */
     /* milliseconds per stage of the last tick, HIP events on the engine's
      * stream (stats on); ms_p2p brackets the single P2P kernel launch,
-     * ms_trav_count / ms_trav_write sum the traversal kernels' launches */
+     * ms_trav_steps brackets the traversal's iteration kernels */
     float ms_upload, ms_sort, ms_build, ms_upward, ms_traverse, ms_l2l, ms_p2p,
         ms_download, ms_total;
-    float ms_trav_count, ms_trav_write;
+    float ms_trav_steps;
 } ps_hip_stats_t;
 
 /* ps_init, H:1699-1767: device state for up to max_particles particles and

@@ -61,7 +61,7 @@ class PsStats(C.Structure):
                 ("ms_upload", C.c_float), ("ms_sort", C.c_float), ("ms_build", C.c_float),
                 ("ms_upward", C.c_float), ("ms_traverse", C.c_float), ("ms_l2l", C.c_float),
                 ("ms_p2p", C.c_float), ("ms_download", C.c_float), ("ms_total", C.c_float),
-                ("ms_trav_count", C.c_float), ("ms_trav_write", C.c_float)]
+                ("ms_trav_steps", C.c_float)]
 
     def as_dict(self):
         d = {}

@@ -130,7 +130,7 @@ struct ps_hip_engine {
     int trav = PS_HIP_TRAV_MT;
     int kahan = 0;
     int stats_on = 0;
-    int trav_lanes = 16;  // lanes per traversal entry
+    int trav_lanes = 8;  // lanes per traversal entry
 
     // host-pointer staging, one buffer per array (allocated on first use)
     DevBuf stage[9];
@@ -144,7 +144,7 @@ struct ps_hip_engine {
     // the far-mask arena fill over the whole tick
     DevBuf ent[2][4];
     DevBuf arena_b, masks_b;
-    DevBuf cnt_b, bsum_b, block_pairs_b, tstate_b;
+    DevBuf far_b, block_pairs_b, tstate_b;
     // small device / pinned host scalars
     DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs | u64 leaf pairs
     uint32_t* h_small = nullptr;  // pinned mirror
@@ -216,7 +216,7 @@ struct ps_hip_engine {
         for (auto& b : stage) release(b);
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
-                         &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &cnt_b,     &bsum_b,
+                         &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,
                          &small_b,   &block_pairs_b, &tstate_b};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
@@ -321,10 +321,10 @@ uint32_t entries_bound(const ps_hip_engine* e, bool mt, int k) {
     return e->ls[std::min(k + 1, 11)];
 }
 
-// Launch the whole traversal: every iteration is count -> plan -> write
-// with sizes in device memory, no host synchronisation in between.
-// An internal source at iteration k sits at depth k (each iteration expands
-// every near internal source), so a tree of depth D needs D + 1 iterations.
+// Launch the whole traversal: one kernel per iteration, sizes in device
+// memory, no host synchronisation in between.  An internal source at
+// iteration k sits at depth k (each iteration expands every near internal
+// source), so a tree of depth D needs D + 1 iterations.
 int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     const uint32_t n_nodes = e->n_nodes;
     PsTravState* st = reinterpret_cast<PsTravState*>(e->tstate_b.p);
@@ -346,21 +346,18 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     t.p2p_off = e->p2p_off();
     t.p2p_len = e->p2p_len();
     t.p2p_moff = e->p2p_moff();
-    uint4* cnt = reinterpret_cast<uint4*>(e->cnt_b.p);
-    uint4* bsum = reinterpret_cast<uint4*>(e->bsum_b.p);
+    unsigned long long* far_part = reinterpret_cast<unsigned long long*>(e->far_b.p);
+    size_t far_n = 0;
+    if (e->stats_on) (void)hipEventRecord(e->ev_it[0], e->s);
     for (int k = 0; k < iters; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
         const uint32_t ub = std::min(entries_bound(e, mt, k), n_nodes);
-        if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 0], e->s);
-        ps_launch_trav_count(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub, cnt,
-                             bsum, e->kahan, e->trav_lanes, e->s);
-        if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 1], e->s);
-        ps_launch_trav_plan(st, k, bsum, e->trav_lanes, e->s);
-        if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 2], e->s);
-        ps_launch_trav_write(t, e->entries(cur), e->arena(), e->masks(), st, k, ub, cnt, bsum,
-                             e->entries(nxt), e->trav_lanes, e->s);
-        if (e->stats_on) (void)hipEventRecord(e->ev_it[4 * k + 3], e->s);
+        ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
+                            e->entries(nxt), far_part + far_n, e->kahan, e->trav_lanes, e->s);
+        far_n += ps_trav_blocks(ub, e->trav_lanes);
     }
+    if (e->stats_on) (void)hipEventRecord(e->ev_it[1], e->s);
+    ps_reduce_u64(far_part, far_n, &st->m2l_pairs, e->s);
     return map_err(hipGetLastError());
 }
 
@@ -375,8 +372,7 @@ int stage_traverse(ps_hip_engine* e) {
     const size_t M = e->n_nodes;
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[b][k], 4 * M, e->s));
-    PS_CK(grow(e->cnt_b, 16 * M, e->s));
-    PS_CK(grow(e->bsum_b, 16 * ((size_t)ps_trav_blocks((uint32_t)M, 64) + 1), e->s));
+    PS_CK(grow(e->far_b, 8 * (size_t)ps_trav_blocks(0xFFFFFFFFu, 8) * (size_t)iters, e->s));
     PS_CK(grow(e->tstate_b, sizeof(PsTravState), e->s));
     // first use: room for ~96 list words per particle; a larger need is met
     // by growing and running the traversal again
@@ -390,27 +386,21 @@ int stage_traverse(ps_hip_engine* e) {
         PS_CK(hipMemcpyAsync(&hst, e->tstate_b.p, sizeof(hst), hipMemcpyDeviceToHost, e->s));
         PS_CK(hipStreamSynchronize(e->s));
         if (!hst.overflow) break;
-        if (attempt >= 4 || hst.arena_need > 0xFFFFFFFFull || hst.mask_need > 0xFFFFFFFFull)
+        if (attempt >= 4 || hst.arena_ctr > 0xFFFFFFFFull || hst.mask_ctr > 0xFFFFFFFFull)
             return PS_HIP_EOOM;
         // grow past what was asked for, clear the partial results, run again
-        PS_CK(grow(e->arena_b, std::max<size_t>(2 * e->arena_b.bytes, 5 * hst.arena_need), e->s));
-        PS_CK(grow(e->masks_b, std::max<size_t>(2 * e->masks_b.bytes, 10 * hst.mask_need), e->s));
+        PS_CK(grow(e->arena_b, std::max<size_t>(2 * e->arena_b.bytes, 5 * hst.arena_ctr), e->s));
+        PS_CK(grow(e->masks_b, std::max<size_t>(2 * e->masks_b.bytes, 10 * hst.mask_ctr), e->s));
         ps_fill_u32(reinterpret_cast<uint32_t*>(e->loc()), 0u, 4 * (size_t)e->n_nodes, e->s);
         ps_fill_u32(e->p2p_len(), 0u, e->n_nodes, e->s);
     }
     e->st.m2l_pairs = hst.m2l_pairs;
-    e->st.list_entries = hst.list_entries;
+    e->st.list_entries = hst.arena_ctr - 1;
     e->st.iterations = iters;
     if (e->stats_on) {
-        float c = 0.f, w = 0.f;
-        for (int k = 0; k < iters; ++k) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, e->ev_it[4 * k], e->ev_it[4 * k + 1]) == hipSuccess) c += ms;
-            if (hipEventElapsedTime(&ms, e->ev_it[4 * k + 2], e->ev_it[4 * k + 3]) == hipSuccess)
-                w += ms;
-        }
-        e->st.ms_trav_count = c;
-        e->st.ms_trav_write = w;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e->ev_it[0], e->ev_it[1]) == hipSuccess)
+            e->st.ms_trav_steps = ms;
     }
     return PS_HIP_OK;
 }

@@ -95,41 +95,32 @@ struct PsTravEntries {
 // Device-resident bookkeeping of one traversal: every iteration is launched
 // without host synchronisation; sizes live here.
 struct PsTravState {
-    uint32_t n_ent[16];      // entries of iteration k (n_ent[0] from the init)
-    uint32_t list_base;      // arena offset where this iteration's write appends
-    uint32_t mask_base;      // mask offset of the next iteration's entries
-    uint32_t arena_used;     // arena words in use
-    uint32_t mask_used;      // mask words in use
-    uint32_t arena_cap;      // capacities of the arenas
+    uint32_t n_ent[16];            // entries of iteration k (n_ent[0] from the init)
+    uint32_t arena_cap;            // capacities of the arenas (words)
     uint32_t mask_cap;
-    uint32_t overflow;       // 1: a capacity was exceeded, the traversal is redone
+    uint32_t overflow;             // 1: a capacity was exceeded, the traversal is redone
     uint32_t pad;
-    unsigned long long arena_need, mask_need;  // sizes asked for at the overflow
-    unsigned long long m2l_pairs, list_entries;
+    unsigned long long arena_ctr;  // list words reserved (or asked for, at an overflow)
+    unsigned long long mask_ctr;   // mask words reserved (or asked for)
+    unsigned long long m2l_pairs;  // filled by a reduction of the per-block far counts
 };
 // iteration-0 entries: depth-1 leaves and all depth-2 nodes (MT) with list
 // {root}; resets the state (arena word 0 = root, one mask word per entry)
 void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* link,
                          PsTravEntries ent, PsTravState* st, uint32_t* arena,
                          uint32_t arena_cap, uint32_t mask_cap, hipStream_t s);
-// MAC + M2L for every (entry target, listed source) of iteration k; far
-// masks; per-entry output sizes cnt = (next list length, next entries, next
-// mask words, far) and their per-block sums.  lanes: lanes per entry (8, 16,
-// 32 or 64); max_ent bounds the launch (the count itself is st->n_ent[k]).
-void ps_launch_trav_count(PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* arena,
-                          unsigned long long* masks, const PsTravState* st, int k,
-                          uint32_t max_ent, uint4* cnt, uint4* bsum, int kahan, int lanes,
-                          hipStream_t s);
-// blocks of the count / write launches for max_ent entries (bsum length)
+// Blocks of the iteration launch for max_ent entries (length of its far_part
+// slice; never more than ps_trav_blocks(UINT32_MAX, 8)).
 uint32_t ps_trav_blocks(uint32_t max_ent, int lanes);
-// after the count of iteration k (same lanes): scan of the block sums that
-// held entries, next sizes, arena bases, capacity check (one block)
-void ps_launch_trav_plan(PsTravState* st, int k, uint4* bsum, int lanes, hipStream_t s);
-// next lists (appended to the arena) and next entries
-void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
-                          const unsigned long long* masks, const PsTravState* st, int k,
-                          uint32_t max_ent, const uint4* cnt, const uint4* bsum,
-                          PsTravEntries next, int lanes, hipStream_t s);
+// Iteration k: MAC + M2L for every (entry target, listed source), far masks,
+// P2P lists of final leaves, next lists appended to the arena and next
+// entries (count in st->n_ent[k + 1]); far pairs per block into far_part.
+// lanes: lanes per entry (8, 16, 32 or 64); max_ent bounds the launch (the
+// count itself is st->n_ent[k]).
+void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t* arena,
+                         unsigned long long* masks, PsTravState* st, int k, uint32_t max_ent,
+                         PsTravEntries next, unsigned long long* far_part, int kahan, int lanes,
+                         hipStream_t s);
 
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------
 // mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);

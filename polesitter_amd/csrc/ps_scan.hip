@@ -215,18 +215,32 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void reduce_u64_kernel(const unsigned long long* in,
-                                                             size_t n, unsigned long long* out) {
-    __shared__ unsigned long long ws[kWaves];
+// one block; each thread issues kRedV independent loads per round
+constexpr int kRedThreads = 1024;
+constexpr int kRedV = 8;
+
+__global__ __launch_bounds__(kRedThreads) void reduce_u64_kernel(const unsigned long long* in,
+                                                                 size_t n,
+                                                                 unsigned long long* out) {
+    __shared__ unsigned long long ws[kRedThreads / 64];
     unsigned long long v = 0;
-    for (size_t i = threadIdx.x; i < n; i += kBlock) v += in[i];
+    for (size_t base = 0; base < n; base += (size_t)kRedThreads * kRedV) {
+        unsigned long long t[kRedV];
+#pragma unroll
+        for (int j = 0; j < kRedV; ++j) {
+            const size_t i = base + (size_t)j * kRedThreads + threadIdx.x;
+            t[j] = i < n ? in[i] : 0ull;
+        }
+#pragma unroll
+        for (int j = 0; j < kRedV; ++j) v += t[j];
+    }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long s = 0;
-        for (int w = 0; w < kWaves; ++w) s += ws[w];
+        for (int w = 0; w < kRedThreads / 64; ++w) s += ws[w];
         *out = s;
     }
 }
@@ -335,5 +349,5 @@ void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 
 void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* out,
                    hipStream_t s) {
-    reduce_u64_kernel<<<1, kBlock, 0, s>>>(in, n, out);
+    reduce_u64_kernel<<<1, kRedThreads, 0, s>>>(in, n, out);
 }

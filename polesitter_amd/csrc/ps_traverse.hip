@@ -15,17 +15,24 @@
 // This visits exactly the reference's pairs (the pair set, not the order),
 // and every target owns its accumulations: no atomics on the results.
 //
-// One iteration = count -> plan -> write, a group of G lanes (8 to 64;
-// several groups per wavefront) per entry:
+// One iteration = one kernel, a group of G lanes (8 to 64; several groups per
+// wavefront) per entry.  The grid is fixed (at most kMaxBlocks blocks); block
+// b owns one contiguous range of the iteration's entries, sized on the device
+// from the entry count, and takes it in LDS segments of kSeg entries:
 //   count  MAC for each listed source (the group takes the list in strides),
 //          M2L for far ones, a 64-bit far mask per 64 list entries, the
-//          output sizes and their per-block sums; a final leaf records
-//          (list, masks) for P2P.
-//   plan   one block scans the block sums and does the bookkeeping (next
-//          entry count, arena bases, capacity check).
-//   write  for continuing entries only: the mask picks the near sources,
-//          near leaves are copied and near internal nodes expanded to their
-//          children, compacted with a wave prefix sum into the list arena.
+//          entry's output sizes into LDS; a final leaf records (list, masks)
+//          for P2P.
+//   place  block prefix sum of the sizes; one atomic add per counter (list
+//          words, next entries, mask words) reserves the segment's output,
+//          checked against the arena capacities.
+//   write  for continuing entries only, while the list is still in cache:
+//          the mask picks the near sources, near leaves are copied and near
+//          internal nodes expanded to their children, compacted with a group
+//          prefix sum into the reserved list space; next entries likewise.
+// Where a segment's output lands depends on the order of the atomics; what
+// each entry computes does not (its list order is fixed by its producer), so
+// the results are deterministic.
 // Lists are appended to one arena that lives for the whole tick, so the P2P
 // kernel reads each leaf's final list in place.
 #include "ps_device.h"
@@ -33,8 +40,18 @@
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kBlock = 256;            // init kernel
+constexpr int kStep = 1024;            // threads of an iteration block
+constexpr uint32_t kMaxBlocks = 512;   // grid cap of an iteration (2 blocks per CU)
+constexpr uint32_t kSeg = 2048;        // entries per LDS segment (one reservation each)
+
+// entries per block range: a multiple of the groups per block, the ranges
+// of nblk blocks covering n_e
+__device__ __forceinline__ uint32_t trav_range(uint32_t n_e, uint32_t nblk, uint32_t groups) {
+    uint32_t r = (uint32_t)(((unsigned long long)n_e + nblk - 1) / nblk);
+    r = (r + groups - 1) / groups * groups;
+    return r > groups ? r : groups;
+}
 
 __global__ __launch_bounds__(kBlock) void trav_init_kernel(uint32_t ls1, uint32_t ls2,
                                                            uint32_t ls3,
@@ -47,8 +64,8 @@ __global__ __launch_bounds__(kBlock) void trav_init_kernel(uint32_t ls1, uint32_
     if (k == 0) {
         PsTravState z = {};
         z.n_ent[0] = ls3 - ls1;
-        z.arena_used = 1;  // arena word 0 holds the list {root}
-        z.mask_used = ls3 - ls1;
+        z.arena_ctr = 1;  // arena word 0 holds the list {root}
+        z.mask_ctr = ls3 - ls1;
         z.arena_cap = arena_cap;
         z.mask_cap = mask_cap;
         *st = z;
@@ -60,115 +77,6 @@ __global__ __launch_bounds__(kBlock) void trav_init_kernel(uint32_t ls1, uint32_
     // jobs: every node at depth 2 and every leaf above it (H:1780)
     ent.len[k] = (n >= ls2 || (link[n].y & PS_LEAF)) ? 1u : 0u;
     ent.moff[k] = k;  // one mask word per entry
-}
-
-// One block: exclusive scan of the count kernel's per-block sums (in place),
-// then the iteration's bookkeeping -- next entry count, the arena bases of
-// the write, and the capacity check.  Only the blocks that held entries are
-// scanned (groups = entries per count block); each thread takes kPlanV
-// consecutive sums per round, loaded together.
-constexpr int kPlanThreads = 1024;
-constexpr int kPlanV = 8;
-
-__global__ __launch_bounds__(kPlanThreads) void trav_plan_kernel(PsTravState* st, int k,
-                                                                 uint4* __restrict__ bsum,
-                                                                 uint32_t groups) {
-    __shared__ uint4 wtot[kPlanThreads / 64];
-    __shared__ unsigned long long wt[4][kPlanThreads / 64];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t n_e = st->n_ent[k];
-    const uint32_t nb = st->overflow ? 0u : (uint32_t)(((unsigned long long)n_e + groups - 1) / groups);
-    uint4 carry = make_uint4(0u, 0u, 0u, 0u);  // same value in every thread
-    // exact 64-bit totals (the 32-bit running offsets can wrap on huge inputs;
-    // the capacity check then fails on the totals)
-    unsigned long long t_list = 0, t_ent = 0, t_chunk = 0, t_far = 0;
-    for (uint32_t base = 0; base < nb; base += kPlanThreads * kPlanV) {
-        const uint32_t i0 = base + tid * kPlanV;
-        uint4 v[kPlanV];
-#pragma unroll
-        for (int j = 0; j < kPlanV; ++j)
-            v[j] = (i0 + j < nb) ? bsum[i0 + j] : make_uint4(0u, 0u, 0u, 0u);
-        uint4 sum = make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-        for (int j = 0; j < kPlanV; ++j) {
-            sum.x += v[j].x; sum.y += v[j].y; sum.z += v[j].z; sum.w += v[j].w;
-            t_list += v[j].x; t_ent += v[j].y; t_chunk += v[j].z; t_far += v[j].w;
-        }
-        uint4 inc = sum;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint4 o;
-            o.x = __shfl_up(inc.x, d, 64);
-            o.y = __shfl_up(inc.y, d, 64);
-            o.z = __shfl_up(inc.z, d, 64);
-            o.w = __shfl_up(inc.w, d, 64);
-            if (lane >= (uint32_t)d) {
-                inc.x += o.x; inc.y += o.y; inc.z += o.z; inc.w += o.w;
-            }
-        }
-        if (lane == 63) wtot[wave] = inc;
-        __syncthreads();
-        uint4 run = carry, all = carry;
-        for (uint32_t w = 0; w < kPlanThreads / 64; ++w) {
-            const uint4 t = wtot[w];
-            if (w < wave) {
-                run.x += t.x; run.y += t.y; run.z += t.z; run.w += t.w;
-            }
-            all.x += t.x; all.y += t.y; all.z += t.z; all.w += t.w;
-        }
-        run.x += inc.x - sum.x; run.y += inc.y - sum.y;
-        run.z += inc.z - sum.z; run.w += inc.w - sum.w;
-#pragma unroll
-        for (int j = 0; j < kPlanV; ++j) {
-            if (i0 + j < nb) bsum[i0 + j] = run;
-            run.x += v[j].x; run.y += v[j].y; run.z += v[j].z; run.w += v[j].w;
-        }
-        carry = all;
-        __syncthreads();  // wtot is rewritten by the next round
-    }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        t_list += __shfl_down(t_list, d, 64);
-        t_ent += __shfl_down(t_ent, d, 64);
-        t_chunk += __shfl_down(t_chunk, d, 64);
-        t_far += __shfl_down(t_far, d, 64);
-    }
-    if (lane == 0) {
-        wt[0][wave] = t_list;
-        wt[1][wave] = t_ent;
-        wt[2][wave] = t_chunk;
-        wt[3][wave] = t_far;
-    }
-    __syncthreads();
-    if (tid != 0) return;
-    unsigned long long list_n = 0, ent_n = 0, chunk_n = 0, far_total = 0;
-    for (int w = 0; w < kPlanThreads / 64; ++w) {
-        list_n += wt[0][w];
-        ent_n += wt[1][w];
-        chunk_n += wt[2][w];
-        far_total += wt[3][w];
-    }
-    PsTravState& S = *st;
-    S.m2l_pairs += far_total;
-    S.list_entries += list_n;
-    if (S.overflow) {
-        S.n_ent[k + 1] = 0;
-        return;
-    }
-    const unsigned long long na = (unsigned long long)S.arena_used + list_n;
-    const unsigned long long nm = (unsigned long long)S.mask_used + chunk_n;
-    if (na > S.arena_cap || nm > S.mask_cap || ent_n > 0xFFFFFFFFull) {
-        S.overflow = 1;
-        S.arena_need = na;
-        S.mask_need = nm;
-        S.n_ent[k + 1] = 0;
-        return;
-    }
-    S.list_base = S.arena_used;
-    S.mask_base = S.mask_used;
-    S.arena_used = (uint32_t)na;
-    S.mask_used = (uint32_t)nm;
-    S.n_ent[k + 1] = (uint32_t)ent_n;
 }
 
 __device__ __forceinline__ uint32_t chunks_of(uint32_t len) { return (len + 63u) >> 6; }
@@ -187,219 +95,344 @@ __device__ __forceinline__ unsigned long long group_bits(bool pred, uint32_t gsh
     return G == 64 ? b : (b & ((1ull << G) - 1ull));
 }
 
-// G lanes per entry (8, 16, 32 or 64): the group takes the entry's list in
-// strides of G; source geometry, multipole and link are gathered together so
-// one dependent round trip covers the MAC and both outcomes; counts come from
-// ballots.
+template <int G>
+__device__ __forceinline__ uint32_t group_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
+    return v;
+}
+
+// Count of one entry by its group of G lanes (8, 16, 32 or 64).  The list is
+// taken in chunks of 64 sources, one far-mask word each; lane gl holds the
+// U = 64 / G sources k0 + j G + gl, whose list words and then node records
+// (geometry, multipole, link) are all requested before any is used, so a
+// chunk costs two dependent memory round trips.  Returns (list words, next
+// entries, mask words, far pairs) of the entry's output, valid in every lane.
 template <bool kKahan, int G>
-__global__ __launch_bounds__(kBlock) void trav_count_kernel(
-    PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* __restrict__ arena,
-    unsigned long long* __restrict__ masks, const PsTravState* __restrict__ st, int it,
-    uint4* __restrict__ cnt, uint4* __restrict__ bsum) {
-    constexpr int kGroups = kBlock / G;
-    __shared__ uint4 gcnt[kGroups];
-    const uint32_t grp = threadIdx.x / G;
-    const uint32_t e = blockIdx.x * kGroups + grp;
-    const uint32_t gl = threadIdx.x & (G - 1);
-    const uint32_t gshift = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
-    const bool active = e < st->n_ent[it] && !st->overflow;  // uniform per group
-    uint4 c = make_uint4(0u, 0u, 0u, 0u);
-    if (active) {
-        const uint32_t mask_cap = st->mask_cap;
-        const uint32_t tn = ent.tgt[e];
-        const uint32_t o = ent.off[e];
-        const uint32_t L = ent.len[e];
-        const uint32_t mo = ent.moff[e];
-        const float4 tg = t.geom[tn];
-        const uint2 tl = t.link[tn];
-        const float4 tloc = t.loc[tn];
-        float ex = 0.f, ey = 0.f, ez = 0.f;
-        float cx = 0.f, cy = 0.f, cz = 0.f;
-        uint32_t nfar = 0, nleaf = 0, nchild = 0;
+__device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
+                                             const PsTravEntries& ent,
+                                             const uint32_t* __restrict__ arena,
+                                             unsigned long long* __restrict__ masks,
+                                             uint32_t mask_cap, uint32_t e, uint32_t gl,
+                                             uint32_t gshift) {
+    constexpr int U = 64 / G;
+    const uint32_t tn = ent.tgt[e];
+    const uint32_t o = ent.off[e];
+    const uint32_t L = ent.len[e];
+    const uint32_t mo = ent.moff[e];
+    const float4 tg = t.geom[tn];
+    const uint2 tl = t.link[tn];
+    float ex = 0.f, ey = 0.f, ez = 0.f;
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    uint32_t nfar = 0, nleaf = 0, nchild = 0;
+    for (uint32_t k0 = 0; k0 < L; k0 += 64) {
+        uint32_t s[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const uint32_t k = k0 + j * G + gl;
+            s[j] = k < L ? arena[o + k] : PS_NONE;
+        }
+        float4 sg[U], smp[U];
+        uint32_t sy[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            if (s[j] < t.n_nodes) {
+                sg[j] = t.geom[s[j]];
+                smp[j] = t.mp[s[j]];
+                sy[j] = t.link[s[j]].y;
+            } else {
+                sg[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                smp[j] = sg[j];
+                sy[j] = 0u;
+            }
+        }
         unsigned long long word = 0;
-        for (uint32_t k0 = 0; k0 < L; k0 += G) {
-            const uint32_t k = k0 + gl;
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const uint32_t k = k0 + j * G + gl;
             bool far = false, near_leaf = false;
-            uint32_t cc = 0;
             if (k < L) {
-                const uint32_t s = arena[o + k];
-                if (s < t.n_nodes) {  // always true when counts and writes agree
-                    const float4 sg = t.geom[s];
-                    const float4 smp = t.mp[s];
-                    const uint2 sl = t.link[s];
+                if (s[j] < t.n_nodes) {  // always true when counts and writes agree
                     float dx, dy, dz, d2;
-                    far = ps_far_d(tg, sg, theta2, dx, dy, dz, d2);
+                    far = ps_far_d(tg, sg[j], theta2, dx, dy, dz, d2);
                     if (far) {
                         if (kKahan) {
                             float px = 0.f, py = 0.f, pz = 0.f;
-                            ps_m2l_d(dx, dy, dz, d2, smp, px, py, pz);
+                            ps_m2l_d(dx, dy, dz, d2, smp[j], px, py, pz);
                             ps_kahan_add(ex, cx, px);
                             ps_kahan_add(ey, cy, py);
                             ps_kahan_add(ez, cz, pz);
                         } else {
-                            ps_m2l_d(dx, dy, dz, d2, smp, ex, ey, ez);
+                            ps_m2l_d(dx, dy, dz, d2, smp[j], ex, ey, ez);
                         }
-                    } else if (sl.y & PS_LEAF) {
+                    } else if (sy[j] & PS_LEAF) {
                         near_leaf = true;
                     } else {
-                        cc = (uint32_t)__popc(sl.y & 0xFFu);
+                        nchild += (uint32_t)__popc(sy[j] & 0xFFu);
                     }
                 } else {
                     far = true;  // an invalid id is skipped by every consumer
                 }
             }
             const unsigned long long fb = group_bits<G>(far, gshift);
+            word |= fb << (j * G);
             nfar += (uint32_t)__popcll(fb);
             nleaf += (uint32_t)__popcll(group_bits<G>(near_leaf, gshift));
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-                nchild += (uint32_t)__popcll(group_bits<G>((cc >> b) & 1u, gshift)) << b;
-            word |= fb << (k0 & 63u);
-            if (((k0 + G) & 63u) == 0 || k0 + G >= L) {
-                if (gl == 0 && mo + (k0 >> 6) < mask_cap) masks[mo + (k0 >> 6)] = word;
-                word = 0;
-            }
         }
-        if (kKahan) {
+        if (gl == 0 && mo + (k0 >> 6) < mask_cap) masks[mo + (k0 >> 6)] = word;
+    }
+    nchild = group_sum_u32<G>(nchild);
+    if (kKahan) {
 #pragma unroll
-            for (int d = G / 2; d > 0; d >>= 1) {
-                ps_kahan_merge(ex, cx, __shfl_xor(ex, d, G), __shfl_xor(cx, d, G));
-                ps_kahan_merge(ey, cy, __shfl_xor(ey, d, G), __shfl_xor(cy, d, G));
-                ps_kahan_merge(ez, cz, __shfl_xor(ez, d, G), __shfl_xor(cz, d, G));
+        for (int d = G / 2; d > 0; d >>= 1) {
+            ps_kahan_merge(ex, cx, __shfl_xor(ex, d, G), __shfl_xor(cx, d, G));
+            ps_kahan_merge(ey, cy, __shfl_xor(ey, d, G), __shfl_xor(cy, d, G));
+            ps_kahan_merge(ez, cz, __shfl_xor(ez, d, G), __shfl_xor(cz, d, G));
+        }
+        ex -= cx;
+        ey -= cy;
+        ez -= cz;
+    } else {
+        ex = group_sum<G>(ex);
+        ey = group_sum<G>(ey);
+        ez = group_sum<G>(ez);
+    }
+    const uint32_t out = nleaf + nchild;
+    uint4 c = make_uint4(0u, 0u, 0u, nfar);
+    if (tl.y & PS_LEAF) {
+        if (nchild == 0) {  // final: the near entries of this list are the P2P sources
+            if (nleaf && gl == 0) {
+                t.p2p_off[tn] = o;
+                t.p2p_len[tn] = L;
+                t.p2p_moff[tn] = mo;
             }
-            ex -= cx;
-            ey -= cy;
-            ez -= cz;
         } else {
-            ex = group_sum<G>(ex);
-            ey = group_sum<G>(ey);
-            ez = group_sum<G>(ez);
+            c = make_uint4(out, 1u, chunks_of(out), nfar);
         }
-        const uint32_t out = nleaf + nchild;
-        c = make_uint4(0u, 0u, 0u, nfar);
-        if (tl.y & PS_LEAF) {
-            if (nchild == 0) {  // final: the near entries of this list are the P2P sources
-                if (nleaf && gl == 0) {
-                    t.p2p_off[tn] = o;
-                    t.p2p_len[tn] = L;
-                    t.p2p_moff[tn] = mo;
-                }
-            } else {
-                c = make_uint4(out, 1u, chunks_of(out), nfar);
-            }
-        } else if (out) {
-            const uint32_t nk = ps_child_count(tl);
-            c = make_uint4(out, nk, nk * chunks_of(out), nfar);
-        }
-        if (gl == 0) {
-            if (nfar) {
-                float4 l = tloc;
-                l.y += ex;
-                l.z += ey;
-                l.w += ez;
-                t.loc[tn] = l;
-            }
-            cnt[e] = c;
-        }
+    } else if (out) {
+        const uint32_t nk = ps_child_count(tl);
+        c = make_uint4(out, nk, nk * chunks_of(out), nfar);
     }
-    if (gl == 0) gcnt[grp] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint4 b = make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-        for (int g = 0; g < kGroups; ++g) {
-            b.x += gcnt[g].x;
-            b.y += gcnt[g].y;
-            b.z += gcnt[g].z;
-            b.w += gcnt[g].w;
-        }
-        bsum[blockIdx.x] = b;
+    if (gl == 0 && nfar) {
+        float4 l = t.loc[tn];
+        l.y += ex;
+        l.z += ey;
+        l.w += ez;
+        t.loc[tn] = l;
     }
+    return c;
 }
 
+// Write of one continuing entry by its group: out = its next list length
+// (> 0), p = (list offset, next entry index, mask word offset) in the arenas.
+// Per chunk of 64 sources: the mask word and the U list words of each lane
+// in one round trip, the links of the near ones in a second; outputs keep
+// the list order (a prefix sum over the group per j).
 template <int G>
-__global__ __launch_bounds__(kBlock) void trav_write_kernel(
-    PsTravTree t, PsTravEntries ent, uint32_t* __restrict__ arena,
-    const unsigned long long* __restrict__ masks, const PsTravState* __restrict__ st, int it,
-    const uint4* __restrict__ cnt, const uint4* __restrict__ bsum, PsTravEntries next) {
-    constexpr int kGroups = kBlock / G;
-    __shared__ uint4 gpos[kGroups];
-    const uint32_t grp = threadIdx.x / G;
-    const uint32_t e = blockIdx.x * kGroups + grp;
-    const uint32_t gl = threadIdx.x & (G - 1);
-    const uint32_t n_ent = st->n_ent[it];
-    if (st->overflow || blockIdx.x * kGroups >= n_ent) return;  // uniform per block
-    const uint4 c = e < n_ent ? cnt[e] : make_uint4(0u, 0u, 0u, 0u);
-    if (gl == 0) gpos[grp] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {  // exclusive prefix of the block's entries, from its base
-        uint4 run = bsum[blockIdx.x];
-        for (int g = 0; g < kGroups; ++g) {
-            const uint4 cg = gpos[g];
-            gpos[g] = run;
-            run.x += cg.x;
-            run.y += cg.y;
-            run.z += cg.z;
-        }
-    }
-    __syncthreads();
-    if (e >= n_ent) return;
-    if (c.x == 0) return;  // final or empty: nothing to pass on (uniform per group)
-    const uint4 p = gpos[grp];
-    const uint32_t list_base = st->list_base;
-    const uint32_t mask_base = st->mask_base;
+__device__ __forceinline__ void write_entry(const PsTravTree& t, const PsTravEntries& ent,
+                                            uint32_t* __restrict__ arena,
+                                            const unsigned long long* __restrict__ masks,
+                                            uint32_t e, uint32_t out, uint4 p,
+                                            const PsTravEntries& next, uint32_t gl) {
+    constexpr int U = 64 / G;
     const uint32_t tn = ent.tgt[e];
     const uint32_t o = ent.off[e];
     const uint32_t L = ent.len[e];
     const uint32_t mo = ent.moff[e];
     const uint2 tl = t.link[tn];
-    // the count pass's masks and link reads give exactly c.x outputs; the
+    // the count pass's masks and link reads give exactly `out` outputs; the
     // bound only makes that an invariant of the stores
-    uint32_t base = list_base + p.x;
-    const uint32_t limit = base + c.x;
-    for (uint32_t k0 = 0; k0 < L; k0 += G) {
-        const uint32_t k = k0 + gl;
+    uint32_t base = p.x;
+    const uint32_t limit = base + out;
+    for (uint32_t k0 = 0; k0 < L; k0 += 64) {
         const unsigned long long fm = masks[mo + (k0 >> 6)];
-        uint32_t s = 0, nout = 0;
-        uint2 sl = make_uint2(0u, 0u);
-        if (k < L && !((fm >> ((k0 & 63u) + gl)) & 1ull)) {
-            s = arena[o + k];
-            sl = t.link[s];
-            nout = (sl.y & PS_LEAF) ? 1u : (uint32_t)__popc(sl.y & 0xFFu);
-        }
-        uint32_t inc = nout;
+        uint32_t s[U];
 #pragma unroll
-        for (int d = 1; d < G; d <<= 1) {
-            const uint32_t v = __shfl_up(inc, d, G);
-            if (gl >= (uint32_t)d) inc += v;
+        for (int j = 0; j < U; ++j) {
+            const uint32_t k = k0 + j * G + gl;
+            s[j] = k < L ? arena[o + k] : PS_NONE;
         }
-        const uint32_t total = __shfl(inc, G - 1, G);
-        const uint32_t w = base + inc - nout;
-        if (nout) {
-            if (sl.y & PS_LEAF) {
-                if (w < limit) arena[w] = s;
-            } else {
-                for (uint32_t r = 0; r < nout; ++r)
-                    if (w + r < limit) arena[w + r] = sl.x + r;
+        uint2 sl[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const bool near = s[j] != PS_NONE && !((fm >> (j * G + gl)) & 1ull);
+            if (!near) s[j] = PS_NONE;
+            sl[j] = (near && s[j] < t.n_nodes) ? t.link[s[j]] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const uint32_t nout =
+                s[j] == PS_NONE ? 0u
+                                : ((sl[j].y & PS_LEAF) ? 1u : (uint32_t)__popc(sl[j].y & 0xFFu));
+            uint32_t inc = nout;
+#pragma unroll
+            for (int d = 1; d < G; d <<= 1) {
+                const uint32_t v = __shfl_up(inc, d, G);
+                if (gl >= (uint32_t)d) inc += v;
             }
+            const uint32_t total = __shfl(inc, G - 1, G);
+            const uint32_t w = base + inc - nout;
+            if (nout) {
+                if (sl[j].y & PS_LEAF) {
+                    if (w < limit) arena[w] = s[j];
+                } else {
+                    for (uint32_t r = 0; r < nout; ++r)
+                        if (w + r < limit) arena[w + r] = sl[j].x + r;
+                }
+            }
+            base += total;
         }
-        base += total;
     }
-    const uint32_t nch = chunks_of(c.x);
+    const uint32_t nch = chunks_of(out);
     if (tl.y & PS_LEAF) {
         if (gl == 0) {
             next.tgt[p.y] = tn;
-            next.off[p.y] = list_base + p.x;
-            next.len[p.y] = c.x;
-            next.moff[p.y] = mask_base + p.z;
+            next.off[p.y] = p.x;
+            next.len[p.y] = out;
+            next.moff[p.y] = p.z;
         }
     } else {
-        for (uint32_t r = gl; r < c.y; r += G) {
+        const uint32_t nk = ps_child_count(tl);
+        for (uint32_t r = gl; r < nk; r += G) {
             next.tgt[p.y + r] = tl.x + r;
-            next.off[p.y + r] = list_base + p.x;
-            next.len[p.y + r] = c.x;
-            next.moff[p.y + r] = mask_base + p.z + r * nch;
+            next.off[p.y + r] = p.x;
+            next.len[p.y + r] = out;
+            next.moff[p.y + r] = p.z + r * nch;
         }
+    }
+}
+
+// next item of a block-local work queue, the same value in every lane of the group
+template <int G>
+__device__ __forceinline__ uint32_t group_take(uint32_t* q, uint32_t gl) {
+    uint32_t v = 0;
+    if (gl == 0) v = atomicAdd(q, 1u);
+    return __shfl(v, 0, G);
+}
+
+template <bool kKahan, int G>
+__global__ __launch_bounds__(kStep) void trav_step_kernel(
+    PsTravTree t, float theta2, PsTravEntries ent, uint32_t* __restrict__ arena,
+    unsigned long long* __restrict__ masks, PsTravState* __restrict__ st, int it,
+    PsTravEntries next, unsigned long long* __restrict__ far_part) {
+    constexpr uint32_t kGroups = kStep / G;
+    __shared__ uint4 scnt[kSeg];  // sizes of the segment's entries, then their offsets
+    __shared__ uint4 wtot[kStep / 64];
+    __shared__ uint32_t sbase[4];  // list, entry and mask bases; 1 = reserved
+    __shared__ uint32_t queue[2];  // work queues of the count and write phases
+    __shared__ unsigned long long sfar[kGroups];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t grp = tid / G;
+    const uint32_t gl = tid & (G - 1);
+    const uint32_t gshift = lane & ~(uint32_t)(G - 1);
+    // consecutive ranges to the blocks of one XCD (blocks go round-robin over
+    // the 8 XCDs), so neighbouring targets share an L2
+    const uint32_t nblk = gridDim.x;
+    const uint32_t bid = (nblk & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (nblk >> 3) + (blockIdx.x >> 3);
+    // another block may raise the flag meanwhile: the traversal is then redone
+    const uint32_t ovf = __atomic_load_n(&st->overflow, __ATOMIC_RELAXED);
+    const uint32_t n_e = ovf ? 0u : st->n_ent[it];
+    const uint32_t arena_cap = st->arena_cap;
+    const uint32_t mask_cap = st->mask_cap;
+    const uint32_t R = trav_range(n_e, nblk, kGroups);
+    const unsigned long long r0 = (unsigned long long)bid * R;
+    const uint32_t b0 = (uint32_t)(r0 < n_e ? r0 : n_e);
+    const uint32_t b1 = (uint32_t)(r0 + R < n_e ? r0 + R : n_e);
+    unsigned long long far_acc = 0;
+    if (tid == 0) {
+        queue[0] = 0u;
+        queue[1] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t seg = b0; seg < b1; seg += kSeg) {  // uniform per block
+        const uint32_t segn = b1 - seg < kSeg ? b1 - seg : kSeg;
+        if (tid == 0) queue[1] = 0u;  // the write phase of the last segment is over
+        // count
+        for (uint32_t i = group_take<G>(&queue[0], gl); i < segn; i = group_take<G>(&queue[0], gl)) {
+            const uint4 c = count_entry<kKahan, G>(t, theta2, ent, arena, masks, mask_cap, seg + i,
+                                                   gl, gshift);
+            if (gl == 0) scnt[i] = c;
+            far_acc += c.w;
+        }
+        __syncthreads();
+        // place: exclusive prefix of (list words, entries, mask words) in
+        // place, the entry's list length kept in .w
+        uint4 carry = make_uint4(0u, 0u, 0u, 0u);  // same value in every thread
+        for (uint32_t j0 = 0; j0 < segn; j0 += kStep) {
+            const uint32_t j = j0 + tid;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (j < segn) v = scnt[j];
+            uint4 inc = v;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t ox = __shfl_up(inc.x, d, 64);
+                const uint32_t oy = __shfl_up(inc.y, d, 64);
+                const uint32_t oz = __shfl_up(inc.z, d, 64);
+                if (lane >= (uint32_t)d) {
+                    inc.x += ox;
+                    inc.y += oy;
+                    inc.z += oz;
+                }
+            }
+            if (lane == 63) wtot[wave] = inc;
+            __syncthreads();
+            uint4 run = carry, all = carry;
+#pragma unroll
+            for (uint32_t w = 0; w < kStep / 64; ++w) {
+                const uint4 tw = wtot[w];
+                if (w < wave) {
+                    run.x += tw.x; run.y += tw.y; run.z += tw.z;
+                }
+                all.x += tw.x; all.y += tw.y; all.z += tw.z;
+            }
+            if (j < segn)
+                scnt[j] = make_uint4(run.x + inc.x - v.x, run.y + inc.y - v.y,
+                                     run.z + inc.z - v.z, v.x);
+            carry = all;
+            __syncthreads();  // wtot is rewritten by the next round
+        }
+        if (wave == 0) {  // reserve the segment's output: three atomics in flight at once
+            unsigned long long b = 0;
+            if (carry.y) {
+                if (lane == 0) b = atomicAdd(&st->arena_ctr, (unsigned long long)carry.x);
+                if (lane == 1) b = atomicAdd(&st->mask_ctr, (unsigned long long)carry.z);
+                if (lane == 2) b = atomicAdd(&st->n_ent[it + 1], carry.y);
+            }
+            const unsigned long long ba = __shfl(b, 0, 64);
+            const unsigned long long bm = __shfl(b, 1, 64);
+            const unsigned long long be = __shfl(b, 2, 64);
+            if (lane == 0) {
+                const uint32_t ok =
+                    (carry.y && ba + carry.x <= arena_cap && bm + carry.z <= mask_cap) ? 1u : 0u;
+                if (carry.y && !ok) __atomic_store_n(&st->overflow, 1u, __ATOMIC_RELAXED);
+                sbase[0] = (uint32_t)ba;
+                sbase[1] = (uint32_t)be;
+                sbase[2] = (uint32_t)bm;
+                sbase[3] = ok;
+                queue[0] = 0u;  // the count phase is over
+            }
+        }
+        __syncthreads();
+        // write
+        if (sbase[3]) {
+            const uint32_t la = sbase[0], le = sbase[1], lm = sbase[2];
+            for (uint32_t i = group_take<G>(&queue[1], gl); i < segn;
+                 i = group_take<G>(&queue[1], gl)) {
+                const uint4 q = scnt[i];
+                if (q.w == 0) continue;  // final or empty: nothing to pass on
+                write_entry<G>(t, ent, arena, masks, seg + i, q.w,
+                               make_uint4(la + q.x, le + q.y, lm + q.z, 0u), next, gl);
+            }
+        }
+        __syncthreads();  // scnt and sbase are rewritten by the next segment
+    }
+    if (gl == 0) sfar[grp] = far_acc;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long f = 0;
+#pragma unroll
+        for (uint32_t g = 0; g < kGroups; ++g) f += sfar[g];
+        far_part[blockIdx.x] = f;
     }
 }
 
@@ -414,49 +447,33 @@ void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* 
                                                mask_cap);
 }
 
-void ps_launch_trav_plan(PsTravState* st, int k, uint4* bsum, int lanes, hipStream_t s) {
-    const int g = (lanes == 8 || lanes == 32 || lanes == 64) ? lanes : 16;
-    trav_plan_kernel<<<1, kPlanThreads, 0, s>>>(st, k, bsum, (uint32_t)(kBlock / g));
-}
+namespace {
+int lane_group(int lanes) { return (lanes == 8 || lanes == 32 || lanes == 64) ? lanes : 16; }
+}  // namespace
 
 uint32_t ps_trav_blocks(uint32_t max_ent, int lanes) {
-    const int g = (lanes == 8 || lanes == 32 || lanes == 64) ? lanes : 16;
-    return (max_ent + kBlock / g - 1) / (kBlock / g);
+    const uint32_t groups = kStep / lane_group(lanes);
+    const uint32_t b = (uint32_t)(((unsigned long long)max_ent + groups - 1) / groups);
+    return b < 1u ? 1u : (b < kMaxBlocks ? b : kMaxBlocks);
 }
 
-void ps_launch_trav_count(PsTravTree t, float theta2, PsTravEntries ent, const uint32_t* arena,
-                          unsigned long long* masks, const PsTravState* st, int k,
-                          uint32_t max_ent, uint4* cnt, uint4* bsum, int kahan, int lanes,
-                          hipStream_t s) {
-    if (max_ent == 0) return;
-#define PS_COUNT(KH, G)                                                                       \
-    trav_count_kernel<KH, G><<<(max_ent + kBlock / G - 1) / (kBlock / G), kBlock, 0, s>>>(     \
-        t, theta2, ent, arena, masks, st, k, cnt, bsum)
-    if (kahan) {
-        if (lanes == 8) PS_COUNT(true, 8);
-        else if (lanes == 32) PS_COUNT(true, 32);
-        else if (lanes == 64) PS_COUNT(true, 64);
-        else PS_COUNT(true, 16);
-    } else {
-        if (lanes == 8) PS_COUNT(false, 8);
-        else if (lanes == 32) PS_COUNT(false, 32);
-        else if (lanes == 64) PS_COUNT(false, 64);
-        else PS_COUNT(false, 16);
+void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t* arena,
+                         unsigned long long* masks, PsTravState* st, int k, uint32_t max_ent,
+                         PsTravEntries next, unsigned long long* far_part, int kahan, int lanes,
+                         hipStream_t s) {
+    const uint32_t nb = ps_trav_blocks(max_ent, lanes);
+#define PS_STEP(KH, G)                                                                    \
+    trav_step_kernel<KH, G><<<nb, kStep, 0, s>>>(t, theta2, ent, arena, masks, st, k, next, \
+                                                  far_part)
+    switch (lane_group(lanes) + (kahan ? 1 : 0)) {
+        case 8: PS_STEP(false, 8); break;
+        case 9: PS_STEP(true, 8); break;
+        case 32: PS_STEP(false, 32); break;
+        case 33: PS_STEP(true, 32); break;
+        case 64: PS_STEP(false, 64); break;
+        case 65: PS_STEP(true, 64); break;
+        case 17: PS_STEP(true, 16); break;
+        default: PS_STEP(false, 16); break;
     }
-#undef PS_COUNT
-}
-
-void ps_launch_trav_write(PsTravTree t, PsTravEntries ent, uint32_t* arena,
-                          const unsigned long long* masks, const PsTravState* st, int k,
-                          uint32_t max_ent, const uint4* cnt, const uint4* bsum,
-                          PsTravEntries next, int lanes, hipStream_t s) {
-    if (max_ent == 0) return;
-#define PS_WRITE(G)                                                                           \
-    trav_write_kernel<G><<<(max_ent + kBlock / G - 1) / (kBlock / G), kBlock, 0, s>>>(         \
-        t, ent, arena, masks, st, k, cnt, bsum, next)
-    if (lanes == 8) PS_WRITE(8);
-    else if (lanes == 32) PS_WRITE(32);
-    else if (lanes == 64) PS_WRITE(64);
-    else PS_WRITE(16);
-#undef PS_WRITE
+#undef PS_STEP
 }

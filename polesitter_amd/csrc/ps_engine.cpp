@@ -124,13 +124,17 @@ enum Stage { kSort = 1, kBuild = 2, kUpward = 3, kInteract = 4, kDownward = 5, k
 struct ps_hip_engine {
     int device = 0;
     hipStream_t s = nullptr;
+    hipStream_t s2 = nullptr;  // second stream: the big-leaf P2P kernel beside the other one
+    hipEvent_t fork = nullptr, join = nullptr;
     size_t max_p = 0;
     size_t node_cap = 0;
     float theta = 1.0f;
     int trav = PS_HIP_TRAV_MT;
     int kahan = 0;
-    int stats_on = 0;
+    int stats_on = 0;   // stage events
+    int count_on = 0;   // pair counts (extra reductions)
     int trav_lanes = 8;  // lanes per traversal entry
+    uint32_t p2p_big = 24;  // depth-10 leaves of >= this many particles: wavefront per leaf
 
     // host-pointer staging, one buffer per array (allocated on first use)
     DevBuf stage[9];
@@ -144,7 +148,7 @@ struct ps_hip_engine {
     // the far-mask arena fill over the whole tick
     DevBuf ent[2][4];
     DevBuf arena_b, masks_b;
-    DevBuf far_b, block_pairs_b, tstate_b;
+    DevBuf far_b, block_pairs_b, tstate_b, big_b;
     // small device / pinned host scalars
     DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs | u64 leaf pairs
     uint32_t* h_small = nullptr;  // pinned mirror
@@ -217,7 +221,7 @@ struct ps_hip_engine {
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
                          &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,
-                         &small_b,   &block_pairs_b, &tstate_b};
+                         &small_b,   &block_pairs_b, &tstate_b, &big_b};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
             for (auto& b : es) release(b);
@@ -227,6 +231,11 @@ struct ps_hip_engine {
             if (e) (void)hipEventDestroy(e);
         for (auto& e : ev_it)
             if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        fork = join = nullptr;
+        if (s2) (void)hipStreamDestroy(s2);
+        s2 = nullptr;
         if (s) (void)hipStreamDestroy(s);
         s = nullptr;
     }
@@ -412,18 +421,22 @@ int stage_l2l(ps_hip_engine* e) {
 }
 
 int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
-    unsigned long long* pc = nullptr;
-    if (e->stats_on) {
-        pc = e->pairs();
-        PS_CK(grow(e->block_pairs_b, 16 * (size_t)ps_p2p_blocks(e->n), e->s));
-    }
+    unsigned long long* pc = e->count_on ? e->pairs() : nullptr;
+    // the depth-10 level holds every leaf of more than one particle
+    const uint32_t lbeg = e->max_depth == 10 ? e->ls[10] : 0u;
+    const uint32_t lend = e->max_depth == 10 ? e->ls[11] : 0u;
+    if (e->count_on)
+        PS_CK(grow(e->block_pairs_b, 16 * (size_t)ps_p2p_blocks(e->n, lend - lbeg), e->s));
     if (!e->arena_b.p) PS_CK(grow(e->arena_b, 256, e->s));
     if (!e->masks_b.p) PS_CK(grow(e->masks_b, 256, e->s));
+    PS_CK(grow(e->big_b, 4 * ((size_t)(lend - lbeg) + 1), e->s));
     if (ev_before >= 0) e->record(ev_before);
     ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
                   e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
-                  e->fz, mode, e->kahan,
-                  reinterpret_cast<unsigned long long*>(e->block_pairs_b.p), pc, e->s);
+                  e->fz, mode, e->kahan, lbeg, lend, e->p2p_big,
+                  reinterpret_cast<uint32_t*>(e->big_b.p),
+                  reinterpret_cast<unsigned long long*>(e->block_pairs_b.p), pc, e->s, e->s2,
+                  e->fork, e->join);
     return map_err(hipGetLastError());
 }
 
@@ -446,6 +459,8 @@ int run_tick(ps_hip_engine* e) {
 struct PrepWS {
     int dev = -1;
     hipStream_t s = nullptr;
+    hipStream_t s2 = nullptr;  // second stream: the big-leaf P2P kernel beside the other one
+    hipEvent_t fork = nullptr, join = nullptr;
     DevBuf part, prm, x, y, z, code;
     float* h_out = nullptr;
 };
@@ -494,6 +509,9 @@ int ps_hip_engine_create(ps_hip_engine_t** out, size_t max_particles, size_t max
     e->trav = traversal == PS_HIP_TRAV_ST ? PS_HIP_TRAV_ST : PS_HIP_TRAV_MT;
     hipError_t r = hipGetDevice(&e->device);
     if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking);
+    if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking);
+    if (r == hipSuccess) r = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+    if (r == hipSuccess) r = hipEventCreateWithFlags(&e->join, hipEventDisableTiming);
     for (int k = 0; k < 12 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev[k]);
     for (int k = 0; k < 64 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev_it[k]);
     if (r == hipSuccess) r = alloc_fixed(e);
@@ -529,11 +547,17 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
             e->theta = value > 0.0 ? (float)value : 1.0f;
             return PS_HIP_OK;
         case PS_HIP_OPT_STATS:
+            if (value != 0.0 && value != 1.0 && value != 2.0) return PS_HIP_EINVAL;
             e->stats_on = value != 0.0;
+            e->count_on = value == 1.0;
             return PS_HIP_OK;
         case PS_HIP_OPT_TRAV_LANES:
             if (value != 8 && value != 16 && value != 32 && value != 64) return PS_HIP_EINVAL;
             e->trav_lanes = (int)value;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_P2P_BIG:
+            if (!(value >= 0.0 && value <= 4294967295.0)) return PS_HIP_EINVAL;
+            e->p2p_big = (uint32_t)value;
             return PS_HIP_OK;
         default:
             return PS_HIP_EINVAL;
@@ -624,7 +648,7 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
     for (int k = 0; k < 9; ++k)
         if (host[k] != dev[k])
             PS_CK(hipMemcpyAsync(host[k], dev[k], n * 4, hipMemcpyDeviceToHost, e->s));
-    if (e->stats_on)
+    if (e->count_on)
         PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 16, hipMemcpyDeviceToHost, e->s));
     PS_CK(hipEventRecord(e->ev[7], e->s));
     PS_CK(hipStreamSynchronize(e->s));
@@ -632,9 +656,11 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
     e->st.n_nodes = e->n_nodes;
     e->st.max_depth = e->max_depth;
     std::memcpy(e->st.level_start, e->ls, sizeof(e->ls));
-    if (e->stats_on) {
+    if (e->count_on) {
         e->st.p2p_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 32);
         e->st.p2p_leaf_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 34);
+    }
+    if (e->stats_on) {
         e->st.ms_sort = e->elapsed(1, 2);
         e->st.ms_build = e->elapsed(2, 3);
         e->st.ms_upward = e->elapsed(3, 4);

@@ -26,7 +26,7 @@ hipError_t ps_scan_dev_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_
                            size_t n_max, unsigned long long* total64, void* tmp, hipStream_t s);
 // graph-safe fills (no memset nodes)
 void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
-// *out = sum of in[0..n) (one block, no atomics)
+// *out = sum of in[0..n) (cleared by a kernel, then one atomic add per block)
 void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* out,
                    hipStream_t s);
 
@@ -125,17 +125,23 @@ void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------
 // mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);
 // 2: f += m*sum (P2P only).  A leaf's near sources are the non-far entries of
-// its final traversal list.
+// its final traversal list.  Leaves in [lbeg, lend) (the depth-10 nodes) with
+// >= tmin particles go to a wavefront-per-leaf kernel, the rest to a
+// thread-per-particle kernel (tmin = 0: all of them).  big: scratch of
+// 1 + (lend - lbeg) words for the list of those leaves.  With s_big, fork and
+// join set, the big-leaf kernel runs on s_big beside the other; all work is
+// ordered after earlier work on s and before later work on s.
 // pair_count (stats, may be null): per-block partial sums in block_pairs
-// (>= 2 * ps_p2p_blocks(n) entries), reduced into pair_count[0] (particle
-// pairs) and pair_count[1] (target leaf x source leaf pairs).
-uint32_t ps_p2p_blocks(size_t n);
+// (>= 2 * ps_p2p_blocks(n, lend - lbeg) entries), reduced into pair_count[0]
+// (particle pairs) and pair_count[1] (target leaf x source leaf pairs).
+uint32_t ps_p2p_blocks(size_t n, uint32_t n_big_nodes);
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                    const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
+                   uint32_t lbeg, uint32_t lend, uint32_t tmin, uint32_t* big,
                    unsigned long long* block_pairs, unsigned long long* pair_count,
-                   hipStream_t s);
+                   hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join);
 
 // ---- ps_direct.hip: FP64 direct sum (the error oracle on the device) -------
 void ps_launch_direct_f64(const float4* pos4, size_t n, const uint32_t* targets, size_t nt,

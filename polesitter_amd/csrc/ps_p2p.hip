@@ -1,12 +1,15 @@
 // ps_p2p.hip -- near field (P2P) fused with L2P (replaces the leaf x leaf
 // loop of H:1279-1452 and L2P of H:1158-1235).
 //
-// One thread per target particle, particles in sorted order: the lanes of a
-// wave are spatially adjacent, and in dense leaves they share one leaf and so
-// the same source list -- every source load is the same address across the
-// wave.  A leaf's sources are the near (mask bit clear) entries of its final
-// traversal list, read in place from the list arena: near leaves only, self
-// included (the self term is zero: d = 0).  Output per particle, one write:
+// Two kernels split the target leaves.  A depth-10 leaf of >= tmin particles
+// gets a wavefront: its 64-target rounds share the leaf's source list, so the
+// loop runs on scalar control flow and every source particle is a scalar load
+// broadcast to the wave.  Every other particle gets a thread; particles are in
+// sorted order, so the lanes of a leaf walk the same list in step and their
+// source loads coalesce.  A leaf's sources are the near (mask bit clear)
+// entries of its final traversal list, read in place from the list arena:
+// near leaves only, self included (the self term is zero: d = 0).
+// Output per particle, one write:
 //   f = m * field(leaf) + m * sum_j m_j d / (|d|^2 + eps^2)^(3/2)
 // FP32 with v_rsq_f32; optional Kahan-compensated accumulation.
 #include "ps_device.h"
@@ -16,6 +19,133 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// accumulate the pairs of target p with the particles [j0, j1) of one source
+// leaf; j0, j1 wave-uniform: the source loads become scalar loads shared by
+// the wave, four particles per round
+template <bool kKahan>
+__device__ __forceinline__ void p2p_uniform_range(float4 p, const float4* __restrict__ pos4,
+                                                  uint32_t j0, uint32_t j1, float& ax, float& ay,
+                                                  float& az, float& cx, float& cy, float& cz) {
+    uint32_t j = j0;
+    for (; j + 4 <= j1; j += 4) {
+        const float4 q0 = pos4[j], q1 = pos4[j + 1], q2 = pos4[j + 2], q3 = pos4[j + 3];
+        if (kKahan) {
+            const float4 qs[4] = {q0, q1, q2, q3};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float tx = 0.f, ty = 0.f, tz = 0.f;
+                ps_pair(p, qs[u], tx, ty, tz);
+                ps_kahan_add(ax, cx, tx);
+                ps_kahan_add(ay, cy, ty);
+                ps_kahan_add(az, cz, tz);
+            }
+        } else {
+            ps_pair(p, q0, ax, ay, az);
+            ps_pair(p, q1, ax, ay, az);
+            ps_pair(p, q2, ax, ay, az);
+            ps_pair(p, q3, ax, ay, az);
+        }
+    }
+    for (; j < j1; ++j) {
+        const float4 q = pos4[j];
+        if (kKahan) {
+            float tx = 0.f, ty = 0.f, tz = 0.f;
+            ps_pair(p, q, tx, ty, tz);
+            ps_kahan_add(ax, cx, tx);
+            ps_kahan_add(ay, cy, ty);
+            ps_kahan_add(az, cz, tz);
+        } else {
+            ps_pair(p, q, ax, ay, az);
+        }
+    }
+}
+
+// the big-leaf kernel's targets: depth-10 leaves [lbeg, lend) of >= tmin particles
+__device__ __forceinline__ bool p2p_is_big(uint32_t lf, uint32_t lbeg, uint32_t lend,
+                                           uint32_t tmin, const uint2* __restrict__ link,
+                                           const uint32_t* __restrict__ nend) {
+    return lf >= lbeg && lf < lend && nend[lf] - link[lf].x >= tmin;
+}
+
+// accumulate target p against the near sources of one final list (wave-uniform
+// list: scalar loads); pairs += particle pairs, nsrc += source leaves
+template <bool kKahan>
+__device__ __forceinline__ void p2p_uniform_list(float4 p, size_t n, uint32_t L, uint32_t o,
+                                                 uint32_t mo, const float4* __restrict__ pos4,
+                                                 const uint32_t* __restrict__ arena,
+                                                 const unsigned long long* __restrict__ masks,
+                                                 const uint2* __restrict__ link,
+                                                 const uint32_t* __restrict__ nend, float& ax,
+                                                 float& ay, float& az, float& cx, float& cy,
+                                                 float& cz, uint32_t& pairs, uint32_t& nsrc) {
+    for (uint32_t k0 = 0; k0 < L; k0 += 64) {
+        unsigned long long near = ~masks[mo + (k0 >> 6)];
+        if (L - k0 < 64) near &= (1ull << (L - k0)) - 1ull;
+        while (near) {
+            const uint32_t b = (uint32_t)__ffsll((long long)near) - 1u;
+            near &= near - 1ull;
+            ++nsrc;
+            const uint32_t s = arena[o + k0 + b];
+            const uint32_t j0 = link[s].x;
+            const uint32_t j1e = nend[s];
+            const uint32_t j1 = j1e < (uint32_t)n ? j1e : (uint32_t)n;
+            pairs += j1e - j0;
+            p2p_uniform_range<kKahan>(p, pos4, j0, j1, ax, ay, az, cx, cy, cz);
+        }
+    }
+}
+
+// f for particle i from its leaf's field and its P2P sum (mode 0: both;
+// 1: field only; 2: adds the sum to f)
+template <int kMode>
+__device__ __forceinline__ void p2p_store(size_t i, float4 p, float4 fl, float ax, float ay,
+                                          float az, float* __restrict__ fx,
+                                          float* __restrict__ fy, float* __restrict__ fz) {
+#pragma clang fp contract(off)
+    if (kMode == 0) {  // L2P (first write, H:1230-1232) then the P2P sum
+        fx[i] = p.w * fl.y + p.w * ax;
+        fy[i] = p.w * fl.z + p.w * ay;
+        fz[i] = p.w * fl.w + p.w * az;
+    } else if (kMode == 1) {
+        fx[i] = p.w * fl.y;
+        fy[i] = p.w * fl.z;
+        fz[i] = p.w * fl.w;
+    } else {
+        fx[i] = fx[i] + p.w * ax;
+        fy[i] = fy[i] + p.w * ay;
+        fz[i] = fz[i] + p.w * az;
+    }
+}
+
+// per-block partial sums of the stats, reduced by another kernel
+__device__ __forceinline__ void p2p_block_stats(uint32_t pairs, uint32_t nsrc,
+                                                unsigned long long* block_pairs, uint32_t slot,
+                                                uint32_t stride) {
+    __shared__ unsigned long long wsum[2][kBlock / 64];
+    unsigned long long v = pairs, u = nsrc;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        v += __shfl_down(v, d, 64);
+        u += __shfl_down(u, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wsum[0][threadIdx.x >> 6] = v;
+        wsum[1][threadIdx.x >> 6] = u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = 0, c = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            b += wsum[0][w];
+            c += wsum[1][w];
+        }
+        block_pairs[slot] = b;
+        block_pairs[stride + slot] = c;
+    }
+}
+
+// One thread per target particle of a small leaf (particles of big leaves
+// are left to p2p_big_kernel).
 template <int kMode, bool kKahan>
 __global__ __launch_bounds__(kBlock) void p2p_kernel(
     size_t n, const float4* __restrict__ pos4, const uint32_t* __restrict__ leaf_of,
@@ -23,17 +153,20 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
     const uint32_t* __restrict__ p2p_len, const uint32_t* __restrict__ p2p_moff,
     const uint32_t* __restrict__ arena, const unsigned long long* __restrict__ masks,
     const uint2* __restrict__ link, const uint32_t* __restrict__ nend, float* __restrict__ fx,
-    float* __restrict__ fy, float* __restrict__ fz, unsigned long long* block_pairs,
-    uint32_t nblocks) {
+    float* __restrict__ fy, float* __restrict__ fz, uint32_t lbeg, uint32_t lend,
+    uint32_t tmin, unsigned long long* block_pairs, uint32_t stride) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = i < n;
     float ax = 0.f, ay = 0.f, az = 0.f;
     float cx = 0.f, cy = 0.f, cz = 0.f;
     uint32_t pairs = 0, nsrc = 0;  // particle pairs; near source leaves
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t lf = 0;
+    bool valid = i < n;
     if (valid) {
         lf = leaf_of[i];
+        valid = !p2p_is_big(lf, lbeg, lend, tmin, link, nend);
+    }
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid) {
         p = pos4[i];
         const uint32_t L = (kMode != 1) ? p2p_len[lf] : 0u;
         if (L) {
@@ -66,93 +199,169 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
                 }
             }
         }
-    }
-    if (block_pairs) {  // per-block partial sums, reduced by another kernel
         // leaf pairs: the first particle of each leaf counts its near sources
-        if (valid && i != link[lf].x) nsrc = 0;
-        __shared__ unsigned long long wsum[2][kBlock / 64];
-        unsigned long long v = pairs, u = nsrc;
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) {
-            v += __shfl_down(v, d, 64);
-            u += __shfl_down(u, d, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            wsum[0][threadIdx.x >> 6] = v;
-            wsum[1][threadIdx.x >> 6] = u;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long b = 0, c = 0;
-            for (int w = 0; w < kBlock / 64; ++w) {
-                b += wsum[0][w];
-                c += wsum[1][w];
-            }
-            block_pairs[blockIdx.x] = b;
-            block_pairs[nblocks + blockIdx.x] = c;
-        }
+        if (i != link[lf].x) nsrc = 0;
     }
+    if (block_pairs) p2p_block_stats(pairs, nsrc, block_pairs, blockIdx.x, stride);
     if (!valid) return;
-    {
-#pragma clang fp contract(off)
-        if (kMode == 0) {  // L2P (first write, H:1230-1232) then the P2P sum
-            const float4 fl = loc[lf];
-            fx[i] = p.w * fl.y + p.w * ax;
-            fy[i] = p.w * fl.z + p.w * ay;
-            fz[i] = p.w * fl.w + p.w * az;
-        } else if (kMode == 1) {
-            const float4 fl = loc[lf];
-            fx[i] = p.w * fl.y;
-            fy[i] = p.w * fl.z;
-            fz[i] = p.w * fl.w;
-        } else {
-            fx[i] = fx[i] + p.w * ax;
-            fy[i] = fy[i] + p.w * ay;
-            fz[i] = fz[i] + p.w * az;
-        }
+    if (kKahan) {
+        ax -= cx;
+        ay -= cy;
+        az -= cz;
     }
+    p2p_store<kMode>(i, p, (kMode != 2) ? loc[lf] : make_float4(0.f, 0.f, 0.f, 0.f), ax, ay, az,
+                     fx, fy, fz);
 }
 
-template <int kMode>
+// List of the big leaves: a block scan over the depth-10 nodes, one atomic
+// add per block for its slice of the list (big[0] = count, ids from big[1]).
+// The order of the slices does not matter: every leaf is computed alone.
+constexpr int kCollect = 1024;
+
+__global__ __launch_bounds__(kCollect) void p2p_collect_kernel(uint32_t lbeg, uint32_t lend,
+                                                               uint32_t tmin,
+                                                               const uint2* __restrict__ link,
+                                                               const uint32_t* __restrict__ nend,
+                                                               uint32_t* __restrict__ big) {
+    __shared__ uint32_t wcnt[kCollect / 64];
+    __shared__ uint32_t base;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t node = lbeg + blockIdx.x * kCollect + threadIdx.x;
+    const bool take = node < lend && nend[node] - link[node].x >= tmin;
+    const unsigned long long b = __ballot(take);
+    if (lane == 0) wcnt[wave] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kCollect / 64; ++w) {
+            const uint32_t c = wcnt[w];
+            wcnt[w] = t;
+            t += c;
+        }
+        base = t ? atomicAdd(&big[0], t) : 0u;
+    }
+    __syncthreads();
+    if (take) big[1 + base + wcnt[wave] + (uint32_t)__popcll(b & ps_lanemask_lt())] = node;
+}
+
+// One wavefront per big leaf (depth-10, >= tmin particles), 64 targets per
+// round: the whole wave shares the leaf's source list, so the loop runs on
+// scalar control flow and each source particle is one scalar load broadcast
+// to the 64 lanes.  Wave w takes list entry w (the grid covers every depth-10
+// node; the hardware balances the uneven leaves).
+constexpr int kBigWaves = kBlock / 64;
+
+template <int kMode, bool kKahan>
+__global__ __launch_bounds__(kBlock) void p2p_big_kernel(
+    size_t n, const float4* __restrict__ pos4, const float4* __restrict__ loc,
+    const uint32_t* __restrict__ p2p_off, const uint32_t* __restrict__ p2p_len,
+    const uint32_t* __restrict__ p2p_moff, const uint32_t* __restrict__ arena,
+    const unsigned long long* __restrict__ masks, const uint2* __restrict__ link,
+    const uint32_t* __restrict__ nend, float* __restrict__ fx, float* __restrict__ fy,
+    float* __restrict__ fz, const uint32_t* __restrict__ big, unsigned long long* block_pairs,
+    uint32_t slot0, uint32_t stride) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * kBigWaves + (threadIdx.x >> 6));
+    uint32_t pairs = 0, nsrc = 0;
+    if (w < big[0]) {
+        const uint32_t node = big[1 + w];
+        const uint32_t t0 = link[node].x;
+        const uint32_t te = nend[node] < (uint32_t)n ? nend[node] : (uint32_t)n;
+        const uint32_t L = (kMode != 1) ? p2p_len[node] : 0u;
+        const uint32_t o = L ? p2p_off[node] : 0u;
+        const uint32_t mo = L ? p2p_moff[node] : 0u;
+        const float4 fl = (kMode != 2) ? loc[node] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t c0 = t0; c0 < te; c0 += 64) {
+            const uint32_t i = c0 + lane;
+            const bool valid = i < te;
+            const float4 p = valid ? pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float ax = 0.f, ay = 0.f, az = 0.f;
+            float cx = 0.f, cy = 0.f, cz = 0.f;
+            uint32_t wp = 0, ws = 0;
+            p2p_uniform_list<kKahan>(p, n, L, o, mo, pos4, arena, masks, link, nend, ax, ay, az,
+                                     cx, cy, cz, wp, ws);
+            if (valid) {
+                pairs += wp;
+                if (kKahan) {
+                    ax -= cx;
+                    ay -= cy;
+                    az -= cz;
+                }
+                p2p_store<kMode>(i, p, fl, ax, ay, az, fx, fy, fz);
+            }
+            if (c0 == t0 && lane == 0) nsrc = ws;
+        }
+    }
+    if (block_pairs) p2p_block_stats(pairs, nsrc, block_pairs, slot0 + blockIdx.x, stride);
+}
+
+uint32_t big_blocks(uint32_t n_nodes10) { return (n_nodes10 + kBigWaves - 1) / kBigWaves; }
+
+template <int kMode, bool kKahan>
 void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                  const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                  const uint32_t* arena, const unsigned long long* masks, const uint2* link,
-                 const uint32_t* nend, float* fx, float* fy, float* fz, int kahan,
-                 unsigned long long* bp, hipStream_t s) {
-    unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
-    if (kahan)
-        p2p_kernel<kMode, true><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
-                                                          p2p_len, p2p_moff, arena, masks, link,
-                                                          nend, fx, fy, fz, bp, blocks);
-    else
-        p2p_kernel<kMode, false><<<blocks, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off,
-                                                           p2p_len, p2p_moff, arena, masks, link,
-                                                           nend, fx, fy, fz, bp, blocks);
+                 const uint32_t* nend, float* fx, float* fy, float* fz, uint32_t lbeg,
+                 uint32_t lend, uint32_t tmin, const uint32_t* big, unsigned long long* bp,
+                 hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join) {
+    const uint32_t nb1 = (uint32_t)((n + kBlock - 1) / kBlock);
+    const uint32_t nb2 = big_blocks(lend - lbeg);
+    // the two kernels write disjoint particles: the big-leaf one runs on its
+    // own stream beside the other (fork / join through events)
+    const bool two = nb2 && s_big && fork && join;
+    if (nb2) {
+        if (two) {
+            (void)hipEventRecord(fork, s);
+            (void)hipStreamWaitEvent(s_big, fork, 0);
+        }
+        p2p_big_kernel<kMode, kKahan><<<nb2, kBlock, 0, two ? s_big : s>>>(
+            n, pos4, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link, nend, fx, fy, fz, big,
+            bp, nb1, nb1 + nb2);
+    }
+    p2p_kernel<kMode, kKahan><<<nb1, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off, p2p_len,
+                                                     p2p_moff, arena, masks, link, nend, fx, fy,
+                                                     fz, lbeg, lend, tmin, bp, nb1 + nb2);
+    if (two) {
+        (void)hipEventRecord(join, s_big);
+        (void)hipStreamWaitEvent(s, join, 0);
+    }
 }
 
 }  // namespace
 
-uint32_t ps_p2p_blocks(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+uint32_t ps_p2p_blocks(size_t n, uint32_t n_big_nodes) {
+    return (uint32_t)((n + kBlock - 1) / kBlock) + big_blocks(n_big_nodes);
+}
 
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                    const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
+                   uint32_t lbeg, uint32_t lend, uint32_t tmin, uint32_t* big,
                    unsigned long long* block_pairs, unsigned long long* pair_count,
-                   hipStream_t s) {
+                   hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join) {
     if (n == 0) return;
+    if (tmin == 0 || lend < lbeg || !big) lend = lbeg;  // no big-leaf kernel
     unsigned long long* bp = pair_count ? block_pairs : nullptr;
-    if (mode == 0)
-        launch_mode<0>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
-                       nend, fx, fy, fz, kahan, bp, s);
-    else if (mode == 1)
-        launch_mode<1>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
-                       nend, fx, fy, fz, kahan, bp, s);
-    else
-        launch_mode<2>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link,
-                       nend, fx, fy, fz, kahan, bp, s);
+    if (lend > lbeg) {
+        ps_fill_u32(big, 0u, 1, s);
+        p2p_collect_kernel<<<(lend - lbeg + kCollect - 1) / kCollect, kCollect, 0, s>>>(
+            lbeg, lend, tmin, link, nend, big);
+    }
+#define PS_P2P(M, K)                                                                           \
+    launch_mode<M, K>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link, \
+                      nend, fx, fy, fz, lbeg, lend, tmin, big, bp, s, s_big, fork, join)
+    switch (mode * 2 + (kahan ? 1 : 0)) {
+        case 0: PS_P2P(0, false); break;
+        case 1: PS_P2P(0, true); break;
+        case 2: PS_P2P(1, false); break;
+        case 3: PS_P2P(1, true); break;
+        case 4: PS_P2P(2, false); break;
+        default: PS_P2P(2, true); break;
+    }
+#undef PS_P2P
     if (bp) {
-        const uint32_t nb = ps_p2p_blocks(n);
+        const uint32_t nb = ps_p2p_blocks(n, lend - lbeg);
         ps_reduce_u64(bp, nb, pair_count, s);
         ps_reduce_u64(bp + nb, nb, pair_count + 1, s);
     }

@@ -215,21 +215,28 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-// one block; each thread issues kRedV independent loads per round
+// Sum of n u64 values into *out: each block sums its slice (kRedV
+// independent loads per thread per round) and adds it to *out, which the
+// launcher clears first.
 constexpr int kRedThreads = 1024;
 constexpr int kRedV = 8;
+constexpr size_t kRedSlice = (size_t)kRedThreads * kRedV * 4;  // values per block
+
+__global__ void clear_u64_kernel(unsigned long long* p) { *p = 0ull; }
 
 __global__ __launch_bounds__(kRedThreads) void reduce_u64_kernel(const unsigned long long* in,
                                                                  size_t n,
                                                                  unsigned long long* out) {
     __shared__ unsigned long long ws[kRedThreads / 64];
+    const size_t lo = (size_t)blockIdx.x * kRedSlice;
+    const size_t hi = lo + kRedSlice < n ? lo + kRedSlice : n;
     unsigned long long v = 0;
-    for (size_t base = 0; base < n; base += (size_t)kRedThreads * kRedV) {
+    for (size_t base = lo; base < hi; base += (size_t)kRedThreads * kRedV) {
         unsigned long long t[kRedV];
 #pragma unroll
         for (int j = 0; j < kRedV; ++j) {
             const size_t i = base + (size_t)j * kRedThreads + threadIdx.x;
-            t[j] = i < n ? in[i] : 0ull;
+            t[j] = i < hi ? in[i] : 0ull;
         }
 #pragma unroll
         for (int j = 0; j < kRedV; ++j) v += t[j];
@@ -241,7 +248,7 @@ __global__ __launch_bounds__(kRedThreads) void reduce_u64_kernel(const unsigned 
     if (threadIdx.x == 0) {
         unsigned long long s = 0;
         for (int w = 0; w < kRedThreads / 64; ++w) s += ws[w];
-        *out = s;
+        atomicAdd(out, s);
     }
 }
 
@@ -349,5 +356,7 @@ void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 
 void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* out,
                    hipStream_t s) {
-    reduce_u64_kernel<<<1, kRedThreads, 0, s>>>(in, n, out);
+    const unsigned blocks = (unsigned)((n + kRedSlice - 1) / kRedSlice);
+    clear_u64_kernel<<<1, 1, 0, s>>>(out);
+    if (blocks) reduce_u64_kernel<<<blocks, kRedThreads, 0, s>>>(in, n, out);
 }

@@ -81,6 +81,30 @@ def test_traversal_lane_groups(oracle, lanes):
     assert_tick_parity(g, o)
 
 
+def dense_clusters(n, seed):
+    # two tight clusters in a wide uniform background: depth-10 leaves of up to
+    # a few hundred particles (several 64-target rounds, partial last rounds)
+    rng = np.random.default_rng(seed)
+    k = n // 3
+    a = rng.normal(0.0, 0.01, (k, 3))
+    b = rng.normal(0.0, 0.02, (k, 3)) + [1.0, 0.5, -0.5]
+    c = rng.uniform(-4.0, 4.0, (n - 2 * k, 3))
+    p = np.concatenate([a, b, c]).astype(np.float32)
+    m = (rng.random(n) + 0.5).astype(np.float32) / n
+    return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), m
+
+
+@pytest.mark.parametrize("p2p_big", [0, 1, 32, 200])
+def test_p2p_big_leaf_kernel(oracle, p2p_big):
+    # the wavefront-per-leaf P2P path and the thread-per-particle path give the
+    # reference's pair set and forces for every split between them
+    x, y, z, m = dense_clusters(60_000, seed=51)
+    o = oracle.tick(x, y, z, m, theta=2.0)
+    g = gpu_tick(x, y, z, m, theta=2.0, p2p_big=p2p_big)
+    f, _ = assert_tick_parity(g, o)
+    log(f"  p2p_big={p2p_big} force rms {f:.2e} pairs {g['stats']['p2p_pairs']}")
+
+
 @pytest.mark.parametrize("theta", [0.5, 1.0, 3.0, 6.0])
 def test_other_theta(oracle, theta):
     # theta < ~1.04 reproduces the reference's depth-2 defect (a target can take

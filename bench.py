@@ -66,6 +66,21 @@ def workload(name):
     return n, x, y, z, m, v, desc
 
 
+def measured_traffic(workload, family):
+    """Calibrated HBM bytes per tick of a kernel family from the committed
+    rocprofv3 --pmc summary (profiles/traffic_<workload>.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                t = line.split()
+                if len(t) == 3 and t[0] == "traffic_bytes_per_tick" and t[1] == family:
+                    return float(t[2])
+    except OSError:
+        pass
+    return None
+
+
 def fp64_error(n, pos_sorted_forces, ids, ref_fn, seed=4096):
     """RMS / max relative error on 4096 seeded original indices."""
     from polesitter_amd import workloads as W
@@ -91,17 +106,22 @@ def cpu_baseline(n, x, y, z, m, theta, ticks, gpu_ref_fn):
     if O.ref_available("shipped_mt"):
         build, kind = "shipped_mt", "reference"
         t0 = time.time()
-        secs = O.ref_time_ticks(x, y, z, m, theta=theta, thrd_cnt=thr, n_ticks=ticks + 1,
-                                build=build)
+        secs = []
+        for k in range(ticks + 1):  # one call per tick, each from the initial state
+            secs.append(float(O.ref_time_ticks(x, y, z, m, theta=theta, thrd_cnt=thr,
+                                               n_ticks=1, build=build)[0]))
+            log(f"bench: reference CPU tick {k + 1}/{ticks + 1}: {secs[-1]:.3f} s")
         wall = time.time() - t0
         tick_s = float(np.median(secs[1:]))
         # the reference's error on the same initial state
         r = O.ref_tick(x, y, z, m, theta=theta, thrd_cnt=thr, build=build, dump=False)
+        log("bench: reference CPU tick for the FP64 error done")
         f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
         err = fp64_error(n, f, r["id"], lambda pos: gpu_ref_fn(r, pos))
-        sample = (f"{ticks} ticks (+1 warm-up) of the full workload, median; "
+        sample = (f"{ticks} ticks (+1 warm-up) of the full workload from its initial state, "
+                  f"median; "
                   f"src/polesitter.h built -O3 -ffast-math -march=x86-64-v3 -DPS_MULTITHREADING, "
-                  f"thrd_cnt={thr}; {wall:.1f} s of CPU time")
+                  f"thrd_cnt={thr}; {wall:.1f} s wall on {thr} threads")
         return dict(value=n / tick_s, unit="particles/s", cores=thr, kind=kind, sample=sample,
                     tick_s=tick_s, fp64_rms_err=err[0], fp64_max_err=err[1])
     # fallback: the single-threaded C restatement
@@ -122,6 +142,10 @@ def main(argv=None):
     ap.add_argument("--theta", type=float, default=2.0)
     ap.add_argument("--kahan", action="store_true")
     ap.add_argument("--trav-lanes", type=int, default=8, choices=(8, 16, 32, 64))
+    ap.add_argument("--morton", default="avx2", choices=("avx2", "trunc"),
+                    help="quantization of ps_prepare_particles: the reference's AVX2 path "
+                         "(round to nearest, the shipped build the CPU baseline runs) or its "
+                         "scalar path (truncation)")
     ap.add_argument("--p2p-big", type=int, default=24,
                     help="P2P wavefront-per-leaf threshold in particles (0 = off)")
     ap.add_argument("--cpu-ticks", type=int, default=3)
@@ -146,9 +170,11 @@ def main(argv=None):
     tmp = api.DeviceArray(3 * n, np.float32)
     dt = 1e-3
 
+    mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
+
     def step():
         s.reset_ids()
-        s.tick()
+        s.tick(mmode)
         api.check("gather", L.ps_hip_step_gather(s.ctx.engine, s.id.ptr, vel[0].ptr, vel[1].ptr,
                                                  vel[2].ptr, tmp.ptr, n))
         api.check("euler", L.ps_hip_step_euler(s.ctx.engine, s.x.ptr, s.y.ptr, s.z.ptr,
@@ -160,7 +186,7 @@ def main(argv=None):
     err = None
     if not a.no_error:
         s.reset_ids()
-        s.tick()
+        s.tick(mmode)
         r = s.result()
         f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
         err = fp64_error(n, f, r["id"], s.direct_f64)
@@ -192,13 +218,21 @@ def main(argv=None):
     }
     dom = max(kern, key=lambda k: kern[k]["ms"])
     ach = kern[dom]["flop"] / (kern[dom]["ms"] * 1e-3) / 1e12
+    traffic = measured_traffic(a.workload, dom)
     roof = dict(kernel=dom, bound="mfma", achieved=round(ach, 4), peak=FP32_PEAK_TFLOPS,
-                unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 6), traffic=None,
+                unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 6),
+                traffic=None if traffic is None else round(traffic),
                 algorithmic=("41 FLOP per M2L pair" if dom == "trav_step"
                              else "20 FLOP per particle pair"),
                 ms_per_tick=round(kern[dom]["ms"], 4),
+                launch="all of the family's dispatches in one tick (achieved = FLOP per tick / "
+                       "their summed duration from HIP events on the engine's stream)",
                 note="FP32 VALU work (no MFMA); peak is the published FP32 figure, the "
-                     "non-packed VALU ceiling is half of it")
+                     "non-packed VALU ceiling is half of it; traffic = calibrated "
+                     "FETCH_SIZE + WRITE_SIZE bytes per tick from profiles/traffic_" +
+                     a.workload + ".md")
+    if traffic is not None:
+        roof["traffic_GBps"] = round(traffic / (kern[dom]["ms"] * 1e-3) / 1e9, 1)
 
     line = dict(metric=METRIC, value=round(value, 1), unit="particles/s", n_gpus=1,
                 steps=a.steps, warmup=a.warmup, ms_per_step=round(ms_step, 4),
@@ -208,6 +242,7 @@ def main(argv=None):
                             theta=a.theta, traversal="MT depth-2 jobs (PS_MULTITHREADING)",
                             step="tick + id-gather + semi-implicit Euler dt=1e-3",
                             kahan=bool(a.kahan), trav_lanes=a.trav_lanes, p2p_big=a.p2p_big,
+                            morton=a.morton,
                             pointers="device-resident"),
                 roofline=roof,
                 stages_ms={k: round(mean("ms_" + k), 4) for k in

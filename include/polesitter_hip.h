@@ -129,6 +129,11 @@ int ps_hip_op_m2l(const float* tgeom4, const float* sgeom4, const float* smp4,
  * against a bit loop; out2 = (mismatches, sum of codes) */
 int ps_hip_op_morton_exhaustive(uint64_t* out2);
 
+/* no reference counterpart: a plain streaming copy of `bytes` (a multiple of
+ * 16) between two device buffers, synchronous -- the known-size kernel that
+ * calibrates the profiler's HBM byte counters */
+int ps_hip_op_copy(const void* src, void* dst, size_t bytes);
+
 /* no reference counterpart: FP64 direct sum (tests/accuracy.c:42-72) on the
  * device for nt target positions (host array) against all n sources (device
  * arrays); out3 (host) = 3 doubles per target. */

@@ -99,6 +99,7 @@ def hip_lib():
             "ps_hip_debug_set_local": ([_vp, C.c_uint32, _vp], C.c_int),
             "ps_hip_op_m2l": ([_vp, _vp, _vp, _f, _vp], C.c_int),
             "ps_hip_op_morton_exhaustive": ([_vp], C.c_int),
+            "ps_hip_op_copy": ([_vp, _vp, C.c_size_t], C.c_int),
             "ps_hip_direct_f64": ([_vp, _vp, _vp, _vp, _sz, _vp, _sz, _vp], C.c_int),
             "ps_hip_step_reset_ids": ([_vp, _vp, _sz], C.c_int),
             "ps_hip_step_gather": ([_vp, _vp, _vp, _vp, _vp, _vp, _sz], C.c_int),

@@ -757,6 +757,14 @@ int ps_hip_op_morton_exhaustive(uint64_t* out2) {
     return map_err(r);
 }
 
+int ps_hip_op_copy(const void* src, void* dst, size_t bytes) {
+    if (!src || !dst || (bytes & 15u) || !is_device_ptr(src) || !is_device_ptr(dst))
+        return PS_HIP_EINVAL;
+    PS_CK(ps_op_copy(src, dst, bytes, nullptr));
+    PS_CK(hipDeviceSynchronize());
+    return PS_HIP_OK;
+}
+
 int ps_hip_direct_f64(const float* x, const float* y, const float* z, const float* mass,
                       size_t n, const uint32_t* targets, size_t nt, double* out3) {
     if (!x || !y || !z || !mass || !targets || !out3 || n == 0) return PS_HIP_EINVAL;

@@ -162,3 +162,5 @@ void ps_launch_iota(uint32_t* id, size_t n, hipStream_t s);
 hipError_t ps_op_m2l(const float* tgeom, const float* sgeom, const float* smp, float theta,
                      float* out4, hipStream_t s);
 hipError_t ps_op_morton_exhaustive(unsigned long long* out2, hipStream_t s);
+// plain copy of bytes (a multiple of 16) between device buffers
+hipError_t ps_op_copy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -151,6 +151,8 @@ def main(argv=None):
     ap.add_argument("--cpu-ticks", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--host-ticks", type=int, default=3,
+                    help="ticks timed with host (pageable numpy) arrays, PCIe included; 0 = skip")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -253,6 +255,34 @@ def main(argv=None):
                             iterations=int(stats[-1]["iterations"])))
     if err:
         line["fp64_rms_err"], line["fp64_max_err"] = err
+    if a.host_ticks > 0:
+        # the same tick through host arrays: ps_calc_forces stages them through
+        # device buffers and copies the sorted state and forces back
+        hx, hy, hz, hm = (np.ascontiguousarray(t, np.float32).copy() for t in (x, y, z, m))
+        hf = [np.zeros(n, np.float32) for _ in range(3)]
+        hid = np.arange(n, dtype=np.uint32)
+        hcodes = np.zeros(n, np.uint32)
+        harrs = api.make_arrs(hx, hy, hz, hm, hf[0], hf[1], hf[2], hid, n)
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 0))
+
+        def host_tick():
+            hid[:] = np.arange(n, dtype=np.uint32)
+            rc, lo, _, rg = api.ps_prepare_particles(harrs, hcodes, mmode)
+            api.check("ps_prepare_particles", rc)
+            c = np.float32(np.float32(lo) + np.float32(rg) / np.float32(2.0))
+            api.check("ps_calc_forces", api.ps_calc_forces(s.ctx, harrs, hcodes, c, c, c,
+                                                           np.float32(rg) / np.float32(2.0)))
+
+        host_tick()
+        t0 = time.perf_counter()
+        for _ in range(a.host_ticks):
+            host_tick()
+        th = (time.perf_counter() - t0) / a.host_ticks
+        line["host_pointers"] = dict(value=round(n / th, 1), unit="particles/s",
+                                     ms_per_tick=round(1e3 * th, 4), ticks=a.host_ticks,
+                                     note="prepare + calc_forces on pageable host arrays "
+                                          "(uploads, tick, sorted state and forces copied "
+                                          "back); not the headline value")
     if not a.no_cpu_baseline:
         def ref_fn(r, pos):
             # FP64 direct sum on the device over the reference's sorted state

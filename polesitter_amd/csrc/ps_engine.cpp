@@ -204,6 +204,7 @@ struct ps_hip_engine {
         a.y = reinterpret_cast<float*>(base + 3 * max_p * 4);
         a.z = reinterpret_cast<float*>(base + 4 * max_p * 4);
         a.m = reinterpret_cast<float*>(base + 5 * max_p * 4);
+        a.aux = nullptr;
         return a;
     }
 
@@ -292,7 +293,8 @@ int stage_sort(ps_hip_engine* e) {
 
 int stage_build(ps_hip_engine* e) {
     uint32_t* small = e->small_u32();
-    PS_CK(ps_tree_count(e->code, e->n, small, small + 12, e->totals(), e->tree_tmp.p, e->s));
+    PS_CK(ps_tree_count(e->code, nullptr, e->n, small, small + 12, e->totals(), e->tree_tmp.p,
+                        e->s));
     PS_CK(hipMemcpyAsync(e->h_small, e->small_b.p, 96, hipMemcpyDeviceToHost, e->s));
     PS_CK(hipStreamSynchronize(e->s));
     const uint32_t flags = e->h_small[12];
@@ -305,8 +307,8 @@ int stage_build(ps_hip_engine* e) {
     e->max_depth = 0;
     for (int d = 0; d <= 10; ++d)
         if (e->ls[d + 1] > e->ls[d]) e->max_depth = (uint32_t)d;
-    PS_CK(ps_tree_write(e->code, e->n, e->root, e->tree_tmp.p, e->geom(), e->link(), e->nend(),
-                        e->parent(), e->leaf_of(), e->n_nodes, e->s));
+    PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
+                        e->link(), e->nend(), e->parent(), e->leaf_of(), e->n_nodes, e->s));
     ps_fill_u32(reinterpret_cast<uint32_t*>(e->loc()), 0u, 4 * (size_t)e->n_nodes, e->s);
     ps_fill_u32(e->p2p_len(), 0u, e->n_nodes, e->s);
     e->built = true;
@@ -321,13 +323,9 @@ int stage_upward(ps_hip_engine* e) {
 }
 
 // Upper bound on the entries of iteration k: MT targets at iteration k lie
-// at depths 1..k+2, ST targets at depths 0..k.
+// at depths 0..k+2, ST targets at depths 0..k.
 uint32_t entries_bound(const ps_hip_engine* e, bool mt, int k) {
-    if (mt) {
-        const int d = std::min(k + 3, 11);
-        return e->ls[d] - e->ls[1];
-    }
-    return e->ls[std::min(k + 1, 11)];
+    return e->ls[std::min(k + (mt ? 3 : 1), 11)];
 }
 
 // Launch the whole traversal: one kernel per iteration, sizes in device
@@ -339,12 +337,9 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     PsTravState* st = reinterpret_cast<PsTravState*>(e->tstate_b.p);
     const uint32_t arena_cap = (uint32_t)std::min<size_t>(e->arena_b.bytes / 4, 0xFFFFFFFFull);
     const uint32_t mask_cap = (uint32_t)std::min<size_t>(e->masks_b.bytes / 8, 0xFFFFFFFFull);
-    if (mt)
-        ps_launch_trav_init(e->ls[1], e->ls[3], e->ls[2], e->link(), e->entries(0), st,
-                            e->arena(), arena_cap, mask_cap, e->s);
-    else
-        ps_launch_trav_init(0u, 1u, 0u, e->link(), e->entries(0), st, e->arena(), arena_cap,
-                            mask_cap, e->s);
+    ps_launch_trav_init(e->small_u32(), mt ? 1 : 0, 1u, nullptr, e->nend(), e->link(),
+                        e->entries(0), st, e->arena(), arena_cap, mask_cap,
+                        mt ? e->ls[3] : 1u, e->s);
     PsTravTree t;
     t.n_nodes = n_nodes;
     t.geom = e->geom();
@@ -376,7 +371,7 @@ int stage_traverse(ps_hip_engine* e) {
         volatile float t = e->theta;  // fl(theta * theta), H:1064
         theta2 = t * t;
     }
-    const bool mt = e->trav == PS_HIP_TRAV_MT && e->n_nodes > 1;
+    const bool mt = e->trav == PS_HIP_TRAV_MT;
     const int iters = (int)e->max_depth + 1;
     const size_t M = e->n_nodes;
     for (int b = 0; b < 2; ++b)
@@ -385,7 +380,7 @@ int stage_traverse(ps_hip_engine* e) {
     PS_CK(grow(e->tstate_b, sizeof(PsTravState), e->s));
     // first use: room for ~96 list words per particle; a larger need is met
     // by growing and running the traversal again
-    const size_t n0 = std::max<size_t>(e->ls[3] - e->ls[1], 1) + 16;
+    const size_t n0 = std::max<size_t>(e->ls[3], 1) + 16;
     PS_CK(grow(e->arena_b, 4 * std::max<size_t>(96 * e->n + 4096, 1 << 16), e->s));
     PS_CK(grow(e->masks_b, 8 * std::max<size_t>(4 * M + n0 + 4096, n0), e->s));
 
@@ -422,18 +417,18 @@ int stage_l2l(ps_hip_engine* e) {
 
 int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
     unsigned long long* pc = e->count_on ? e->pairs() : nullptr;
-    // the depth-10 level holds every leaf of more than one particle
-    const uint32_t lbeg = e->max_depth == 10 ? e->ls[10] : 0u;
-    const uint32_t lend = e->max_depth == 10 ? e->ls[11] : 0u;
+    // the depth-10 level holds every leaf of more than one particle (its range
+    // is read on the device: ls[10], ls[11])
+    const uint32_t bound10 = e->ls[11] - e->ls[10];
     if (e->count_on)
-        PS_CK(grow(e->block_pairs_b, 16 * (size_t)ps_p2p_blocks(e->n, lend - lbeg), e->s));
+        PS_CK(grow(e->block_pairs_b, 16 * (size_t)ps_p2p_blocks(e->n, bound10), e->s));
     if (!e->arena_b.p) PS_CK(grow(e->arena_b, 256, e->s));
     if (!e->masks_b.p) PS_CK(grow(e->masks_b, 256, e->s));
-    PS_CK(grow(e->big_b, 4 * ((size_t)(lend - lbeg) + 1), e->s));
+    PS_CK(grow(e->big_b, 4 * ((size_t)bound10 + 1), e->s));
     if (ev_before >= 0) e->record(ev_before);
     ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
                   e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
-                  e->fz, mode, e->kahan, lbeg, lend, e->p2p_big,
+                  e->fz, mode, e->kahan, e->small_u32() + 10, bound10, e->p2p_big,
                   reinterpret_cast<uint32_t*>(e->big_b.p),
                   reinterpret_cast<unsigned long long*>(e->block_pairs_b.p), pc, e->s, e->s2,
                   e->fork, e->join);

@@ -38,6 +38,13 @@ void ps_launch_bbox(const float* x, const float* y, const float* z, size_t n, fl
 // mode 1: AVX2 rounding for the first 8*floor(n/8), scalar tail (H:2040-2072)
 void ps_launch_morton(const float* x, const float* y, const float* z, size_t n,
                       const float* prm, uint32_t* codes, int mode, hipStream_t s);
+// Many systems in one set of arrays, system k = particles [off[k], off[k+1])
+// (device array, every system non-empty): per system its cube prm[k] = (min,
+// max, range, 1023/range) and root cell root[k] = (min + range/2 (x3),
+// range/2); sys[i] = system of particle i; codes with each system's cube.
+void ps_launch_prepare_seg(const float* x, const float* y, const float* z, size_t n,
+                           const uint32_t* off, uint32_t n_sys, float4* prm, float4* root,
+                           uint32_t* sys, uint32_t* codes, int mode, hipStream_t s);
 
 // ---- ps_sort.hip: stable LSD radix sort (3 x 10-bit digits), H:1588-1693 ----
 struct PsSortArrays {
@@ -47,11 +54,13 @@ struct PsSortArrays {
     float* y;
     float* z;
     float* m;
+    uint32_t* aux;  // optional second key (system id, < 2^10), null when unused
 };
 size_t ps_sort_tmp_bytes(size_t n);
 // Sorts `io` in place by key bits 0..29 (keys must be < 2^30), moving the
 // payload; t0/t1 are two scratch sets of n elements.  pos4 (optional) gets the
-// sorted (x, y, z, m).
+// sorted (x, y, z, m).  With aux set (in all three sets) a fourth pass sorts
+// stably by aux: the order is (aux, key).
 hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
                          size_t n, void* tmp, hipStream_t s);
 
@@ -59,13 +68,22 @@ hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, floa
 size_t ps_tree_tmp_bytes(size_t n);
 // Phase 1: per-depth node counts.  Writes level_start[0..11] (device) and
 // flags[0] |= 1 if any key has bits 30/31 set.  total64[0] = node count.
-hipError_t ps_tree_count(const uint32_t* codes, size_t n, uint32_t* level_start,
-                         uint32_t* flags, unsigned long long* total64, void* tmp,
+// sys (optional): the system of each particle (sorted by (system, code));
+// each system then gets its own root, in system order at depth 0.
+hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
+                         uint32_t* level_start, uint32_t* flags, unsigned long long* total64,
+                         void* tmp, hipStream_t s);
+// Phase 2: writes every node (needs phase 1's tmp untouched).  The root cell
+// is `root`, or roots[sys[i]] per system when roots is set.
+hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, float4 root,
+                         const float4* roots, void* tmp, float4* geom, uint2* link,
+                         uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, size_t n_nodes,
                          hipStream_t s);
-// Phase 2: writes every node (needs phase 1's tmp untouched).
-hipError_t ps_tree_write(const uint32_t* codes, size_t n, float4 root, void* tmp,
-                         float4* geom, uint2* link, uint32_t* nend, uint32_t* parent,
-                         uint32_t* leaf_of, size_t n_nodes, hipStream_t s);
+// Upward (depths 10..0) and L2L (depths 1..10) with the level starts read
+// from device memory ls[0..11]; fixed grids, no host-side sizes.
+void ps_launch_upward_dev(const uint32_t* ls, const float4* pos4, const float4* geom,
+                          const uint2* link, const uint32_t* nend, float4* mp, hipStream_t s);
+void ps_launch_l2l_dev(const uint32_t* ls, const uint32_t* parent, float4* loc, hipStream_t s);
 void ps_launch_upward_level(uint32_t begin, uint32_t end, const float4* pos4,
                             const float4* geom, const uint2* link, const uint32_t* nend,
                             float4* mp, hipStream_t s);
@@ -104,11 +122,15 @@ struct PsTravState {
     unsigned long long mask_ctr;   // mask words reserved (or asked for)
     unsigned long long m2l_pairs;  // filled by a reduction of the per-block far counts
 };
-// iteration-0 entries: depth-1 leaves and all depth-2 nodes (MT) with list
-// {root}; resets the state (arena word 0 = root, one mask word per entry)
-void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* link,
-                         PsTravEntries ent, PsTravState* st, uint32_t* arena,
-                         uint32_t arena_cap, uint32_t mask_cap, hipStream_t s);
+// Iteration-0 entries from the level starts ls[0..11] in device memory.  MT:
+// every node of depth <= 2 is an entry, with the list {its system's root} at
+// depth 2 or for a leaf above it; ST: the roots, each with itself.  Roots are
+// nodes [0, n_sys); sys (optional) = system of each particle.  Resets the
+// state.  max_ent bounds the entry count (grid size only).
+void ps_launch_trav_init(const uint32_t* ls, int mt, uint32_t n_sys, const uint32_t* sys,
+                         const uint32_t* nend, const uint2* link, PsTravEntries ent,
+                         PsTravState* st, uint32_t* arena, uint32_t arena_cap, uint32_t mask_cap,
+                         uint32_t max_ent, hipStream_t s);
 // Blocks of the iteration launch for max_ent entries (length of its far_part
 // slice; never more than ps_trav_blocks(UINT32_MAX, 8)).
 uint32_t ps_trav_blocks(uint32_t max_ent, int lanes);
@@ -125,21 +147,22 @@ void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------
 // mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);
 // 2: f += m*sum (P2P only).  A leaf's near sources are the non-far entries of
-// its final traversal list.  Leaves in [lbeg, lend) (the depth-10 nodes) with
-// >= tmin particles go to a wavefront-per-leaf kernel, the rest to a
-// thread-per-particle kernel (tmin = 0: all of them).  big: scratch of
-// 1 + (lend - lbeg) words for the list of those leaves.  With s_big, fork and
+// its final traversal list.  Leaves in [lrange[0], lrange[1]) (device memory:
+// the depth-10 level) with >= tmin particles go to a wavefront-per-leaf kernel,
+// the rest to a thread-per-particle kernel (tmin = 0: all of them).  bound10
+// bounds the depth-10 node count (grid sizes only); big: scratch of
+// 1 + bound10 words for the list of the big leaves.  With s_big, fork and
 // join set, the big-leaf kernel runs on s_big beside the other; all work is
 // ordered after earlier work on s and before later work on s.
 // pair_count (stats, may be null): per-block partial sums in block_pairs
-// (>= 2 * ps_p2p_blocks(n, lend - lbeg) entries), reduced into pair_count[0]
+// (>= 2 * ps_p2p_blocks(n, bound10) entries), reduced into pair_count[0]
 // (particle pairs) and pair_count[1] (target leaf x source leaf pairs).
-uint32_t ps_p2p_blocks(size_t n, uint32_t n_big_nodes);
+uint32_t ps_p2p_blocks(size_t n, uint32_t bound10);
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                    const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
-                   uint32_t lbeg, uint32_t lend, uint32_t tmin, uint32_t* big,
+                   const uint32_t* lrange, uint32_t bound10, uint32_t tmin, uint32_t* big,
                    unsigned long long* block_pairs, unsigned long long* pair_count,
                    hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join);
 

@@ -153,9 +153,10 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
     const uint32_t* __restrict__ p2p_len, const uint32_t* __restrict__ p2p_moff,
     const uint32_t* __restrict__ arena, const unsigned long long* __restrict__ masks,
     const uint2* __restrict__ link, const uint32_t* __restrict__ nend, float* __restrict__ fx,
-    float* __restrict__ fy, float* __restrict__ fz, uint32_t lbeg, uint32_t lend,
+    float* __restrict__ fy, float* __restrict__ fz, const uint32_t* __restrict__ lrange,
     uint32_t tmin, unsigned long long* block_pairs, uint32_t stride) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lbeg = lrange[0], lend = tmin ? lrange[1] : lrange[0];
     float ax = 0.f, ay = 0.f, az = 0.f;
     float cx = 0.f, cy = 0.f, cz = 0.f;
     uint32_t pairs = 0, nsrc = 0;  // particle pairs; near source leaves
@@ -218,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
 // The order of the slices does not matter: every leaf is computed alone.
 constexpr int kCollect = 1024;
 
-__global__ __launch_bounds__(kCollect) void p2p_collect_kernel(uint32_t lbeg, uint32_t lend,
+__global__ __launch_bounds__(kCollect) void p2p_collect_kernel(const uint32_t* __restrict__ lrange,
                                                                uint32_t tmin,
                                                                const uint2* __restrict__ link,
                                                                const uint32_t* __restrict__ nend,
@@ -226,30 +227,36 @@ __global__ __launch_bounds__(kCollect) void p2p_collect_kernel(uint32_t lbeg, ui
     __shared__ uint32_t wcnt[kCollect / 64];
     __shared__ uint32_t base;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t node = lbeg + blockIdx.x * kCollect + threadIdx.x;
-    const bool take = node < lend && nend[node] - link[node].x >= tmin;
-    const unsigned long long b = __ballot(take);
-    if (lane == 0) wcnt[wave] = (uint32_t)__popcll(b);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (int w = 0; w < kCollect / 64; ++w) {
-            const uint32_t c = wcnt[w];
-            wcnt[w] = t;
-            t += c;
+    const uint32_t lbeg = lrange[0], lend = lrange[1];
+    for (uint32_t n0 = lbeg + blockIdx.x * kCollect; n0 < lend; n0 += gridDim.x * kCollect) {
+        const uint32_t node = n0 + threadIdx.x;
+        const bool take = node < lend && nend[node] - link[node].x >= tmin;
+        const unsigned long long b = __ballot(take);
+        if (lane == 0) wcnt[wave] = (uint32_t)__popcll(b);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int w = 0; w < kCollect / 64; ++w) {
+                const uint32_t c = wcnt[w];
+                wcnt[w] = t;
+                t += c;
+            }
+            base = t ? atomicAdd(&big[0], t) : 0u;
         }
-        base = t ? atomicAdd(&big[0], t) : 0u;
+        __syncthreads();
+        if (take) big[1 + base + wcnt[wave] + (uint32_t)__popcll(b & ps_lanemask_lt())] = node;
+        __syncthreads();  // wcnt and base are rewritten by the next round
     }
-    __syncthreads();
-    if (take) big[1 + base + wcnt[wave] + (uint32_t)__popcll(b & ps_lanemask_lt())] = node;
 }
 
 // One wavefront per big leaf (depth-10, >= tmin particles), 64 targets per
 // round: the whole wave shares the leaf's source list, so the loop runs on
 // scalar control flow and each source particle is one scalar load broadcast
-// to the 64 lanes.  Wave w takes list entry w (the grid covers every depth-10
-// node; the hardware balances the uneven leaves).
+// to the 64 lanes.  Wave w takes list entries w, w + waves, ...; the grid is
+// large (up to kBigBlocks), so the hardware balances the uneven leaves.
 constexpr int kBigWaves = kBlock / 64;
+constexpr uint32_t kBigBlocks = 65536;
+constexpr uint32_t kCollectBlocks = 2048;
 
 template <int kMode, bool kKahan>
 __global__ __launch_bounds__(kBlock) void p2p_big_kernel(
@@ -261,9 +268,10 @@ __global__ __launch_bounds__(kBlock) void p2p_big_kernel(
     float* __restrict__ fz, const uint32_t* __restrict__ big, unsigned long long* block_pairs,
     uint32_t slot0, uint32_t stride) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * kBigWaves + (threadIdx.x >> 6));
+    const uint32_t nbig = big[0];
     uint32_t pairs = 0, nsrc = 0;
-    if (w < big[0]) {
+    for (uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * kBigWaves + (threadIdx.x >> 6));
+         w < nbig; w += gridDim.x * kBigWaves) {
         const uint32_t node = big[1 + w];
         const uint32_t t0 = link[node].x;
         const uint32_t te = nend[node] < (uint32_t)n ? nend[node] : (uint32_t)n;
@@ -289,23 +297,26 @@ __global__ __launch_bounds__(kBlock) void p2p_big_kernel(
                 }
                 p2p_store<kMode>(i, p, fl, ax, ay, az, fx, fy, fz);
             }
-            if (c0 == t0 && lane == 0) nsrc = ws;
+            if (c0 == t0 && lane == 0) nsrc += ws;
         }
     }
     if (block_pairs) p2p_block_stats(pairs, nsrc, block_pairs, slot0 + blockIdx.x, stride);
 }
 
-uint32_t big_blocks(uint32_t n_nodes10) { return (n_nodes10 + kBigWaves - 1) / kBigWaves; }
+uint32_t big_blocks(uint32_t n_nodes10) {
+    const uint32_t b = (n_nodes10 + kBigWaves - 1) / kBigWaves;
+    return b < kBigBlocks ? b : kBigBlocks;
+}
 
 template <int kMode, bool kKahan>
 void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                  const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                  const uint32_t* arena, const unsigned long long* masks, const uint2* link,
-                 const uint32_t* nend, float* fx, float* fy, float* fz, uint32_t lbeg,
-                 uint32_t lend, uint32_t tmin, const uint32_t* big, unsigned long long* bp,
+                 const uint32_t* nend, float* fx, float* fy, float* fz, const uint32_t* lrange,
+                 uint32_t bound10, uint32_t tmin, const uint32_t* big, unsigned long long* bp,
                  hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join) {
     const uint32_t nb1 = (uint32_t)((n + kBlock - 1) / kBlock);
-    const uint32_t nb2 = big_blocks(lend - lbeg);
+    const uint32_t nb2 = tmin ? big_blocks(bound10) : 0u;
     // the two kernels write disjoint particles: the big-leaf one runs on its
     // own stream beside the other (fork / join through events)
     const bool two = nb2 && s_big && fork && join;
@@ -320,7 +331,7 @@ void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const fl
     }
     p2p_kernel<kMode, kKahan><<<nb1, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off, p2p_len,
                                                      p2p_moff, arena, masks, link, nend, fx, fy,
-                                                     fz, lbeg, lend, tmin, bp, nb1 + nb2);
+                                                     fz, lrange, nb2 ? tmin : 0u, bp, nb1 + nb2);
     if (two) {
         (void)hipEventRecord(join, s_big);
         (void)hipStreamWaitEvent(s, join, 0);
@@ -329,28 +340,29 @@ void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const fl
 
 }  // namespace
 
-uint32_t ps_p2p_blocks(size_t n, uint32_t n_big_nodes) {
-    return (uint32_t)((n + kBlock - 1) / kBlock) + big_blocks(n_big_nodes);
+uint32_t ps_p2p_blocks(size_t n, uint32_t bound10) {
+    return (uint32_t)((n + kBlock - 1) / kBlock) + big_blocks(bound10);
 }
 
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                    const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
-                   uint32_t lbeg, uint32_t lend, uint32_t tmin, uint32_t* big,
+                   const uint32_t* lrange, uint32_t bound10, uint32_t tmin, uint32_t* big,
                    unsigned long long* block_pairs, unsigned long long* pair_count,
                    hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join) {
     if (n == 0) return;
-    if (tmin == 0 || lend < lbeg || !big) lend = lbeg;  // no big-leaf kernel
+    if (!big || bound10 == 0) tmin = 0;  // no big-leaf kernel
     unsigned long long* bp = pair_count ? block_pairs : nullptr;
-    if (lend > lbeg) {
+    if (tmin) {
         ps_fill_u32(big, 0u, 1, s);
-        p2p_collect_kernel<<<(lend - lbeg + kCollect - 1) / kCollect, kCollect, 0, s>>>(
-            lbeg, lend, tmin, link, nend, big);
+        uint32_t cb = (bound10 + kCollect - 1) / kCollect;
+        p2p_collect_kernel<<<cb < kCollectBlocks ? cb : kCollectBlocks, kCollect, 0, s>>>(
+            lrange, tmin, link, nend, big);
     }
 #define PS_P2P(M, K)                                                                           \
     launch_mode<M, K>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link, \
-                      nend, fx, fy, fz, lbeg, lend, tmin, big, bp, s, s_big, fork, join)
+                      nend, fx, fy, fz, lrange, bound10, tmin, big, bp, s, s_big, fork, join)
     switch (mode * 2 + (kahan ? 1 : 0)) {
         case 0: PS_P2P(0, false); break;
         case 1: PS_P2P(0, true); break;
@@ -361,7 +373,7 @@ void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const 
     }
 #undef PS_P2P
     if (bp) {
-        const uint32_t nb = ps_p2p_blocks(n, lend - lbeg);
+        const uint32_t nb = ps_p2p_blocks(n, tmin ? bound10 : 0u);
         ps_reduce_u64(bp, nb, pair_count, s);
         ps_reduce_u64(bp + nb, nb, pair_count + 1, s);
     }

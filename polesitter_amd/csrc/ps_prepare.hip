@@ -122,7 +122,93 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const float* __restrict_
     codes[i] = ps_morton3(qx, qy, qz);
 }
 
+// ---- many systems in one set of arrays: system s is [off[s], off[s+1]) ----
+
+constexpr int kSegBlock = 1024;
+
+// one block per system: its bounding cube, root cell and quantisation
+// parameters (as bbox_final_kernel), prm[s] = (min, max, range, 1023/range),
+// root[s] = (min + range/2 (x3), range/2)
+__global__ __launch_bounds__(kSegBlock) void bbox_seg_kernel(const float* __restrict__ x,
+                                                             const float* __restrict__ y,
+                                                             const float* __restrict__ z,
+                                                             const uint32_t* __restrict__ off,
+                                                             float4* __restrict__ prm,
+                                                             float4* __restrict__ root) {
+    __shared__ float slo[kSegBlock / 64], shi[kSegBlock / 64];
+    const uint32_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    for (uint32_t i = b + threadIdx.x; i < e; i += kSegBlock) {
+        float a = x[i], c = y[i], d = z[i];
+        lo = fminf(lo, fminf(a, fminf(c, d)));
+        hi = fmaxf(hi, fmaxf(a, fmaxf(c, d)));
+    }
+    wave_minmax(lo, hi);
+    if ((threadIdx.x & 63) == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma clang fp contract(off)
+        for (int w = 1; w < kSegBlock / 64; ++w) {
+            lo = fminf(lo, slo[w]);
+            hi = fmaxf(hi, shi[w]);
+        }
+        float range = hi - lo;  // H:2008-2011
+        if (range < 0.001f) range = 0.001f;
+        prm[blockIdx.x] = make_float4(lo, hi, range, 1023.0f / range);
+        const float half = range * 0.5f;  // README.md:104-126: root at min + range/2
+        const float c = lo + half;
+        root[blockIdx.x] = make_float4(c, c, c, half);
+    }
+}
+
+// sys[i] = the system of particle i
+__global__ __launch_bounds__(kSegBlock) void sys_fill_kernel(const uint32_t* __restrict__ off,
+                                                             uint32_t* __restrict__ sys) {
+    const uint32_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+    for (uint32_t i = b + threadIdx.x; i < e; i += kSegBlock) sys[i] = blockIdx.x;
+}
+
+// Morton codes with each system's own cube; the AVX2 mode covers the first
+// 8 floor(N_s / 8) particles of each system, as a separate prepare call would
+__global__ __launch_bounds__(kBlock) void morton_seg_kernel(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    size_t n, const uint32_t* __restrict__ sys, const uint32_t* __restrict__ off,
+    const float4* __restrict__ prm, uint32_t* __restrict__ codes, int mode) {
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = sys[i];
+    const float4 p = prm[s];
+    const uint32_t b = off[s], e = off[s + 1];
+    const uint32_t nvec = b + ((e - b) & ~7u);
+    float vx = x[i], vy = y[i], vz = z[i];
+    uint32_t qx, qy, qz;
+    if (mode == 1 && i < nvec) {
+        qx = q_rne(vx, p.x, p.w);
+        qy = q_rne(vy, p.x, p.w);
+        qz = q_rne(vz, p.x, p.w);
+    } else {
+        qx = q_trunc(vx, p.x, p.z);
+        qy = q_trunc(vy, p.x, p.z);
+        qz = q_trunc(vz, p.x, p.z);
+    }
+    codes[i] = ps_morton3(qx, qy, qz);
+}
+
 }  // namespace
+
+void ps_launch_prepare_seg(const float* x, const float* y, const float* z, size_t n,
+                           const uint32_t* off, uint32_t n_sys, float4* prm, float4* root,
+                           uint32_t* sys, uint32_t* codes, int mode, hipStream_t s) {
+    if (n == 0 || n_sys == 0) return;
+    bbox_seg_kernel<<<n_sys, kSegBlock, 0, s>>>(x, y, z, off, prm, root);
+    sys_fill_kernel<<<n_sys, kSegBlock, 0, s>>>(off, sys);
+    morton_seg_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(x, y, z, n, sys,
+                                                                              off, prm, codes,
+                                                                              mode);
+}
 
 void ps_launch_bbox(const float* x, const float* y, const float* z, size_t n, float2* part,
                     float* prm, hipStream_t s) {

@@ -14,7 +14,9 @@
 //             to consecutive addresses; the payload (id, x, y, z, mass) is
 //             gathered from the tile's own input range, which is L2-resident.
 // Tiles are 4096 keys (256 threads x 16).  Ping-pong: io -> t0 -> t1 -> io,
-// so the sorted data ends in the caller's arrays (H:1688-1690).
+// so the sorted data ends in the caller's arrays (H:1688-1690).  Many systems
+// sorted at once carry their system id as a second key: a fourth, stable pass
+// by it groups the systems again, each in code order.
 #include "ps_device.h"
 #include "ps_kernels.h"
 
@@ -50,8 +52,9 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
     const float* __restrict__ xin, const float* __restrict__ yin, const float* __restrict__ zin,
     const float* __restrict__ min_, uint32_t* __restrict__ kout, uint32_t* __restrict__ iout,
     float* __restrict__ xout, float* __restrict__ yout, float* __restrict__ zout,
-    float* __restrict__ mout, float4* __restrict__ pos4, size_t n, int shift,
-    const uint32_t* __restrict__ offs, uint32_t n_tiles) {
+    float* __restrict__ mout, float4* __restrict__ pos4, const uint32_t* __restrict__ ain,
+    uint32_t* __restrict__ aout, size_t n, int shift, const uint32_t* __restrict__ offs,
+    uint32_t n_tiles) {
     __shared__ uint32_t keys[kTile];
     __shared__ uint16_t sidx[kTile];
     __shared__ uint32_t tstart[kRadix];
@@ -162,6 +165,7 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
         zout[g] = vz;
         mout[g] = vm;
         if (pos4) pos4[g] = make_float4(vx, vy, vz, vm);
+        if (aout) aout[g] = ain[src];
     }
 }
 
@@ -182,19 +186,27 @@ hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, floa
     const size_t table_n = (size_t)kRadix * tiles;
     uint32_t* table = reinterpret_cast<uint32_t*>(tmp);
     void* scan_tmp = reinterpret_cast<char*>(tmp) + align256(table_n * sizeof(uint32_t));
-    const PsSortArrays* src[3] = {&io, &t0, &t1};
-    const PsSortArrays* dst[3] = {&t0, &t1, &io};
-    for (int pass = 0; pass < 3; ++pass) {
-        const int shift = pass * kRadixBits;
+    // three passes over the code (io -> t0 -> t1 -> io); with a second key,
+    // io -> t0 -> t1 -> t0 by code, then t0 -> io by aux (code carried)
+    const bool two = io.aux && t0.aux && t1.aux;
+    const PsSortArrays* src[4] = {&io, &t0, &t1, &t0};
+    const PsSortArrays* dst[4] = {&t0, &t1, two ? &t0 : &io, &io};
+    const int passes = two ? 4 : 3;
+    for (int pass = 0; pass < passes; ++pass) {
+        const bool by_aux = pass == 3;
+        const int shift = by_aux ? 0 : pass * kRadixBits;
         const PsSortArrays& a = *src[pass];
         const PsSortArrays& b = *dst[pass];
-        sort_upsweep_kernel<<<tiles, kBlock, 0, s>>>(a.key, n, shift, table, tiles);
+        const uint32_t* kin = by_aux ? a.aux : a.key;
+        uint32_t* kout = by_aux ? b.aux : b.key;
+        const uint32_t* ain = two ? (by_aux ? a.key : a.aux) : nullptr;
+        uint32_t* aout = two ? (by_aux ? b.key : b.aux) : nullptr;
+        sort_upsweep_kernel<<<tiles, kBlock, 0, s>>>(kin, n, shift, table, tiles);
         hipError_t e = ps_scan_u32(table, table, table_n, nullptr, scan_tmp, s);
         if (e != hipSuccess) return e;
-        sort_downsweep_kernel<<<tiles, kBlock, 0, s>>>(a.key, a.id, a.x, a.y, a.z, a.m, b.key,
-                                                       b.id, b.x, b.y, b.z, b.m,
-                                                       pass == 2 ? pos4 : nullptr, n, shift,
-                                                       table, tiles);
+        sort_downsweep_kernel<<<tiles, kBlock, 0, s>>>(
+            kin, a.id, a.x, a.y, a.z, a.m, kout, b.id, b.x, b.y, b.z, b.m,
+            pass == passes - 1 ? pos4 : nullptr, ain, aout, n, shift, table, tiles);
     }
     return hipGetLastError();
 }

@@ -53,30 +53,45 @@ __device__ __forceinline__ uint32_t trav_range(uint32_t n_e, uint32_t nblk, uint
     return r > groups ? r : groups;
 }
 
-__global__ __launch_bounds__(kBlock) void trav_init_kernel(uint32_t ls1, uint32_t ls2,
-                                                           uint32_t ls3,
+// Iteration-0 entries.  The level starts come from device memory.  MT (the
+// reference's threaded build, H:1772-1801): entry k is node k for every node
+// of depth <= 2, with the list {root of its system} when the node is at depth 2
+// or a leaf above it, else empty.  ST: entry k is root k with the list {k}.
+// Roots are nodes [0, n_sys) (one per system, in system order); arena word s
+// holds root s.
+__global__ __launch_bounds__(kBlock) void trav_init_kernel(const uint32_t* __restrict__ ls,
+                                                           int mt, uint32_t n_sys,
+                                                           const uint32_t* __restrict__ sys,
+                                                           const uint32_t* __restrict__ nend,
                                                            const uint2* __restrict__ link,
                                                            PsTravEntries ent, PsTravState* st,
                                                            uint32_t* arena, uint32_t arena_cap,
                                                            uint32_t mask_cap) {
-    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t n = ls1 + k;
-    if (k == 0) {
-        PsTravState z = {};
-        z.n_ent[0] = ls3 - ls1;
-        z.arena_ctr = 1;  // arena word 0 holds the list {root}
-        z.mask_ctr = ls3 - ls1;
-        z.arena_cap = arena_cap;
-        z.mask_cap = mask_cap;
-        *st = z;
-        arena[0] = 0;
+    const uint32_t n_e = mt ? ls[3] : n_sys;
+    const uint32_t ls2 = ls[2];
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n_e || k < n_sys;
+         k += gridDim.x * kBlock) {
+        if (k == 0) {
+            PsTravState z = {};
+            z.n_ent[0] = n_e;
+            z.arena_ctr = n_sys;  // the root lists
+            z.mask_ctr = n_e;     // one mask word per entry
+            z.arena_cap = arena_cap;
+            z.mask_cap = mask_cap;
+            *st = z;
+        }
+        if (k < n_sys) arena[k] = k;
+        if (k >= n_e) continue;
+        ent.tgt[k] = k;
+        ent.moff[k] = k;
+        if (mt) {
+            ent.off[k] = sys ? sys[nend[k] - 1u] : 0u;
+            ent.len[k] = (k >= ls2 || (link[k].y & PS_LEAF)) ? 1u : 0u;
+        } else {
+            ent.off[k] = k;
+            ent.len[k] = 1u;
+        }
     }
-    if (n >= ls3) return;
-    ent.tgt[k] = n;
-    ent.off[k] = 0;
-    // jobs: every node at depth 2 and every leaf above it (H:1780)
-    ent.len[k] = (n >= ls2 || (link[n].y & PS_LEAF)) ? 1u : 0u;
-    ent.moff[k] = k;  // one mask word per entry
 }
 
 __device__ __forceinline__ uint32_t chunks_of(uint32_t len) { return (len + 63u) >> 6; }
@@ -438,13 +453,16 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
 
 }  // namespace
 
-void ps_launch_trav_init(uint32_t ls1, uint32_t ls3, uint32_t ls2, const uint2* link,
-                         PsTravEntries ent, PsTravState* st, uint32_t* arena,
-                         uint32_t arena_cap, uint32_t mask_cap, hipStream_t s) {
-    uint32_t n = ls3 > ls1 ? ls3 - ls1 : 1u;
+void ps_launch_trav_init(const uint32_t* ls, int mt, uint32_t n_sys, const uint32_t* sys,
+                         const uint32_t* nend, const uint2* link, PsTravEntries ent,
+                         PsTravState* st, uint32_t* arena, uint32_t arena_cap, uint32_t mask_cap,
+                         uint32_t max_ent, hipStream_t s) {
+    uint32_t n = max_ent > n_sys ? max_ent : n_sys;
     uint32_t blocks = (n + kBlock - 1) / kBlock;
-    trav_init_kernel<<<blocks, kBlock, 0, s>>>(ls1, ls2, ls3, link, ent, st, arena, arena_cap,
-                                               mask_cap);
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) blocks = 1;
+    trav_init_kernel<<<blocks, kBlock, 0, s>>>(ls, mt, n_sys, sys, nend, link, ent, st, arena,
+                                               arena_cap, mask_cap);
 }
 
 namespace {

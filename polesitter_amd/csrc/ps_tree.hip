@@ -42,10 +42,14 @@ struct PartDepths {
     int end_lo;    // ends at depths [end_lo, leaf]
 };
 
-__device__ __forceinline__ PartDepths part_depths(const uint32_t* codes, size_t n, size_t i,
-                                                  uint32_t c) {
-    int kp = (i > 0) ? octal_lcp(codes[i - 1], c) : -1;
-    int kn = (i + 1 < n) ? octal_lcp(c, codes[i + 1]) : -1;
+// sys (optional): system of each particle; a system boundary shares no digit,
+// so every system opens its own root (the forest of a batched tick)
+__device__ __forceinline__ PartDepths part_depths(const uint32_t* codes,
+                                                  const uint32_t* __restrict__ sys, size_t n,
+                                                  size_t i, uint32_t c) {
+    const uint32_t si = sys ? sys[i] : 0u;
+    int kp = (i > 0 && (!sys || sys[i - 1] == si)) ? octal_lcp(codes[i - 1], c) : -1;
+    int kn = (i + 1 < n && (!sys || sys[i + 1] == si)) ? octal_lcp(c, codes[i + 1]) : -1;
     int K = kp > kn ? kp : kn;
     PartDepths r;
     r.leaf = (K + 1) < PS_MAX_DEPTH ? (K + 1) : PS_MAX_DEPTH;
@@ -55,6 +59,7 @@ __device__ __forceinline__ PartDepths part_depths(const uint32_t* codes, size_t 
 }
 
 __global__ __launch_bounds__(kBlock) void tree_count_kernel(const uint32_t* __restrict__ codes,
+                                                            const uint32_t* __restrict__ sys,
                                                             size_t n, uint32_t* __restrict__ cnt,
                                                             uint32_t nb, uint32_t* flags) {
     __shared__ uint32_t ws[kWaves][kDepths];
@@ -64,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void tree_count_kernel(const uint32_t* __re
     if (valid) {
         uint32_t c = codes[i];
         if (c >> 30) atomicOr(flags, 1u);
-        pd = part_depths(codes, n, i, c);
+        pd = part_depths(codes, sys, n, i, c);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -88,8 +93,9 @@ __global__ void tree_levels_kernel(const uint32_t* off, uint32_t nb,
 }
 
 __global__ __launch_bounds__(kBlock) void tree_write_kernel(
-    const uint32_t* __restrict__ codes, size_t n, const uint32_t* __restrict__ off, uint32_t nb,
-    float4 root, float4* __restrict__ geom, uint2* __restrict__ link,
+    const uint32_t* __restrict__ codes, const uint32_t* __restrict__ sys, size_t n,
+    const uint32_t* __restrict__ off, uint32_t nb, float4 root,
+    const float4* __restrict__ roots, float4* __restrict__ geom, uint2* __restrict__ link,
     uint32_t* __restrict__ nend, uint32_t* __restrict__ parent, uint32_t* __restrict__ leaf_of) {
 #pragma clang fp contract(off)
     __shared__ uint32_t ws[kWaves][kDepths];
@@ -99,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
     PartDepths pd = {-1, kDepths, kDepths};
     if (valid) {
         c = codes[i];
-        pd = part_depths(codes, n, i, c);
+        pd = part_depths(codes, sys, n, i, c);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t le = ps_lanemask_lt() | (1ull << lane);
@@ -122,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
     }
 
     uint32_t* link32 = reinterpret_cast<uint32_t*>(link);
-    float4 g = root;
+    float4 g = roots ? roots[sys[i]] : root;
 #pragma unroll
     for (int d = 0; d < kDepths; ++d) {
         if (d > pd.leaf) continue;
@@ -211,7 +217,82 @@ __global__ __launch_bounds__(kBlock) void l2l_level_kernel(uint32_t begin, uint3
     loc[nidx] = cl;
 }
 
+// the same two passes with the level range read from device memory
+// (ls[d], ls[d+1]) and a fixed grid: nothing depends on host-side sizes, so
+// the launches can be recorded in a graph
+__global__ __launch_bounds__(kBlock) void upward_level_dev_kernel(
+    const uint32_t* __restrict__ ls, int d, const float4* __restrict__ pos4,
+    const float4* __restrict__ geom, const uint2* __restrict__ link,
+    const uint32_t* __restrict__ nend, float4* __restrict__ mp) {
+#pragma clang fp contract(off)
+    const uint32_t begin = ls[d], end = ls[d + 1];
+    for (uint32_t nidx = begin + blockIdx.x * kBlock + threadIdx.x; nidx < end;
+         nidx += gridDim.x * kBlock) {
+        const float4 g = geom[nidx];
+        const uint2 l = link[nidx];
+        float m0 = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+        if (l.y & PS_LEAF) {
+            const uint32_t e = nend[nidx];
+            for (uint32_t i = l.x; i < e; ++i) {
+                float4 p = pos4[i];
+                float dx = p.x - g.x;
+                float dy = p.y - g.y;
+                float dz = p.z - g.z;
+                m0 += p.w;
+                mx += p.w * dx;
+                my += p.w * dy;
+                mz += p.w * dz;
+            }
+        } else {
+            uint32_t ch = l.x;
+            for (uint32_t sl = 0; sl < 8; ++sl) {
+                if (!((l.y >> sl) & 1u)) continue;
+                float4 cg = geom[ch];
+                float4 cm = mp[ch];
+                ++ch;
+                float dx = cg.x - g.x;
+                float dy = cg.y - g.y;
+                float dz = cg.z - g.z;
+                m0 += cm.x;
+                mx += cm.y + cm.x * dx;
+                my += cm.z + cm.x * dy;
+                mz += cm.w + cm.x * dz;
+            }
+        }
+        mp[nidx] = make_float4(m0, mx, my, mz);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void l2l_level_dev_kernel(const uint32_t* __restrict__ ls,
+                                                               int d,
+                                                               const uint32_t* __restrict__ parent,
+                                                               float4* __restrict__ loc) {
+    const uint32_t begin = ls[d], end = ls[d + 1];
+    for (uint32_t nidx = begin + blockIdx.x * kBlock + threadIdx.x; nidx < end;
+         nidx += gridDim.x * kBlock) {
+        float4 pl = loc[parent[nidx]];
+        float4 cl = loc[nidx];
+        cl.y = cl.y + pl.y;
+        cl.z = cl.z + pl.z;
+        cl.w = cl.w + pl.w;
+        loc[nidx] = cl;
+    }
+}
+
+constexpr uint32_t kDevGrid = 2048;  // blocks of the device-range launches
+
 }  // namespace
+
+void ps_launch_upward_dev(const uint32_t* ls, const float4* pos4, const float4* geom,
+                          const uint2* link, const uint32_t* nend, float4* mp, hipStream_t s) {
+    for (int d = PS_MAX_DEPTH; d >= 0; --d)
+        upward_level_dev_kernel<<<kDevGrid, kBlock, 0, s>>>(ls, d, pos4, geom, link, nend, mp);
+}
+
+void ps_launch_l2l_dev(const uint32_t* ls, const uint32_t* parent, float4* loc, hipStream_t s) {
+    for (int d = 1; d <= PS_MAX_DEPTH; ++d)
+        l2l_level_dev_kernel<<<kDevGrid, kBlock, 0, s>>>(ls, d, parent, loc);
+}
 
 size_t ps_tree_tmp_bytes(size_t n) {
     size_t nb = (n + kBlock - 1) / kBlock;
@@ -220,29 +301,30 @@ size_t ps_tree_tmp_bytes(size_t n) {
     return align256(t * sizeof(uint32_t)) + ps_scan_tmp_bytes_u32(t);
 }
 
-hipError_t ps_tree_count(const uint32_t* codes, size_t n, uint32_t* level_start,
-                         uint32_t* flags, unsigned long long* total64, void* tmp,
-                         hipStream_t s) {
+hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
+                         uint32_t* level_start, uint32_t* flags, unsigned long long* total64,
+                         void* tmp, hipStream_t s) {
     const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
     uint32_t* cnt = reinterpret_cast<uint32_t*>(tmp);
     const size_t t = (size_t)kDepths * nb;
     void* scan_tmp = reinterpret_cast<char*>(tmp) + align256(t * sizeof(uint32_t));
     ps_fill_u32(flags, 0u, 1, s);
-    tree_count_kernel<<<nb, kBlock, 0, s>>>(codes, n, cnt, nb, flags);
+    tree_count_kernel<<<nb, kBlock, 0, s>>>(codes, sys, n, cnt, nb, flags);
     hipError_t e = ps_scan_u32(cnt, cnt, t, total64, scan_tmp, s);
     if (e != hipSuccess) return e;
     tree_levels_kernel<<<1, 64, 0, s>>>(cnt, nb, total64, level_start);
     return hipGetLastError();
 }
 
-hipError_t ps_tree_write(const uint32_t* codes, size_t n, float4 root, void* tmp,
-                         float4* geom, uint2* link, uint32_t* nend, uint32_t* parent,
-                         uint32_t* leaf_of, size_t n_nodes, hipStream_t s) {
+hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, float4 root,
+                         const float4* roots, void* tmp, float4* geom, uint2* link,
+                         uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, size_t n_nodes,
+                         hipStream_t s) {
     const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
     const uint32_t* off = reinterpret_cast<const uint32_t*>(tmp);
     ps_fill_u32(reinterpret_cast<uint32_t*>(link), 0u, 2 * n_nodes, s);
-    tree_write_kernel<<<nb, kBlock, 0, s>>>(codes, n, off, nb, root, geom, link, nend, parent,
-                                            leaf_of);
+    tree_write_kernel<<<nb, kBlock, 0, s>>>(codes, sys, n, off, nb, root, roots, geom, link,
+                                            nend, parent, leaf_of);
     return hipGetLastError();
 }
 

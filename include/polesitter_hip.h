@@ -71,6 +71,23 @@ typedef struct {
     float ms_trav_steps;
 } ps_hip_stats_t;
 
+typedef struct ps_hip_batch ps_hip_batch_t;
+
+/* statistics of the last batched tick */
+typedef struct {
+    uint64_t n_particles;
+    uint64_t n_systems;
+    uint64_t n_nodes;       /* nodes of the whole forest */
+    uint64_t m2l_pairs;
+    uint64_t graph_replays; /* ticks run by replaying the recorded graph so far */
+    uint32_t used_graph;    /* 1: the last tick was a graph replay */
+    float ms_tick;          /* device time of the last tick (HIP events) */
+} ps_hip_batch_stats_t;
+
+/* batch options (ps_hip_batch_set_option; the engine options apply too,
+ * except PS_HIP_OPT_STATS) */
+#define PS_HIP_BATCH_OPT_GRAPH 100 /* 1 (default): record the tick once, replay it */
+
 /* ps_init, H:1699-1767: device state for up to max_particles particles and
  * max_nodes octree nodes on the current device. */
 int ps_hip_engine_create(ps_hip_engine_t** out, size_t max_particles,
@@ -151,6 +168,27 @@ int ps_hip_step_euler(ps_hip_engine_t* e, float* x, float* y, float* z,
                       float* vx, float* vy, float* vz, const float* fx,
                       const float* fy, const float* fz, const float* mass,
                       float dt, size_t n);
+
+/* no reference counterpart (batched entry point): one tick of many
+ * independent systems -- for each, ps_prepare_particles (H:1911-2170) and
+ * ps_calc_forces (H:1840-1909) with the root at min + range/2
+ * (README.md:104-126).  The systems are consecutive segments of one set of
+ * device arrays: system k is [offsets[k], offsets[k+1]) (offsets: host array
+ * of n_systems + 1, offsets[0] = 0, every system non-empty).  Each segment is
+ * sorted in place and gets its forces in sorted order, as separate calls
+ * would produce them.  The whole tick is one stream-ordered sequence with
+ * every size in device memory: the first tick of a layout runs directly and
+ * is recorded as a HIP graph, later ticks with the same arrays, offsets and
+ * mode replay it.  morton_mode: PS_HIP_MORTON_TRUNC / _AVX2. */
+int ps_hip_batch_create(ps_hip_batch_t** out, size_t max_particles,
+                        size_t max_systems, float theta, int traversal);
+int ps_hip_batch_destroy(ps_hip_batch_t* b);
+int ps_hip_batch_set_option(ps_hip_batch_t* b, int key, double value);
+int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z,
+                      float* mass, float* fx, float* fy, float* fz,
+                      uint32_t* id, uint32_t* codes, const uint32_t* offsets,
+                      size_t n_systems, int morton_mode);
+int ps_hip_batch_get_stats(const ps_hip_batch_t* b, ps_hip_batch_stats_t* out);
 
 /* no reference counterpart: device memory for callers without a HIP runtime
  * of their own (kind: 1 host->device, 2 device->host, 3 device->device) */

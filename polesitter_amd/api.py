@@ -30,6 +30,7 @@ RESULT_NAMES = {0: "PS_OK", -1: "PS_EOOM", -2: "PS_EINVAL", -3: "PS_ETHRD", -4: 
 TRAV_MT, TRAV_ST = 0, 1
 MORTON_TRUNC, MORTON_AVX2 = 0, 1
 OPT_TRAVERSAL, OPT_KAHAN, OPT_THETA, OPT_STATS, OPT_TRAV_LANES, OPT_P2P_BIG = 1, 2, 3, 4, 5, 6
+BATCH_OPT_GRAPH = 100
 
 
 class PsError(RuntimeError):
@@ -49,6 +50,12 @@ class PsParticleArrs(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
                 ("mass", C.c_void_p), ("fx", C.c_void_p), ("fy", C.c_void_p),
                 ("fz", C.c_void_p), ("id", C.c_void_p), ("cnt", C.c_size_t)]
+
+
+class PsBatchStats(C.Structure):
+    _fields_ = [("n_particles", C.c_uint64), ("n_systems", C.c_uint64), ("n_nodes", C.c_uint64),
+                ("m2l_pairs", C.c_uint64), ("graph_replays", C.c_uint64),
+                ("used_graph", C.c_uint32), ("ms_tick", C.c_float)]
 
 
 class PsStats(C.Structure):
@@ -100,6 +107,11 @@ def hip_lib():
             "ps_hip_op_m2l": ([_vp, _vp, _vp, _f, _vp], C.c_int),
             "ps_hip_op_morton_exhaustive": ([_vp], C.c_int),
             "ps_hip_op_copy": ([_vp, _vp, C.c_size_t], C.c_int),
+            "ps_hip_batch_create": ([C.POINTER(_vp), _sz, _sz, _f, C.c_int], C.c_int),
+            "ps_hip_batch_destroy": ([_vp], C.c_int),
+            "ps_hip_batch_set_option": ([_vp, C.c_int, C.c_double], C.c_int),
+            "ps_hip_batch_tick": ([_vp] * 11 + [_sz, C.c_int], C.c_int),
+            "ps_hip_batch_get_stats": ([_vp, C.POINTER(PsBatchStats)], C.c_int),
             "ps_hip_direct_f64": ([_vp, _vp, _vp, _vp, _sz, _vp, _sz, _vp], C.c_int),
             "ps_hip_step_reset_ids": ([_vp, _vp, _sz], C.c_int),
             "ps_hip_step_gather": ([_vp, _vp, _vp, _vp, _vp, _vp, _sz], C.c_int),
@@ -383,6 +395,92 @@ class Solver:
             self.ctx = None
         for a in (self.x, self.y, self.z, self.m, self.fx, self.fy, self.fz, self.id, self.codes):
             a.free()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """Many independent systems in one set of device arrays (ps_hip_batch_*):
+    system k is particles [offsets[k], offsets[k+1]).  tick() sorts every
+    segment in place and writes its forces, as separate ticks would."""
+
+    def __init__(self, max_particles, max_systems, theta=2.0, traversal=TRAV_MT, graph=True,
+                 kahan=False):
+        L = hip_lib()
+        h = C.c_void_p()
+        check("ps_hip_batch_create", L.ps_hip_batch_create(C.byref(h), int(max_particles),
+                                                           int(max_systems), float(theta),
+                                                           int(traversal)))
+        self.h = h
+        self.max_particles = int(max_particles)
+        if not graph:
+            self.set_option(BATCH_OPT_GRAPH, 0)
+        if kahan:
+            self.set_option(OPT_KAHAN, 1)
+        n = self.max_particles
+        self.x, self.y, self.z, self.m = (DeviceArray(n, np.float32) for _ in range(4))
+        self.fx, self.fy, self.fz = (DeviceArray(n, np.float32) for _ in range(3))
+        self.id = DeviceArray(n, np.uint32)
+        self.codes = DeviceArray(n, np.uint32)
+        self.offsets = None
+
+    def set_option(self, key, value):
+        check("ps_hip_batch_set_option", hip_lib().ps_hip_batch_set_option(self.h, key, value))
+
+    def load(self, systems):
+        """systems: list of (x, y, z, m) float32 arrays; ids = index inside the system."""
+        sizes = [len(t[0]) for t in systems]
+        off = np.zeros(len(sizes) + 1, np.uint32)
+        off[1:] = np.cumsum(sizes)
+        n = int(off[-1])
+        assert n <= self.max_particles
+        cat = [np.concatenate([np.asarray(t[k], np.float32) for t in systems]) for k in range(4)]
+        ids = np.concatenate([np.arange(s_, dtype=np.uint32) for s_ in sizes])
+        self.n = n
+        self.offsets = off
+        self.host = dict(x=cat[0], y=cat[1], z=cat[2], m=cat[3], id=ids)
+        self.restore()
+
+    def restore(self):
+        """upload the loaded (unsorted) state again"""
+        h = self.host
+        n = self.n
+        for d, a in ((self.x, h["x"]), (self.y, h["y"]), (self.z, h["z"]), (self.m, h["m"]),
+                     (self.id, h["id"])):
+            check("ps_hip_memcpy", hip_lib().ps_hip_memcpy(d.ptr, a.ctypes.data, 4 * n, 1))
+
+    def tick(self, mode=MORTON_TRUNC):
+        o = np.ascontiguousarray(self.offsets, np.uint32)
+        check("ps_hip_batch_tick", hip_lib().ps_hip_batch_tick(
+            self.h, self.x.ptr, self.y.ptr, self.z.ptr, self.m.ptr, self.fx.ptr, self.fy.ptr,
+            self.fz.ptr, self.id.ptr, self.codes.ptr, o.ctypes.data, len(o) - 1, int(mode)))
+
+    def result(self):
+        n = self.n
+        out = {}
+        for k, d in (("x", self.x), ("y", self.y), ("z", self.z), ("m", self.m),
+                     ("fx", self.fx), ("fy", self.fy), ("fz", self.fz), ("id", self.id),
+                     ("codes", self.codes)):
+            out[k] = d.numpy()[:n]
+        return out
+
+    def system(self, res, k):
+        a, b = int(self.offsets[k]), int(self.offsets[k + 1])
+        return {key: v[a:b] for key, v in res.items()}
+
+    def stats(self):
+        st = PsBatchStats()
+        check("ps_hip_batch_get_stats", hip_lib().ps_hip_batch_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in PsBatchStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            hip_lib().ps_hip_batch_destroy(self.h)
+            self.h = None
 
     def __enter__(self):
         return self

@@ -6,11 +6,16 @@
 // node arrays for max_nodes, and traversal buffers grown on demand (kept
 // across ticks).  A tick is a stream-ordered kernel sequence,
 //   sort -> build -> upward -> traversal (M2L + P2P lists) -> L2L -> P2P+L2P
-// with host synchronisation only where sizes are data dependent (the node
-// count, and each traversal iteration's output sizes).
+// with two host synchronisations: the node count (checked against the node
+// capacity before the tree is written) and the traversal's overflow flag.
+//
+// A batch (ps_hip_batch_*) runs many independent systems as one forest with
+// every size in device memory, so its whole tick is recorded once in a HIP
+// graph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -118,6 +123,8 @@ size_t worst_case_nodes(size_t n) {
 }
 
 enum Stage { kSort = 1, kBuild = 2, kUpward = 3, kInteract = 4, kDownward = 5, kP2P = 6 };
+
+constexpr int PS_MAX_DEPTH_ITERS = 11;  // traversal iterations of a depth-10 tree
 
 }  // namespace
 
@@ -308,7 +315,8 @@ int stage_build(ps_hip_engine* e) {
     for (int d = 0; d <= 10; ++d)
         if (e->ls[d + 1] > e->ls[d]) e->max_depth = (uint32_t)d;
     PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
-                        e->link(), e->nend(), e->parent(), e->leaf_of(), e->n_nodes, e->s));
+                        e->link(), e->nend(), e->parent(), e->leaf_of(), e->n_nodes, nullptr,
+                        e->s));
     ps_fill_u32(reinterpret_cast<uint32_t*>(e->loc()), 0u, 4 * (size_t)e->n_nodes, e->s);
     ps_fill_u32(e->p2p_len(), 0u, e->n_nodes, e->s);
     e->built = true;
@@ -451,11 +459,143 @@ int run_tick(ps_hip_engine* e) {
     return PS_HIP_OK;
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batched ticks: many systems in one set of device arrays, one forest.
+// ---------------------------------------------------------------------------
+struct ps_hip_batch {
+    ps_hip_engine* e = nullptr;  // streams, options and the tick buffers
+    size_t max_sys = 0;
+    DevBuf off_b;                // u32[n_sys + 1] particle offsets of the systems
+    DevBuf sys_b;                // u32 per particle: its system (the sort's second key)
+    DevBuf aux_b;                // two u32 arrays: the second key's sort scratch
+    DevBuf prm_b, roots_b;       // float4 per system
+    size_t node_cap = 0;         // guaranteed bound for the current layout
+    int use_graph = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // what the captured graph was recorded for
+    std::vector<uint32_t> g_off;
+    void* g_ptr[9] = {nullptr};
+    size_t g_n = 0;
+    int g_mode = -1;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ps_hip_batch_stats_t st;
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        g_mode = -1;
+    }
+};
+
+namespace {
+
+// most nodes of a forest whose system k holds off[k+1] - off[k] particles
+size_t forest_node_bound(const std::vector<uint32_t>& off) {
+    size_t total = 0;
+    for (size_t k = 0; k + 1 < off.size(); ++k) total += worst_case_nodes(off[k + 1] - off[k]);
+    return total;
+}
+
+// every buffer of a batch tick over n particles and `cap` nodes; no
+// allocation may happen while a graph is being recorded
+int batch_ensure(ps_hip_batch* b, size_t n, size_t cap) {
+    ps_hip_engine* e = b->e;
+    hipStream_t s = e->s;
+    DevBuf* node16[] = {&e->geom_b, &e->mp_b, &e->loc_b};
+    for (DevBuf* d : node16) PS_CK(grow(*d, 16 * cap, s));
+    PS_CK(grow(e->link_b, 8 * cap, s));
+    DevBuf* node4[] = {&e->nend_b, &e->parent_b, &e->p2p_off_b, &e->p2p_len_b, &e->p2p_moff_b};
+    for (DevBuf* d : node4) PS_CK(grow(*d, 4 * cap, s));
+    for (int g = 0; g < 2; ++g)
+        for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[g][k], 4 * cap, s));
+    PS_CK(grow(e->far_b, 8 * (size_t)ps_trav_blocks(0xFFFFFFFFu, 8) * PS_MAX_DEPTH_ITERS, s));
+    PS_CK(grow(e->tstate_b, sizeof(PsTravState), s));
+    PS_CK(grow(e->arena_b, 4 * std::max<size_t>(96 * n + 4096, 1 << 16), s));
+    PS_CK(grow(e->masks_b, 8 * (4 * cap + 4096), s));
+    PS_CK(grow(e->big_b, 4 * (std::min(cap, n) + 1), s));
+    PS_CK(grow(b->sys_b, 4 * std::max<size_t>(n, 1), s));
+    PS_CK(grow(b->aux_b, 8 * std::max<size_t>(e->max_p, 1), s));
+    e->node_cap = std::max(e->node_cap, cap);
+    return PS_HIP_OK;
+}
+
+// The batch tick, launches only: per-system prepare, sort by (system,
+// code), forest build, upward, traversal (all PS_MAX_DEPTH_ITERS iterations;
+// the unused ones find no entries), L2L, P2P + L2P.
+int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
+    ps_hip_engine* e = b->e;
+    hipStream_t s = e->s;
+    const size_t n = e->n;
+    const size_t cap = b->node_cap;
+    const uint32_t* off = reinterpret_cast<const uint32_t*>(b->off_b.p);
+    uint32_t* sys = reinterpret_cast<uint32_t*>(b->sys_b.p);
+    float4* prm = reinterpret_cast<float4*>(b->prm_b.p);
+    float4* roots = reinterpret_cast<float4*>(b->roots_b.p);
+    uint32_t* ls = e->small_u32();
+    ps_launch_prepare_seg(e->x, e->y, e->z, n, off, n_sys, prm, roots, sys, e->code, morton_mode,
+                          s);
+    PsSortArrays io = {e->code, e->id, e->x, e->y, e->z, e->m, sys};
+    PsSortArrays t0 = e->sort_set(0), t1 = e->sort_set(1);
+    t0.aux = reinterpret_cast<uint32_t*>(b->aux_b.p);
+    t1.aux = t0.aux + e->max_p;
+    PS_CK(ps_radix_sort(io, t0, t1, e->pos4(), n, e->sort_tmp.p, s));
+    PS_CK(ps_tree_count(e->code, sys, n, ls, ls + 12, e->totals(), e->tree_tmp.p, s));
+    PS_CK(ps_tree_write(e->code, sys, n, make_float4(0.f, 0.f, 0.f, 0.f), roots, e->tree_tmp.p,
+                        e->geom(), e->link(), e->nend(), e->parent(), e->leaf_of(), cap,
+                        e->totals(), s));
+    ps_fill_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 0u, e->totals(), 4, cap, s);
+    ps_fill_u32_count(e->p2p_len(), 0u, e->totals(), 1, cap, s);
+    ps_launch_upward_dev(ls, e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), s);
+
+    float theta2;
+    {
+        volatile float tv = e->theta;  // fl(theta * theta), H:1064
+        theta2 = tv * tv;
+    }
+    const bool mt = e->trav == PS_HIP_TRAV_MT;
+    PsTravState* st = reinterpret_cast<PsTravState*>(e->tstate_b.p);
+    const uint32_t arena_cap = (uint32_t)std::min<size_t>(e->arena_b.bytes / 4, 0xFFFFFFFFull);
+    const uint32_t mask_cap = (uint32_t)std::min<size_t>(e->masks_b.bytes / 8, 0xFFFFFFFFull);
+    const uint32_t ub = (uint32_t)std::min<size_t>(cap, 0xFFFFFFFFull);
+    ps_launch_trav_init(ls, mt ? 1 : 0, n_sys, sys, e->nend(), e->link(), e->entries(0), st,
+                        e->arena(), arena_cap, mask_cap, ub, s);
+    PsTravTree t;
+    t.n_nodes = ub;
+    t.geom = e->geom();
+    t.mp = e->mp();
+    t.link = e->link();
+    t.nend = e->nend();
+    t.loc = e->loc();
+    t.p2p_off = e->p2p_off();
+    t.p2p_len = e->p2p_len();
+    t.p2p_moff = e->p2p_moff();
+    unsigned long long* far_part = reinterpret_cast<unsigned long long*>(e->far_b.p);
+    size_t far_n = 0;
+    for (int k = 0; k < PS_MAX_DEPTH_ITERS; ++k) {
+        const int cur = k & 1, nxt = cur ^ 1;
+        ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
+                            e->entries(nxt), far_part + far_n, e->kahan, e->trav_lanes, s);
+        far_n += ps_trav_blocks(ub, e->trav_lanes);
+    }
+    ps_reduce_u64(far_part, far_n, &st->m2l_pairs, s);
+    ps_launch_l2l_dev(ls, e->parent(), e->loc(), s);
+    const uint32_t bound10 = (uint32_t)std::min<size_t>(std::min(cap, n), 0xFFFFFFFFull);
+    ps_launch_p2p(n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
+                  e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
+                  e->fz, 0, e->kahan, ls + 10, bound10, e->p2p_big,
+                  reinterpret_cast<uint32_t*>(e->big_b.p), nullptr, nullptr, s, e->s2, e->fork,
+                  e->join);
+    return map_err(hipGetLastError());
+}
+
 struct PrepWS {
     int dev = -1;
     hipStream_t s = nullptr;
-    hipStream_t s2 = nullptr;  // second stream: the big-leaf P2P kernel beside the other one
-    hipEvent_t fork = nullptr, join = nullptr;
     DevBuf part, prm, x, y, z, code;
     float* h_out = nullptr;
 };
@@ -558,6 +698,165 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
             return PS_HIP_EINVAL;
     }
 }
+
+int ps_hip_batch_create(ps_hip_batch_t** out, size_t max_particles, size_t max_systems,
+                        float theta, int traversal) {
+    if (!out || max_particles == 0 || max_systems == 0 || max_systems > 1024)
+        return PS_HIP_EINVAL;
+    *out = nullptr;
+    ps_hip_batch* b = new (std::nothrow) ps_hip_batch();
+    if (!b) return PS_HIP_EOOM;
+    std::memset(&b->st, 0, sizeof(b->st));
+    int rc = ps_hip_engine_create(&b->e, max_particles, 1, theta, traversal);
+    if (rc != PS_HIP_OK) {
+        delete b;
+        return rc;
+    }
+    b->max_sys = max_systems;
+    hipStream_t s = b->e->s;
+    hipError_t r = grow(b->off_b, 4 * (max_systems + 1), s);
+    if (r == hipSuccess) r = grow(b->prm_b, 16 * max_systems, s);
+    if (r == hipSuccess) r = grow(b->roots_b, 16 * max_systems, s);
+    if (r == hipSuccess) r = hipEventCreate(&b->ev0);
+    if (r == hipSuccess) r = hipEventCreate(&b->ev1);
+    if (r != hipSuccess) {
+        (void)ps_hip_batch_destroy(b);
+        return map_err(r);
+    }
+    *out = b;
+    return PS_HIP_OK;
+}
+
+int ps_hip_batch_destroy(ps_hip_batch_t* b) {
+    if (!b) return PS_HIP_EINVAL;
+    if (b->e && b->e->s) (void)hipStreamSynchronize(b->e->s);
+    b->drop_graph();
+    DevBuf* bufs[] = {&b->off_b, &b->sys_b, &b->aux_b, &b->prm_b, &b->roots_b};
+    for (DevBuf* d : bufs) release(*d);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->e) (void)ps_hip_engine_destroy(b->e);
+    delete b;
+    return PS_HIP_OK;
+}
+
+int ps_hip_batch_set_option(ps_hip_batch_t* b, int key, double value) {
+    if (!b) return PS_HIP_EINVAL;
+    if (key == PS_HIP_BATCH_OPT_GRAPH) {
+        if (value != 0.0 && value != 1.0) return PS_HIP_EINVAL;
+        b->use_graph = value != 0.0;
+        b->drop_graph();
+        return PS_HIP_OK;
+    }
+    if (key == PS_HIP_OPT_STATS) return PS_HIP_EINVAL;  // a batch times whole ticks only
+    b->drop_graph();  // the recorded launches carry the old settings
+    return ps_hip_engine_set_option(b->e, key, value);
+}
+
+int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z, float* mass, float* fx,
+                      float* fy, float* fz, uint32_t* id, uint32_t* codes,
+                      const uint32_t* offsets, size_t n_systems, int morton_mode) {
+    if (!b || !offsets || n_systems == 0 || n_systems > b->max_sys) return PS_HIP_EINVAL;
+    if (morton_mode != PS_HIP_MORTON_TRUNC && morton_mode != PS_HIP_MORTON_AVX2)
+        return PS_HIP_EINVAL;
+    if (offsets[0] != 0) return PS_HIP_EINVAL;
+    for (size_t k = 0; k < n_systems; ++k)
+        if (offsets[k + 1] <= offsets[k]) return PS_HIP_EINVAL;  // every system non-empty
+    const size_t n = offsets[n_systems];
+    ps_hip_engine* e = b->e;
+    if (n > e->max_p) return PS_HIP_EINVAL;
+    void* arrs[9] = {x, y, z, mass, fx, fy, fz, id, codes};
+    for (void* p : arrs)
+        if (!is_device_ptr(p)) return PS_HIP_EINVAL;  // a batch works on device arrays
+    std::vector<uint32_t> off(offsets, offsets + n_systems + 1);
+    e->stats_on = 0;
+    e->count_on = 0;
+    e->n = n;
+    e->x = x;
+    e->y = y;
+    e->z = z;
+    e->m = mass;
+    e->fx = fx;
+    e->fy = fy;
+    e->fz = fz;
+    e->id = id;
+    e->code = codes;
+    e->bound = false;  // the single-system debug stages do not apply to a forest
+    hipStream_t s = e->s;
+    const uint32_t nsys = (uint32_t)n_systems;
+    const bool same = b->exec && b->g_mode == morton_mode && b->g_n == n && b->g_off == off &&
+                      std::equal(std::begin(arrs), std::end(arrs), std::begin(b->g_ptr));
+    PS_CK(hipEventRecord(b->ev0, s));
+    if (same) {
+        PS_CK(hipGraphLaunch(b->exec, s));
+    } else {
+        b->drop_graph();
+        b->node_cap = forest_node_bound(off);
+        PS_RC(batch_ensure(b, n, b->node_cap));
+        PS_CK(hipMemcpyAsync(b->off_b.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, s));
+        PS_RC(batch_run(b, nsys, morton_mode));
+    }
+    PS_CK(hipEventRecord(b->ev1, s));
+    PsTravState hst;
+    unsigned long long nodes = 0;
+    for (int attempt = 0;; ++attempt) {
+        PS_CK(hipMemcpyAsync(&hst, e->tstate_b.p, sizeof(hst), hipMemcpyDeviceToHost, s));
+        PS_CK(hipMemcpyAsync(&nodes, e->totals(), 8, hipMemcpyDeviceToHost, s));
+        PS_CK(hipStreamSynchronize(s));
+        if (!hst.overflow) break;
+        // an arena was too small: grow past the need and run the tick again
+        // directly (the inputs are unchanged: the sort is idempotent)
+        b->drop_graph();
+        if (attempt >= 4 || hst.arena_ctr > 0xFFFFFFFFull || hst.mask_ctr > 0xFFFFFFFFull)
+            return PS_HIP_EOOM;
+        PS_CK(grow(e->arena_b, std::max<size_t>(2 * e->arena_b.bytes, 5 * hst.arena_ctr), s));
+        PS_CK(grow(e->masks_b, std::max<size_t>(2 * e->masks_b.bytes, 10 * hst.mask_ctr), s));
+        PS_CK(hipEventRecord(b->ev0, s));
+        PS_RC(batch_run(b, nsys, morton_mode));
+        PS_CK(hipEventRecord(b->ev1, s));
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    b->st.n_particles = n;
+    b->st.n_systems = n_systems;
+    b->st.n_nodes = nodes;
+    b->st.m2l_pairs = hst.m2l_pairs;
+    b->st.used_graph = (same && b->exec) ? 1u : 0u;
+    b->st.ms_tick = ms;
+    if (b->st.used_graph) ++b->st.graph_replays;
+    if (b->use_graph && !b->exec) {
+        // record the tick for the next calls with this layout (nothing runs)
+        hipGraph_t g = nullptr;
+        PS_CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_run(b, nsys, morton_mode);
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc != PS_HIP_OK || ec != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return rc != PS_HIP_OK ? rc : map_err(ec != hipSuccess ? ec : hipErrorUnknown);
+        }
+        hipGraphExec_t x_ = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x_, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return map_err(ei);
+        }
+        b->graph = g;
+        b->exec = x_;
+        b->g_off = off;
+        b->g_n = n;
+        b->g_mode = morton_mode;
+        std::copy(std::begin(arrs), std::end(arrs), std::begin(b->g_ptr));
+    }
+    return PS_HIP_OK;
+}
+
+int ps_hip_batch_get_stats(const ps_hip_batch_t* b, ps_hip_batch_stats_t* out) {
+    if (!b || !out) return PS_HIP_EINVAL;
+    *out = b->st;
+    return PS_HIP_OK;
+}
+
 
 int ps_hip_engine_get_stats(const ps_hip_engine_t* e, ps_hip_stats_t* out) {
     if (!e || !out) return PS_HIP_EINVAL;

@@ -26,6 +26,9 @@ hipError_t ps_scan_dev_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_
                            size_t n_max, unsigned long long* total64, void* tmp, hipStream_t s);
 // graph-safe fills (no memset nodes)
 void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// p[0 .. per * min(*count, cap)) = v with the count in device memory
+void ps_fill_u32_count(uint32_t* p, uint32_t v, const unsigned long long* count, uint32_t per,
+                       size_t cap, hipStream_t s);
 // *out = sum of in[0..n) (cleared by a kernel, then one atomic add per block)
 void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* out,
                    hipStream_t s);
@@ -75,10 +78,12 @@ hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
                          void* tmp, hipStream_t s);
 // Phase 2: writes every node (needs phase 1's tmp untouched).  The root cell
 // is `root`, or roots[sys[i]] per system when roots is set.
+// n_nodes_dev (optional): the node count in device memory (then n_nodes is
+// only the capacity).
 hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, float4 root,
                          const float4* roots, void* tmp, float4* geom, uint2* link,
                          uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, size_t n_nodes,
-                         hipStream_t s);
+                         const unsigned long long* n_nodes_dev, hipStream_t s);
 // Upward (depths 10..0) and L2L (depths 1..10) with the level starts read
 // from device memory ls[0..11]; fixed grids, no host-side sizes.
 void ps_launch_upward_dev(const uint32_t* ls, const float4* pos4, const float4* geom,

@@ -215,6 +215,16 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+// p[0 .. per * min(*count, cap)) = v, the count read on the device
+__global__ void fill_u32_count_kernel(uint32_t* p, uint32_t v, const unsigned long long* count,
+                                      uint32_t per, size_t cap) {
+    const unsigned long long c = *count;
+    const size_t n = (size_t)per * (size_t)(c < cap ? c : cap);
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
 // Sum of n u64 values into *out: each block sums its slice (kRedV
 // independent loads per thread per round) and adds it to *out, which the
 // launcher clears first.
@@ -352,6 +362,14 @@ void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     fill_u32_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, v, n);
+}
+
+void ps_fill_u32_count(uint32_t* p, uint32_t v, const unsigned long long* count, uint32_t per,
+                       size_t cap, hipStream_t s) {
+    if (cap == 0 || per == 0) return;
+    size_t blocks = (per * cap + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    fill_u32_count_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, v, count, per, cap);
 }
 
 void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* out,

@@ -319,10 +319,13 @@ hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
 hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, float4 root,
                          const float4* roots, void* tmp, float4* geom, uint2* link,
                          uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, size_t n_nodes,
-                         hipStream_t s) {
+                         const unsigned long long* n_nodes_dev, hipStream_t s) {
     const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
     const uint32_t* off = reinterpret_cast<const uint32_t*>(tmp);
-    ps_fill_u32(reinterpret_cast<uint32_t*>(link), 0u, 2 * n_nodes, s);
+    if (n_nodes_dev)
+        ps_fill_u32_count(reinterpret_cast<uint32_t*>(link), 0u, n_nodes_dev, 2, n_nodes, s);
+    else
+        ps_fill_u32(reinterpret_cast<uint32_t*>(link), 0u, 2 * n_nodes, s);
     tree_write_kernel<<<nb, kBlock, 0, s>>>(codes, sys, n, off, nb, root, roots, geom, link,
                                             nend, parent, leaf_of);
     return hipGetLastError();

@@ -29,7 +29,8 @@ RESULT_NAMES = {0: "PS_OK", -1: "PS_EOOM", -2: "PS_EINVAL", -3: "PS_ETHRD", -4: 
 
 TRAV_MT, TRAV_ST = 0, 1
 MORTON_TRUNC, MORTON_AVX2 = 0, 1
-OPT_TRAVERSAL, OPT_KAHAN, OPT_THETA, OPT_STATS, OPT_TRAV_LANES, OPT_P2P_BIG = 1, 2, 3, 4, 5, 6
+OPT_TRAVERSAL, OPT_KAHAN, OPT_THETA, OPT_STATS, OPT_TRAV_LANES, OPT_P2P_BIG, OPT_LEAF_CAP = \
+    1, 2, 3, 4, 5, 6, 7
 BATCH_OPT_GRAPH = 100
 
 

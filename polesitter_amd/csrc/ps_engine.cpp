@@ -141,7 +141,8 @@ struct ps_hip_engine {
     int stats_on = 0;   // stage events
     int count_on = 0;   // pair counts (extra reductions)
     int trav_lanes = 8;  // lanes per traversal entry
-    uint32_t p2p_big = 24;  // depth-10 leaves of >= this many particles: wavefront per leaf
+    uint32_t p2p_big = 24;  // leaves of >= this many particles: wavefront per leaf
+    int leaf_cap = 1;       // most particles of a leaf above depth 10 (the reference: 1)
 
     // host-pointer staging, one buffer per array (allocated on first use)
     DevBuf stage[9];
@@ -301,7 +302,7 @@ int stage_sort(ps_hip_engine* e) {
 int stage_build(ps_hip_engine* e) {
     uint32_t* small = e->small_u32();
     PS_CK(ps_tree_count(e->code, nullptr, e->n, small, small + 12, e->totals(), e->tree_tmp.p,
-                        e->s));
+                        e->leaf_cap, e->s));
     PS_CK(hipMemcpyAsync(e->h_small, e->small_b.p, 96, hipMemcpyDeviceToHost, e->s));
     PS_CK(hipStreamSynchronize(e->s));
     const uint32_t flags = e->h_small[12];
@@ -316,7 +317,7 @@ int stage_build(ps_hip_engine* e) {
         if (e->ls[d + 1] > e->ls[d]) e->max_depth = (uint32_t)d;
     PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
                         e->link(), e->nend(), e->parent(), e->leaf_of(), e->n_nodes, nullptr,
-                        e->s));
+                        e->leaf_cap, e->s));
     ps_fill_u32(reinterpret_cast<uint32_t*>(e->loc()), 0u, 4 * (size_t)e->n_nodes, e->s);
     ps_fill_u32(e->p2p_len(), 0u, e->n_nodes, e->s);
     e->built = true;
@@ -425,18 +426,18 @@ int stage_l2l(ps_hip_engine* e) {
 
 int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
     unsigned long long* pc = e->count_on ? e->pairs() : nullptr;
-    // the depth-10 level holds every leaf of more than one particle (its range
-    // is read on the device: ls[10], ls[11])
-    const uint32_t bound10 = e->ls[11] - e->ls[10];
+    // candidates for the wavefront-per-leaf kernel: every node (leaves of
+    // more than one particle sit at depth 10 unless leaf_cap > 1)
+    const uint32_t bound = e->n_nodes;
     if (e->count_on)
-        PS_CK(grow(e->block_pairs_b, 16 * (size_t)ps_p2p_blocks(e->n, bound10), e->s));
+        PS_CK(grow(e->block_pairs_b, 16 * (size_t)ps_p2p_blocks(e->n, bound), e->s));
     if (!e->arena_b.p) PS_CK(grow(e->arena_b, 256, e->s));
     if (!e->masks_b.p) PS_CK(grow(e->masks_b, 256, e->s));
-    PS_CK(grow(e->big_b, 4 * ((size_t)bound10 + 1), e->s));
+    PS_CK(grow(e->big_b, 4 * ((size_t)bound + 1), e->s));
     if (ev_before >= 0) e->record(ev_before);
     ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
                   e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
-                  e->fz, mode, e->kahan, e->small_u32() + 10, bound10, e->p2p_big,
+                  e->fz, mode, e->kahan, e->small_u32(), bound, e->p2p_big,
                   reinterpret_cast<uint32_t*>(e->big_b.p),
                   reinterpret_cast<unsigned long long*>(e->block_pairs_b.p), pc, e->s, e->s2,
                   e->fork, e->join);
@@ -517,7 +518,7 @@ int batch_ensure(ps_hip_batch* b, size_t n, size_t cap) {
     PS_CK(grow(e->tstate_b, sizeof(PsTravState), s));
     PS_CK(grow(e->arena_b, 4 * std::max<size_t>(96 * n + 4096, 1 << 16), s));
     PS_CK(grow(e->masks_b, 8 * (4 * cap + 4096), s));
-    PS_CK(grow(e->big_b, 4 * (std::min(cap, n) + 1), s));
+    PS_CK(grow(e->big_b, 4 * (cap + 1), s));
     PS_CK(grow(b->sys_b, 4 * std::max<size_t>(n, 1), s));
     PS_CK(grow(b->aux_b, 8 * std::max<size_t>(e->max_p, 1), s));
     e->node_cap = std::max(e->node_cap, cap);
@@ -544,10 +545,11 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
     t0.aux = reinterpret_cast<uint32_t*>(b->aux_b.p);
     t1.aux = t0.aux + e->max_p;
     PS_CK(ps_radix_sort(io, t0, t1, e->pos4(), n, e->sort_tmp.p, s));
-    PS_CK(ps_tree_count(e->code, sys, n, ls, ls + 12, e->totals(), e->tree_tmp.p, s));
+    PS_CK(ps_tree_count(e->code, sys, n, ls, ls + 12, e->totals(), e->tree_tmp.p, e->leaf_cap,
+                        s));
     PS_CK(ps_tree_write(e->code, sys, n, make_float4(0.f, 0.f, 0.f, 0.f), roots, e->tree_tmp.p,
                         e->geom(), e->link(), e->nend(), e->parent(), e->leaf_of(), cap,
-                        e->totals(), s));
+                        e->totals(), e->leaf_cap, s));
     ps_fill_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 0u, e->totals(), 4, cap, s);
     ps_fill_u32_count(e->p2p_len(), 0u, e->totals(), 1, cap, s);
     ps_launch_upward_dev(ls, e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), s);
@@ -584,10 +586,9 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
     }
     ps_reduce_u64(far_part, far_n, &st->m2l_pairs, s);
     ps_launch_l2l_dev(ls, e->parent(), e->loc(), s);
-    const uint32_t bound10 = (uint32_t)std::min<size_t>(std::min(cap, n), 0xFFFFFFFFull);
     ps_launch_p2p(n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
                   e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
-                  e->fz, 0, e->kahan, ls + 10, bound10, e->p2p_big,
+                  e->fz, 0, e->kahan, ls, ub, e->p2p_big,
                   reinterpret_cast<uint32_t*>(e->big_b.p), nullptr, nullptr, s, e->s2, e->fork,
                   e->join);
     return map_err(hipGetLastError());
@@ -689,6 +690,10 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
         case PS_HIP_OPT_TRAV_LANES:
             if (value != 8 && value != 16 && value != 32 && value != 64) return PS_HIP_EINVAL;
             e->trav_lanes = (int)value;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_LEAF_CAP:
+            if (!(value >= 1.0 && value <= 64.0) || value != (double)(int)value) return PS_HIP_EINVAL;
+            e->leaf_cap = (int)value;
             return PS_HIP_OK;
         case PS_HIP_OPT_P2P_BIG:
             if (!(value >= 0.0 && value <= 4294967295.0)) return PS_HIP_EINVAL;

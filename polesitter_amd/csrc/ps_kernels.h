@@ -73,9 +73,11 @@ size_t ps_tree_tmp_bytes(size_t n);
 // flags[0] |= 1 if any key has bits 30/31 set.  total64[0] = node count.
 // sys (optional): the system of each particle (sorted by (system, code));
 // each system then gets its own root, in system order at depth 0.
+// leaf_cap: a node above depth 10 is a leaf iff it holds <= leaf_cap
+// particles (the reference's rule is leaf_cap = 1).
 hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
                          uint32_t* level_start, uint32_t* flags, unsigned long long* total64,
-                         void* tmp, hipStream_t s);
+                         void* tmp, int leaf_cap, hipStream_t s);
 // Phase 2: writes every node (needs phase 1's tmp untouched).  The root cell
 // is `root`, or roots[sys[i]] per system when roots is set.
 // n_nodes_dev (optional): the node count in device memory (then n_nodes is
@@ -83,7 +85,7 @@ hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
 hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, float4 root,
                          const float4* roots, void* tmp, float4* geom, uint2* link,
                          uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, size_t n_nodes,
-                         const unsigned long long* n_nodes_dev, hipStream_t s);
+                         const unsigned long long* n_nodes_dev, int leaf_cap, hipStream_t s);
 // Upward (depths 10..0) and L2L (depths 1..10) with the level starts read
 // from device memory ls[0..11]; fixed grids, no host-side sizes.
 void ps_launch_upward_dev(const uint32_t* ls, const float4* pos4, const float4* geom,
@@ -152,22 +154,22 @@ void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------
 // mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);
 // 2: f += m*sum (P2P only).  A leaf's near sources are the non-far entries of
-// its final traversal list.  Leaves in [lrange[0], lrange[1]) (device memory:
-// the depth-10 level) with >= tmin particles go to a wavefront-per-leaf kernel,
-// the rest to a thread-per-particle kernel (tmin = 0: all of them).  bound10
-// bounds the depth-10 node count (grid sizes only); big: scratch of
-// 1 + bound10 words for the list of the big leaves.  With s_big, fork and
-// join set, the big-leaf kernel runs on s_big beside the other; all work is
-// ordered after earlier work on s and before later work on s.
+// its final traversal list.  Leaves of >= tmin particles go to a
+// wavefront-per-leaf kernel, the rest to a thread-per-particle kernel (tmin =
+// 0: all of them); ls = level starts in device memory (ls[11] = node count),
+// bound >= the node count (grid sizes only), big: scratch of 1 + bound
+// words for the list of the big leaves.  With s_big, fork and join set, the
+// big-leaf kernel runs on s_big beside the other; all work is ordered after
+// earlier work on s and before later work on s.
 // pair_count (stats, may be null): per-block partial sums in block_pairs
-// (>= 2 * ps_p2p_blocks(n, bound10) entries), reduced into pair_count[0]
+// (>= 2 * ps_p2p_blocks(n, bound) entries), reduced into pair_count[0]
 // (particle pairs) and pair_count[1] (target leaf x source leaf pairs).
-uint32_t ps_p2p_blocks(size_t n, uint32_t bound10);
+uint32_t ps_p2p_blocks(size_t n, uint32_t bound);
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                    const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
-                   const uint32_t* lrange, uint32_t bound10, uint32_t tmin, uint32_t* big,
+                   const uint32_t* ls, uint32_t bound, uint32_t tmin, uint32_t* big,
                    unsigned long long* block_pairs, unsigned long long* pair_count,
                    hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join);
 

@@ -1,8 +1,8 @@
 // ps_p2p.hip -- near field (P2P) fused with L2P (replaces the leaf x leaf
 // loop of H:1279-1452 and L2P of H:1158-1235).
 //
-// Two kernels split the target leaves.  A depth-10 leaf of >= tmin particles
-// gets a wavefront: its 64-target rounds share the leaf's source list, so the
+// Two kernels split the target leaves.  A leaf of >= tmin particles gets a
+// wavefront: its 64-target rounds share the leaf's source list, so the
 // loop runs on scalar control flow and every source particle is a scalar load
 // broadcast to the wave.  Every other particle gets a thread; particles are in
 // sorted order, so the lanes of a leaf walk the same list in step and their
@@ -60,11 +60,11 @@ __device__ __forceinline__ void p2p_uniform_range(float4 p, const float4* __rest
     }
 }
 
-// the big-leaf kernel's targets: depth-10 leaves [lbeg, lend) of >= tmin particles
-__device__ __forceinline__ bool p2p_is_big(uint32_t lf, uint32_t lbeg, uint32_t lend,
-                                           uint32_t tmin, const uint2* __restrict__ link,
+// the big-leaf kernel's targets: leaves of >= tmin particles (tmin > 0)
+__device__ __forceinline__ bool p2p_is_big(uint32_t lf, uint32_t tmin,
+                                           const uint2* __restrict__ link,
                                            const uint32_t* __restrict__ nend) {
-    return lf >= lbeg && lf < lend && nend[lf] - link[lf].x >= tmin;
+    return tmin && nend[lf] - link[lf].x >= tmin;
 }
 
 // accumulate target p against the near sources of one final list (wave-uniform
@@ -153,10 +153,9 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
     const uint32_t* __restrict__ p2p_len, const uint32_t* __restrict__ p2p_moff,
     const uint32_t* __restrict__ arena, const unsigned long long* __restrict__ masks,
     const uint2* __restrict__ link, const uint32_t* __restrict__ nend, float* __restrict__ fx,
-    float* __restrict__ fy, float* __restrict__ fz, const uint32_t* __restrict__ lrange,
-    uint32_t tmin, unsigned long long* block_pairs, uint32_t stride) {
+    float* __restrict__ fy, float* __restrict__ fz, uint32_t tmin,
+    unsigned long long* block_pairs, uint32_t stride) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t lbeg = lrange[0], lend = tmin ? lrange[1] : lrange[0];
     float ax = 0.f, ay = 0.f, az = 0.f;
     float cx = 0.f, cy = 0.f, cz = 0.f;
     uint32_t pairs = 0, nsrc = 0;  // particle pairs; near source leaves
@@ -164,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
     bool valid = i < n;
     if (valid) {
         lf = leaf_of[i];
-        valid = !p2p_is_big(lf, lbeg, lend, tmin, link, nend);
+        valid = !p2p_is_big(lf, tmin, link, nend);
     }
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
     if (valid) {
@@ -219,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
 // The order of the slices does not matter: every leaf is computed alone.
 constexpr int kCollect = 1024;
 
-__global__ __launch_bounds__(kCollect) void p2p_collect_kernel(const uint32_t* __restrict__ lrange,
+__global__ __launch_bounds__(kCollect) void p2p_collect_kernel(const uint32_t* __restrict__ ls,
                                                                uint32_t tmin,
                                                                const uint2* __restrict__ link,
                                                                const uint32_t* __restrict__ nend,
@@ -227,10 +226,11 @@ __global__ __launch_bounds__(kCollect) void p2p_collect_kernel(const uint32_t* _
     __shared__ uint32_t wcnt[kCollect / 64];
     __shared__ uint32_t base;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t lbeg = lrange[0], lend = lrange[1];
-    for (uint32_t n0 = lbeg + blockIdx.x * kCollect; n0 < lend; n0 += gridDim.x * kCollect) {
+    const uint32_t lend = ls[PS_MAX_DEPTH + 1];  // node count
+    for (uint32_t n0 = blockIdx.x * kCollect; n0 < lend; n0 += gridDim.x * kCollect) {
         const uint32_t node = n0 + threadIdx.x;
-        const bool take = node < lend && nend[node] - link[node].x >= tmin;
+        const bool take = node < lend && (link[node].y & PS_LEAF) &&
+                          nend[node] - link[node].x >= tmin;
         const unsigned long long b = __ballot(take);
         if (lane == 0) wcnt[wave] = (uint32_t)__popcll(b);
         __syncthreads();
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kCollect) void p2p_collect_kernel(const uint32_t* _
     }
 }
 
-// One wavefront per big leaf (depth-10, >= tmin particles), 64 targets per
+// One wavefront per big leaf (>= tmin particles), 64 targets per
 // round: the whole wave shares the leaf's source list, so the loop runs on
 // scalar control flow and each source particle is one scalar load broadcast
 // to the 64 lanes.  Wave w takes list entries w, w + waves, ...; the grid is
@@ -312,11 +312,11 @@ template <int kMode, bool kKahan>
 void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                  const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                  const uint32_t* arena, const unsigned long long* masks, const uint2* link,
-                 const uint32_t* nend, float* fx, float* fy, float* fz, const uint32_t* lrange,
-                 uint32_t bound10, uint32_t tmin, const uint32_t* big, unsigned long long* bp,
+                 const uint32_t* nend, float* fx, float* fy, float* fz, uint32_t bound,
+                 uint32_t tmin, const uint32_t* big, unsigned long long* bp,
                  hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join) {
     const uint32_t nb1 = (uint32_t)((n + kBlock - 1) / kBlock);
-    const uint32_t nb2 = tmin ? big_blocks(bound10) : 0u;
+    const uint32_t nb2 = tmin ? big_blocks(bound) : 0u;
     // the two kernels write disjoint particles: the big-leaf one runs on its
     // own stream beside the other (fork / join through events)
     const bool two = nb2 && s_big && fork && join;
@@ -331,7 +331,7 @@ void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const fl
     }
     p2p_kernel<kMode, kKahan><<<nb1, kBlock, 0, s>>>(n, pos4, leaf_of, loc, p2p_off, p2p_len,
                                                      p2p_moff, arena, masks, link, nend, fx, fy,
-                                                     fz, lrange, nb2 ? tmin : 0u, bp, nb1 + nb2);
+                                                     fz, nb2 ? tmin : 0u, bp, nb1 + nb2);
     if (two) {
         (void)hipEventRecord(join, s_big);
         (void)hipStreamWaitEvent(s, join, 0);
@@ -340,29 +340,29 @@ void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const fl
 
 }  // namespace
 
-uint32_t ps_p2p_blocks(size_t n, uint32_t bound10) {
-    return (uint32_t)((n + kBlock - 1) / kBlock) + big_blocks(bound10);
+uint32_t ps_p2p_blocks(size_t n, uint32_t bound) {
+    return (uint32_t)((n + kBlock - 1) / kBlock) + big_blocks(bound);
 }
 
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                    const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
-                   const uint32_t* lrange, uint32_t bound10, uint32_t tmin, uint32_t* big,
+                   const uint32_t* ls, uint32_t bound, uint32_t tmin, uint32_t* big,
                    unsigned long long* block_pairs, unsigned long long* pair_count,
                    hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join) {
     if (n == 0) return;
-    if (!big || bound10 == 0) tmin = 0;  // no big-leaf kernel
+    if (!big || bound == 0) tmin = 0;  // no big-leaf kernel
     unsigned long long* bp = pair_count ? block_pairs : nullptr;
     if (tmin) {
         ps_fill_u32(big, 0u, 1, s);
-        uint32_t cb = (bound10 + kCollect - 1) / kCollect;
+        uint32_t cb = (bound + kCollect - 1) / kCollect;
         p2p_collect_kernel<<<cb < kCollectBlocks ? cb : kCollectBlocks, kCollect, 0, s>>>(
-            lrange, tmin, link, nend, big);
+            ls, tmin, link, nend, big);
     }
 #define PS_P2P(M, K)                                                                           \
     launch_mode<M, K>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link, \
-                      nend, fx, fy, fz, lrange, bound10, tmin, big, bp, s, s_big, fork, join)
+                      nend, fx, fy, fz, bound, tmin, big, bp, s, s_big, fork, join)
     switch (mode * 2 + (kahan ? 1 : 0)) {
         case 0: PS_P2P(0, false); break;
         case 1: PS_P2P(0, true); break;
@@ -373,7 +373,7 @@ void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const 
     }
 #undef PS_P2P
     if (bp) {
-        const uint32_t nb = ps_p2p_blocks(n, tmin ? bound10 : 0u);
+        const uint32_t nb = ps_p2p_blocks(n, tmin ? bound : 0u);
         ps_reduce_u64(bp, nb, pair_count, s);
         ps_reduce_u64(bp + nb, nb, pair_count + 1, s);
     }

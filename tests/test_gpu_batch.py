@@ -58,6 +58,21 @@ def test_batch_equals_single_ticks(mode, trav):
             assert np.array_equal(g[f], o[f]), (k, f)
 
 
+def test_batch_leaf_cap_equals_single_ticks():
+    systems = ragged_systems(seed=65)
+    n = sum(len(s[0]) for s in systems)
+    with api.Batch(n, len(systems), theta=2.0, graph=True) as b:
+        b.set_option(api.OPT_LEAF_CAP, 8)
+        b.load(systems)
+        b.tick()
+        res = b.result()
+    for k, (x, y, z, m) in enumerate(systems):
+        o = gpu_tick(x, y, z, m, theta=2.0, tree=False, leaf_cap=8)
+        g = b.system(res, k)
+        for f in FIELDS:
+            assert np.array_equal(g[f], o[f]), (k, f)
+
+
 def test_batch_pair_counts_match_oracle(oracle):
     systems = ragged_systems(seed=62)
     n = sum(len(s[0]) for s in systems)

@@ -8,7 +8,8 @@ the reference's threaded-build traversal.  One step = one tick
 by the caller's frame step on the device (id-gather of velocities,
 semi-implicit Euler dt = 1e-3), so every tick re-sorts and rebuilds.
 
-    python bench.py [--steps K] [--warmup W] [--workload plummer1m|twoplummer16m|plummer4m]
+    python bench.py [--steps K] [--warmup W]
+                    [--workload plummer1m|twoplummer16m|plummer4m|ensemble512|mlp4m]
 
 Prints one JSON line.  `value` = N * K / (wall time of K steps), bracketed
 by device synchronisation.  `roofline` is the dominant kernel's algorithmic
@@ -133,6 +134,150 @@ def cpu_baseline(n, x, y, z, m, theta, ticks, gpu_ref_fn):
                 tick_s=tick_s)
 
 
+def _timed_steps(s, n, v, steps, warmup, mmode):
+    """wall seconds per step (tick + id-gather + Euler) on a loaded Solver"""
+    from polesitter_amd import api
+    L = api.hip_lib()
+    vel = [api.DeviceArray.from_numpy(vv) for vv in v]
+    tmp = api.DeviceArray(3 * n, np.float32)
+
+    def step():
+        s.reset_ids()
+        s.tick(mmode)
+        api.check("gather", L.ps_hip_step_gather(s.ctx.engine, s.id.ptr, vel[0].ptr, vel[1].ptr,
+                                                 vel[2].ptr, tmp.ptr, n))
+        api.check("euler", L.ps_hip_step_euler(s.ctx.engine, s.x.ptr, s.y.ptr, s.z.ptr,
+                                               vel[0].ptr, vel[1].ptr, vel[2].ptr, s.fx.ptr,
+                                               s.fy.ptr, s.fz.ptr, s.m.ptr, 1e-3, n))
+    for _ in range(warmup):
+        step()
+    api.check("sync", L.ps_hip_synchronize())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    api.check("sync", L.ps_hip_synchronize())
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_mlp(a):
+    """The learned far-field correction at N = 4,194,304 Plummer (a = 1):
+    trained on --mlp-ticks ticks of residuals at the coarse setting (leaf
+    capacity --mlp-cap), validated on another realisation (the plain FMM
+    judges: the correction is used only if it lowers the error), then
+    compared on a third realisation with the plain FMM at the reference
+    setting (leaf capacity 1): FP64 RMS error on 4096 targets and throughput
+    per step (tick + id-gather + Euler).  value = corrected throughput."""
+    from polesitter_amd import api, mlp
+    from polesitter_amd import workloads as W
+    n = 1 << 22
+    mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
+    log(f"bench: {api.device_name()} workload mlp4m: Plummer N={n}, coarse leaf capacity "
+        f"{a.mlp_cap}")
+    zero = [np.zeros(n, np.float32)] * 3
+    x, y, z, m = W.plummer(n, seed=1)
+    val = W.plummer(n, seed=2) + (zero,)
+    t0 = time.time()
+    res = mlp.build_correction(x, y, z, m, zero, train_ticks=a.mlp_ticks, val_ticks=10,
+                               samples=4096, theta=a.theta, leaf_cap=a.mlp_cap, steps=3000,
+                               batch=8192, lr=2e-3, seed=1, val_system=val, log=log)
+    train_s = time.time() - t0
+    L = api.hip_lib()
+    ex, ey, ez, em = W.plummer(n, seed=3)
+    out = {}
+    for name, cap, use in (("plain_reference_setting", 1, False),
+                           ("corrected_coarse", a.mlp_cap, res["enabled"]),
+                           ("plain_coarse", a.mlp_cap, False)):
+        with api.Solver(n, theta=a.theta) as s:
+            api.check("opt", api.ps_set_option(s.ctx, api.OPT_LEAF_CAP, cap))
+            if use:
+                api.check("set_mlp", L.ps_hip_engine_set_mlp(s.ctx.engine,
+                                                             res["params"].ctypes.data))
+            s.load(ex, ey, ez, em)
+            s.reset_ids()
+            s.tick(mmode)
+            r = s.result()
+            f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
+            err = fp64_error(n, f, r["id"], s.direct_f64)
+            s.load(ex, ey, ez, em)
+            dt = _timed_steps(s, n, zero, a.steps, a.warmup, mmode)
+        out[name] = dict(leaf_cap=cap, correction=bool(use), fp64_rms_err=err[0],
+                         fp64_max_err=err[1], ms_per_step=round(1e3 * dt, 4),
+                         particles_per_s=round(n / dt, 1))
+        log(f"bench: {name}: {out[name]}")
+    c, p = out["corrected_coarse"], out["plain_reference_setting"]
+    line = dict(metric=METRIC, value=c["particles_per_s"], unit="particles/s", n_gpus=1,
+                steps=a.steps, warmup=a.warmup, ms_per_step=c["ms_per_step"],
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+                data="synthetic",
+                config=dict(workload="mlp4m", n_particles=n, theta=a.theta, morton=a.morton,
+                            coarse_leaf_cap=a.mlp_cap, train_ticks=a.mlp_ticks,
+                            step="tick + id-gather + semi-implicit Euler dt=1e-3",
+                            net="24-64-64-64-3 GELU, FP32"),
+                results=out, speedup_vs_plain_reference_setting=round(
+                    c["particles_per_s"] / p["particles_per_s"], 3),
+                error_ratio_vs_plain_reference_setting=round(
+                    c["fp64_rms_err"] / p["fp64_rms_err"], 4),
+                validation=dict(plain=res["val_plain"], corrected=res["val_corrected"],
+                                enabled=res["enabled"], train_samples=res["train_samples"],
+                                train_seconds=round(train_s, 1)))
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_ensemble(a):
+    """512 independent systems (N_s log-uniform in [2048, 262144], even ones
+    uniform cubes, odd ones Plummer with a ~ U[0.5, 2], masses 1/N_s) in one
+    batched tick, recorded as a HIP graph and replayed; settings from the
+    autotuner at the reference setting's FP64 error.  One step = one batched
+    tick (prepare + calc_forces of every system); value = total particles /
+    tick."""
+    from polesitter_amd import api, autotune
+    from polesitter_amd import workloads as W
+    systems = W.ensemble(512, seed=1)
+    n = sum(len(t[0]) for t in systems)
+    mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
+    log(f"bench: {api.device_name()} workload ensemble512: 512 systems, {n} particles")
+    tuned = autotune.tune_batch(systems, theta=a.theta, mode=mmode, log=log)
+    cfg = tuned["config"]
+    L = api.hip_lib()
+    with api.Batch(n, len(systems), theta=a.theta, graph=True, kahan=a.kahan) as b:
+        b.set_option(api.OPT_LEAF_CAP, cfg["leaf_cap"])
+        b.set_option(api.OPT_P2P_BIG, cfg["p2p_big"])
+        b.set_option(api.OPT_TRAV_LANES, cfg["trav_lanes"])
+        b.load(systems)
+        a.warmup = max(a.warmup, 1)
+        for _ in range(a.warmup):  # the first one records the graph
+            b.tick(mmode)
+        api.check("sync", L.ps_hip_synchronize())
+        ms_dev = []
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            b.tick(mmode)
+            ms_dev.append(b.stats()["ms_tick"])
+        api.check("sync", L.ps_hip_synchronize())
+        wall = time.perf_counter() - t0
+        st = b.stats()
+    value = n * a.steps / wall
+    line = dict(metric=METRIC, value=round(value, 1), unit="particles/s", n_gpus=1,
+                steps=a.steps, warmup=a.warmup, ms_per_step=round(1e3 * wall / a.steps, 4),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+                data="synthetic",
+                config=dict(workload="ensemble512", n_systems=len(systems), n_particles=n,
+                            theta=a.theta, morton=a.morton, kahan=bool(a.kahan),
+                            traversal="MT depth-2 jobs (PS_MULTITHREADING)",
+                            step="one batched tick (HIP graph replay) of all systems",
+                            tuned=cfg, pointers="device-resident"),
+                device_ms_per_tick=round(float(np.median(ms_dev)), 4),
+                graph_replays=int(st["graph_replays"]), used_graph=int(st["used_graph"]),
+                counts=dict(nodes=int(st["n_nodes"]), m2l_pairs=int(st["m2l_pairs"])),
+                autotune=dict(reference=tuned["reference"], chosen_ms=tuned["ms_tick"],
+                              chosen_fp64_rms_err=tuned["fp64_rms_err"],
+                              candidates=[dict(c, ms=round(ms_, 3), err=round(e_, 6))
+                                          for c, ms_, e_ in tuned["table"]]))
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +296,9 @@ def main(argv=None):
     ap.add_argument("--cpu-ticks", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--mlp-cap", type=int, default=16,
+                    help="mlp4m: leaf capacity of the corrected coarse setting")
+    ap.add_argument("--mlp-ticks", type=int, default=200, help="mlp4m: training ticks")
     ap.add_argument("--host-ticks", type=int, default=3,
                     help="ticks timed with host (pageable numpy) arrays, PCIe included; 0 = skip")
     a = ap.parse_args(argv)
@@ -161,6 +309,10 @@ def main(argv=None):
 
     from polesitter_amd import api
 
+    if a.workload == "ensemble512":
+        return bench_ensemble(a)
+    if a.workload == "mlp4m":
+        return bench_mlp(a)
     n, x, y, z, m, v, desc = workload(a.workload)
     log(f"bench: {api.device_name()} workload {a.workload}: {desc}")
     L = api.hip_lib()

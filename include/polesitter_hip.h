@@ -70,6 +70,7 @@ typedef struct {
     float ms_upload, ms_sort, ms_build, ms_upward, ms_traverse, ms_l2l, ms_p2p,
         ms_download, ms_total;
     float ms_trav_steps;
+    float ms_mlp;             /* the learned correction, when one is set */
 } ps_hip_stats_t;
 
 typedef struct ps_hip_batch ps_hip_batch_t;
@@ -190,6 +191,33 @@ int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z,
                       uint32_t* id, uint32_t* codes, const uint32_t* offsets,
                       size_t n_systems, int morton_mode);
 int ps_hip_batch_get_stats(const ps_hip_batch_t* b, ps_hip_batch_stats_t* out);
+
+/* no reference counterpart: the learned far-field correction.  A small MLP
+ * (24 -> 64 -> 64 -> 64 -> 3, GELU, FP32) maps per-particle leaf-local
+ * features (offsets from the leaf, parent and grandparent centres, each
+ * level's own M2L field, the leaf centroid) to a correction of the
+ * acceleration in units of the leaf's far field: f += m * |E_L| * net(F).
+ * params: PS_HIP_MLP_PARAMS floats, packed W1[64][24] b1[64] W2[64][64]
+ * b2[64] W3[64][64] b3[64] W4[3][64] b4[3].  All arrays are device arrays
+ * unless noted. */
+#define PS_HIP_MLP_IN     24
+#define PS_HIP_MLP_PARAMS 10115
+/* set (params, host or device) or clear (NULL) the correction the engine
+ * applies at the end of every tick */
+int ps_hip_engine_set_mlp(ps_hip_engine_t* e, const float* params);
+/* features feat[n][24] and scales scale[n] of the last tick (sorted order) */
+int ps_hip_mlp_features(ps_hip_engine_t* e, float* feat, float* scale);
+/* out[n][3] = net(feat) */
+int ps_hip_mlp_forward(ps_hip_engine_t* e, const float* params, const float* feat,
+                       size_t n, float* out3);
+/* grad = d/dparams of (1/n) sum |net(feat) - target|^2; *loss_host = that
+ * mean (host pointer, may be NULL) */
+int ps_hip_mlp_grad(ps_hip_engine_t* e, const float* params, const float* feat,
+                    const float* target, size_t n, float* grad, float* loss_host);
+/* one bias-corrected Adam step of params with moments m1, m2 (step >= 1) */
+int ps_hip_mlp_adam(ps_hip_engine_t* e, float* params, const float* grad,
+                    float* m1, float* m2, float lr, float beta1, float beta2,
+                    float eps, int step);
 
 /* no reference counterpart: device memory for callers without a HIP runtime
  * of their own (kind: 1 host->device, 2 device->host, 3 device->device) */

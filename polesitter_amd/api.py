@@ -69,7 +69,7 @@ class PsStats(C.Structure):
                 ("ms_upload", C.c_float), ("ms_sort", C.c_float), ("ms_build", C.c_float),
                 ("ms_upward", C.c_float), ("ms_traverse", C.c_float), ("ms_l2l", C.c_float),
                 ("ms_p2p", C.c_float), ("ms_download", C.c_float), ("ms_total", C.c_float),
-                ("ms_trav_steps", C.c_float)]
+                ("ms_trav_steps", C.c_float), ("ms_mlp", C.c_float)]
 
     def as_dict(self):
         d = {}
@@ -108,6 +108,11 @@ def hip_lib():
             "ps_hip_op_m2l": ([_vp, _vp, _vp, _f, _vp], C.c_int),
             "ps_hip_op_morton_exhaustive": ([_vp], C.c_int),
             "ps_hip_op_copy": ([_vp, _vp, C.c_size_t], C.c_int),
+            "ps_hip_engine_set_mlp": ([_vp, _vp], C.c_int),
+            "ps_hip_mlp_features": ([_vp, _vp, _vp], C.c_int),
+            "ps_hip_mlp_forward": ([_vp, _vp, _vp, _sz, _vp], C.c_int),
+            "ps_hip_mlp_grad": ([_vp, _vp, _vp, _vp, _sz, _vp, C.POINTER(_f)], C.c_int),
+            "ps_hip_mlp_adam": ([_vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, C.c_int], C.c_int),
             "ps_hip_batch_create": ([C.POINTER(_vp), _sz, _sz, _f, C.c_int], C.c_int),
             "ps_hip_batch_destroy": ([_vp], C.c_int),
             "ps_hip_batch_set_option": ([_vp, C.c_int, C.c_double], C.c_int),

@@ -34,7 +34,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 SOURCES = ["ps_scan.hip", "ps_prepare.hip", "ps_sort.hip", "ps_tree.hip", "ps_traverse.hip",
-           "ps_p2p.hip", "ps_direct.hip", "ps_step.hip", "ps_ops.hip", "ps_engine.cpp"]
+           "ps_p2p.hip", "ps_direct.hip", "ps_step.hip", "ps_ops.hip", "ps_mlp.hip",
+           "ps_engine.cpp"]
 
 
 def _run(cmd, cwd=ROOT):

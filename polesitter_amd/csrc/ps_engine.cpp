@@ -143,6 +143,9 @@ struct ps_hip_engine {
     int trav_lanes = 8;  // lanes per traversal entry
     uint32_t p2p_big = 24;  // leaves of >= this many particles: wavefront per leaf
     int leaf_cap = 1;       // most particles of a leaf above depth 10 (the reference: 1)
+    DevBuf mlp_b;           // learned correction parameters (empty: none)
+    bool mlp_on = false;
+    DevBuf mlp_tmp;         // training scratch
 
     // host-pointer staging, one buffer per array (allocated on first use)
     DevBuf stage[9];
@@ -230,7 +233,7 @@ struct ps_hip_engine {
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
                          &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,
-                         &small_b,   &block_pairs_b, &tstate_b, &big_b};
+                         &small_b,   &block_pairs_b, &tstate_b, &big_b,  &mlp_b, &mlp_tmp};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
             for (auto& b : es) release(b);
@@ -457,6 +460,13 @@ int run_tick(ps_hip_engine* e) {
     PS_RC(stage_l2l(e));
     PS_RC(stage_p2p(e, 0, 6));  // records event 6 right before the P2P launch
     e->record(7);
+    if (e->mlp_on) {
+        ps_launch_mlp_apply(e->n, reinterpret_cast<const float*>(e->mlp_b.p), e->pos4(),
+                            e->leaf_of(), e->geom(), e->mp(), e->loc(), e->parent(), e->fx, e->fy,
+                            e->fz, e->s);
+        PS_CK(hipGetLastError());
+    }
+    e->record(8);
     return PS_HIP_OK;
 }
 
@@ -591,6 +601,10 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
                   e->fz, 0, e->kahan, ls, ub, e->p2p_big,
                   reinterpret_cast<uint32_t*>(e->big_b.p), nullptr, nullptr, s, e->s2, e->fork,
                   e->join);
+    if (e->mlp_on)
+        ps_launch_mlp_apply(n, reinterpret_cast<const float*>(e->mlp_b.p), e->pos4(),
+                            e->leaf_of(), e->geom(), e->mp(), e->loc(), e->parent(), e->fx, e->fy,
+                            e->fz, s);
     return map_err(hipGetLastError());
 }
 
@@ -702,6 +716,64 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
         default:
             return PS_HIP_EINVAL;
     }
+}
+
+static_assert(PS_HIP_MLP_PARAMS == PS_MLP_PARAMS && PS_HIP_MLP_IN == PS_MLP_IN,
+              "the public and kernel layouts of the correction agree");
+
+int ps_hip_engine_set_mlp(ps_hip_engine_t* e, const float* params) {
+    if (!e) return PS_HIP_EINVAL;
+    if (!params) {
+        e->mlp_on = false;
+        return PS_HIP_OK;
+    }
+    PS_CK(grow(e->mlp_b, 4 * PS_MLP_PARAMS, e->s));
+    PS_CK(hipMemcpyAsync(e->mlp_b.p, params, 4 * PS_MLP_PARAMS, hipMemcpyDefault, e->s));
+    PS_CK(hipStreamSynchronize(e->s));
+    e->mlp_on = true;
+    return PS_HIP_OK;
+}
+
+int ps_hip_mlp_features(ps_hip_engine_t* e, float* feat, float* scale) {
+    if (!e || !feat || !scale || !e->built) return PS_HIP_EINVAL;
+    if (!is_device_ptr(feat) || !is_device_ptr(scale)) return PS_HIP_EINVAL;
+    ps_launch_mlp_features(e->n, e->pos4(), e->leaf_of(), e->geom(), e->mp(), e->loc(),
+                           e->parent(), feat, scale, e->s);
+    PS_CK(hipGetLastError());
+    PS_CK(hipStreamSynchronize(e->s));
+    return PS_HIP_OK;
+}
+
+int ps_hip_mlp_forward(ps_hip_engine_t* e, const float* params, const float* feat, size_t n,
+                       float* out3) {
+    if (!e || !params || !feat || !out3) return PS_HIP_EINVAL;
+    ps_launch_mlp_forward(params, feat, n, out3, e->s);
+    PS_CK(hipGetLastError());
+    PS_CK(hipStreamSynchronize(e->s));
+    return PS_HIP_OK;
+}
+
+int ps_hip_mlp_grad(ps_hip_engine_t* e, const float* params, const float* feat,
+                    const float* target, size_t n, float* grad, float* loss_host) {
+    if (!e || !params || !feat || !target || !grad || n == 0) return PS_HIP_EINVAL;
+    PS_CK(grow(e->mlp_tmp, 4 * (ps_mlp_grad_tmp_floats(n) + 1), e->s));
+    float* tmp = reinterpret_cast<float*>(e->mlp_tmp.p);
+    float* loss_dev = tmp + ps_mlp_grad_tmp_floats(n);
+    ps_launch_mlp_grad(params, feat, target, n, grad, loss_dev, tmp, e->s);
+    PS_CK(hipGetLastError());
+    float sse = 0.f;
+    PS_CK(hipMemcpyAsync(&sse, loss_dev, 4, hipMemcpyDeviceToHost, e->s));
+    PS_CK(hipStreamSynchronize(e->s));
+    if (loss_host) *loss_host = sse / (float)n;
+    return PS_HIP_OK;
+}
+
+int ps_hip_mlp_adam(ps_hip_engine_t* e, float* params, const float* grad, float* m1, float* m2,
+                    float lr, float beta1, float beta2, float eps, int step) {
+    if (!e || !params || !grad || !m1 || !m2 || step < 1) return PS_HIP_EINVAL;
+    ps_launch_mlp_adam(params, grad, m1, m2, lr, beta1, beta2, eps, step, e->s);
+    PS_CK(hipGetLastError());
+    return PS_HIP_OK;
 }
 
 int ps_hip_batch_create(ps_hip_batch_t** out, size_t max_particles, size_t max_systems,
@@ -949,7 +1021,7 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
             PS_CK(hipMemcpyAsync(host[k], dev[k], n * 4, hipMemcpyDeviceToHost, e->s));
     if (e->count_on)
         PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 16, hipMemcpyDeviceToHost, e->s));
-    PS_CK(hipEventRecord(e->ev[7], e->s));
+    if (e->stats_on) PS_CK(hipEventRecord(e->ev[9], e->s));
     PS_CK(hipStreamSynchronize(e->s));
     e->st.n_particles = n;
     e->st.n_nodes = e->n_nodes;
@@ -966,8 +1038,10 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
         e->st.ms_traverse = e->elapsed(4, 5);
         e->st.ms_l2l = e->elapsed(5, 6);
         e->st.ms_p2p = e->elapsed(6, 7);
+        e->st.ms_mlp = e->elapsed(7, 8);
+        e->st.ms_download = e->elapsed(8, 9);
         e->st.ms_upload = e->elapsed(0, 1);
-        e->st.ms_total = e->elapsed(0, 7);
+        e->st.ms_total = e->elapsed(0, 9);
     }
     return PS_HIP_OK;
 }

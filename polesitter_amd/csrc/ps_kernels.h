@@ -194,3 +194,28 @@ hipError_t ps_op_m2l(const float* tgeom, const float* sgeom, const float* smp, f
 hipError_t ps_op_morton_exhaustive(unsigned long long* out2, hipStream_t s);
 // plain copy of bytes (a multiple of 16) between device buffers
 hipError_t ps_op_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+
+// ---- ps_mlp.hip: learned far-field correction ------------------------------
+#define PS_MLP_IN 24
+#define PS_MLP_HIDDEN 64
+#define PS_MLP_PARAMS 10115  // W1[64][24] b1 W2[64][64] b2 W3[64][64] b3 W4[3][64] b4[3]
+// per particle (sorted order) of the last tick: features feat[n][24] and the
+// leaf's field magnitude scale[n]
+void ps_launch_mlp_features(size_t n, const float4* pos4, const uint32_t* leaf_of,
+                            const float4* geom, const float4* mp, const float4* loc,
+                            const uint32_t* parent, float* feat, float* scale, hipStream_t s);
+// out[n][3] = net(feat)
+void ps_launch_mlp_forward(const float* W, const float* feat, size_t n, float* out,
+                           hipStream_t s);
+// f += m * scale * net(features), fused
+void ps_launch_mlp_apply(size_t n, const float* W, const float4* pos4, const uint32_t* leaf_of,
+                         const float4* geom, const float4* mp, const float4* loc,
+                         const uint32_t* parent, float* fx, float* fy, float* fz, hipStream_t s);
+// gradient of (1/n) sum |net(feat) - target|^2 into grad[PS_MLP_PARAMS] and the
+// sum of squared errors into *loss (device); tmp: ps_mlp_grad_tmp_floats(n)
+size_t ps_mlp_grad_tmp_floats(size_t n);
+void ps_launch_mlp_grad(const float* W, const float* feat, const float* target, size_t n,
+                        float* grad, float* loss, float* tmp, hipStream_t s);
+// one Adam step (step counts from 1)
+void ps_launch_mlp_adam(float* W, const float* grad, float* m1, float* m2, float lr, float b1,
+                        float b2, float eps, int step, hipStream_t s);

@@ -207,6 +207,10 @@ int ps_hip_batch_get_stats(const ps_hip_batch_t* b, ps_hip_batch_stats_t* out);
 int ps_hip_engine_set_mlp(ps_hip_engine_t* e, const float* params);
 /* features feat[n][24] and scales scale[n] of the last tick (sorted order) */
 int ps_hip_mlp_features(ps_hip_engine_t* e, float* feat, float* scale);
+/* the same for the sorted positions idx[rows] only: row r is particle idx[r]
+ * (an index >= n gives a zero row) */
+int ps_hip_mlp_features_at(ps_hip_engine_t* e, const uint32_t* idx, size_t rows,
+                           float* feat, float* scale);
 /* out[n][3] = net(feat) */
 int ps_hip_mlp_forward(ps_hip_engine_t* e, const float* params, const float* feat,
                        size_t n, float* out3);

@@ -110,6 +110,7 @@ def hip_lib():
             "ps_hip_op_copy": ([_vp, _vp, C.c_size_t], C.c_int),
             "ps_hip_engine_set_mlp": ([_vp, _vp], C.c_int),
             "ps_hip_mlp_features": ([_vp, _vp, _vp], C.c_int),
+            "ps_hip_mlp_features_at": ([_vp, _vp, _sz, _vp, _vp], C.c_int),
             "ps_hip_mlp_forward": ([_vp, _vp, _vp, _sz, _vp], C.c_int),
             "ps_hip_mlp_grad": ([_vp, _vp, _vp, _vp, _sz, _vp, C.POINTER(_f)], C.c_int),
             "ps_hip_mlp_adam": ([_vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, C.c_int], C.c_int),

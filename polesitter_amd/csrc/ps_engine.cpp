@@ -737,8 +737,21 @@ int ps_hip_engine_set_mlp(ps_hip_engine_t* e, const float* params) {
 int ps_hip_mlp_features(ps_hip_engine_t* e, float* feat, float* scale) {
     if (!e || !feat || !scale || !e->built) return PS_HIP_EINVAL;
     if (!is_device_ptr(feat) || !is_device_ptr(scale)) return PS_HIP_EINVAL;
-    ps_launch_mlp_features(e->n, e->pos4(), e->leaf_of(), e->geom(), e->mp(), e->loc(),
-                           e->parent(), feat, scale, e->s);
+    ps_launch_mlp_features(e->n, nullptr, e->n, e->pos4(), e->leaf_of(), e->geom(), e->mp(),
+                           e->loc(), e->parent(), feat, scale, e->s);
+    PS_CK(hipGetLastError());
+    PS_CK(hipStreamSynchronize(e->s));
+    return PS_HIP_OK;
+}
+
+int ps_hip_mlp_features_at(ps_hip_engine_t* e, const uint32_t* idx, size_t rows, float* feat,
+                           float* scale) {
+    if (!e || !idx || !feat || !scale || !e->built) return PS_HIP_EINVAL;
+    if (rows == 0) return PS_HIP_OK;
+    if (!is_device_ptr(idx) || !is_device_ptr(feat) || !is_device_ptr(scale))
+        return PS_HIP_EINVAL;
+    ps_launch_mlp_features(e->n, idx, rows, e->pos4(), e->leaf_of(), e->geom(), e->mp(),
+                           e->loc(), e->parent(), feat, scale, e->s);
     PS_CK(hipGetLastError());
     PS_CK(hipStreamSynchronize(e->s));
     return PS_HIP_OK;

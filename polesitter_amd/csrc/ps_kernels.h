@@ -199,11 +199,13 @@ hipError_t ps_op_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 #define PS_MLP_IN 24
 #define PS_MLP_HIDDEN 64
 #define PS_MLP_PARAMS 10115  // W1[64][24] b1 W2[64][64] b2 W3[64][64] b3 W4[3][64] b4[3]
-// per particle (sorted order) of the last tick: features feat[n][24] and the
-// leaf's field magnitude scale[n]
-void ps_launch_mlp_features(size_t n, const float4* pos4, const uint32_t* leaf_of,
-                            const float4* geom, const float4* mp, const float4* loc,
-                            const uint32_t* parent, float* feat, float* scale, hipStream_t s);
+// rows of the last tick's features feat[rows][24] and leaf field magnitudes
+// scale[rows]: row r is particle idx[r] (sorted order; idx null: particle r);
+// an index >= n gives a zero row
+void ps_launch_mlp_features(size_t n, const uint32_t* idx, size_t rows, const float4* pos4,
+                            const uint32_t* leaf_of, const float4* geom, const float4* mp,
+                            const float4* loc, const uint32_t* parent, float* feat, float* scale,
+                            hipStream_t s);
 // out[n][3] = net(feat)
 void ps_launch_mlp_forward(const float* W, const float* feat, size_t n, float* out,
                            hipStream_t s);

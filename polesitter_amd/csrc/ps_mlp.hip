@@ -138,17 +138,27 @@ __device__ __forceinline__ void mlp_net(const float* __restrict__ W, const float
     }
 }
 
+// row r of (feat, scale) = features of particle idx[r] (idx null: r); rows
+// whose index is not a particle (>= n) get zeros
 __global__ __launch_bounds__(kFwdBlock) void mlp_features_kernel(
-    size_t n, const float4* __restrict__ pos4, const uint32_t* __restrict__ leaf_of,
-    const float4* __restrict__ geom, const float4* __restrict__ mp,
-    const float4* __restrict__ loc, const uint32_t* __restrict__ parent, float* __restrict__ feat,
-    float* __restrict__ scale) {
-    const size_t i = (size_t)blockIdx.x * kFwdBlock + threadIdx.x;
-    if (i >= n) return;
+    size_t n, const uint32_t* __restrict__ idx, size_t rows, const float4* __restrict__ pos4,
+    const uint32_t* __restrict__ leaf_of, const float4* __restrict__ geom,
+    const float4* __restrict__ mp, const float4* __restrict__ loc,
+    const uint32_t* __restrict__ parent, float* __restrict__ feat, float* __restrict__ scale) {
+    const size_t r = (size_t)blockIdx.x * kFwdBlock + threadIdx.x;
+    if (r >= rows) return;
+    const size_t i = idx ? (size_t)idx[r] : r;
     float f[kIn];
-    scale[i] = mlp_features(i, pos4, leaf_of, geom, mp, loc, parent, f);
+    float s = 0.f;
+    if (i < n) {
+        s = mlp_features(i, pos4, leaf_of, geom, mp, loc, parent, f);
+    } else {
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) feat[i * kIn + k] = f[k];
+        for (int k = 0; k < kIn; ++k) f[k] = 0.f;
+    }
+    scale[r] = s;
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) feat[r * kIn + k] = f[k];
 }
 
 __global__ __launch_bounds__(kFwdBlock) void mlp_forward_kernel(const float* __restrict__ W,
@@ -396,12 +406,13 @@ __global__ __launch_bounds__(256) void mlp_adam_kernel(float* __restrict__ W,
 
 }  // namespace
 
-void ps_launch_mlp_features(size_t n, const float4* pos4, const uint32_t* leaf_of,
-                            const float4* geom, const float4* mp, const float4* loc,
-                            const uint32_t* parent, float* feat, float* scale, hipStream_t s) {
-    if (n == 0) return;
-    mlp_features_kernel<<<(unsigned)((n + kFwdBlock - 1) / kFwdBlock), kFwdBlock, 0, s>>>(
-        n, pos4, leaf_of, geom, mp, loc, parent, feat, scale);
+void ps_launch_mlp_features(size_t n, const uint32_t* idx, size_t rows, const float4* pos4,
+                            const uint32_t* leaf_of, const float4* geom, const float4* mp,
+                            const float4* loc, const uint32_t* parent, float* feat, float* scale,
+                            hipStream_t s) {
+    if (rows == 0) return;
+    mlp_features_kernel<<<(unsigned)((rows + kFwdBlock - 1) / kFwdBlock), kFwdBlock, 0, s>>>(
+        n, idx, rows, pos4, leaf_of, geom, mp, loc, parent, feat, scale);
 }
 
 void ps_launch_mlp_forward(const float* W, const float* feat, size_t n, float* out,

@@ -54,10 +54,11 @@ def _log(log, msg):
         log(msg)
 
 
-def collect(x, y, z, m, v, ticks: int, samples: int = 4096, dt: float = 1e-2,
+def collect(x, y, z, m, v, ticks: int, samples: int = 4096, dt: float = 1e-3,
             theta: float = 2.0, leaf_cap: int = 1, mode: int = api.MORTON_AVX2,
             seed: int = 0, log=None) -> dict:
-    """Residual samples of `ticks` ticks of one evolving system."""
+    """Residual samples of `ticks` ticks of one evolving system (the caller's
+    frame step with time step dt between ticks)."""
     n = len(x)
     L = api.hip_lib()
     F, S, AF, AX, M = [], [], [], [], []
@@ -66,24 +67,28 @@ def collect(x, y, z, m, v, ticks: int, samples: int = 4096, dt: float = 1e-2,
         s.load(x, y, z, m)
         vel = [api.DeviceArray.from_numpy(np.asarray(c, np.float32)) for c in v]
         tmp = api.DeviceArray(3 * n, np.float32)
-        feat = api.DeviceArray(n * N_IN, np.float32)
-        scale = api.DeviceArray(n, np.float32)
+        rows = min(samples, n)
+        idx = api.DeviceArray(rows, np.uint32)
+        feat = api.DeviceArray(rows * N_IN, np.float32)
+        scale = api.DeviceArray(rows, np.float32)
         for k in range(ticks):
             s.reset_ids()
             s.tick(mode)
-            api.check("ps_hip_mlp_features", L.ps_hip_mlp_features(s.ctx.engine, feat.ptr,
-                                                                   scale.ptr))
-            r = s.result()
-            tg = W.sample_targets(n, samples, seed=seed * 100003 + k)
+            ids = s.id.numpy()
+            tg = W.sample_targets(n, rows, seed=seed * 100003 + k)
             inv = np.empty(n, np.int64)
-            inv[r["id"]] = np.arange(n)
-            pos = inv[tg]
+            inv[ids] = np.arange(n)
+            pos = np.ascontiguousarray(inv[tg], np.uint32)
+            idx.upload(pos)
+            api.check("ps_hip_mlp_features_at", L.ps_hip_mlp_features_at(
+                s.ctx.engine, idx.ptr, rows, feat.ptr, scale.ptr))
             fx = s.direct_f64(pos)
-            mass = r["m"][pos].astype(np.float64)
+            mass = s.m.numpy()[pos].astype(np.float64)
             ok = mass > 0
-            fa = np.stack([r["fx"], r["fy"], r["fz"]], 1)[pos].astype(np.float64)
-            fe = feat.numpy().reshape(n, N_IN)[pos]
-            sc = scale.numpy()[pos].astype(np.float64)
+            fa = np.stack([s.fx.numpy()[pos], s.fy.numpy()[pos], s.fz.numpy()[pos]],
+                          1).astype(np.float64)
+            fe = feat.numpy().reshape(rows, N_IN)
+            sc = scale.numpy().astype(np.float64)
             F.append(fe[ok])
             S.append(sc[ok])
             AF.append(fa[ok] / mass[ok, None])
@@ -182,7 +187,7 @@ def train(train_set: dict, steps: int = 3000, batch: int = 8192, lr: float = 1e-
     return t.params(), t, hist
 
 
-def build_correction(x, y, z, m, v, train_ticks=200, val_ticks=20, samples=4096, dt=1e-2,
+def build_correction(x, y, z, m, v, train_ticks=200, val_ticks=20, samples=4096, dt=1e-3,
                      theta=2.0, leaf_cap=1, steps=3000, batch=8192, lr=1e-3, seed=0,
                      val_system=None, log=None) -> dict:
     """Collect, train and validate.  val_system = (x, y, z, m, v) of another

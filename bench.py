@@ -161,33 +161,36 @@ def _timed_steps(s, n, v, steps, warmup, mmode):
 
 def bench_mlp(a):
     """The learned far-field correction at N = 4,194,304 Plummer (a = 1):
-    trained on --mlp-ticks ticks of residuals at the coarse setting (leaf
-    capacity --mlp-cap), validated on another realisation (the plain FMM
-    judges: the correction is used only if it lowers the error), then
-    compared on a third realisation with the plain FMM at the reference
-    setting (leaf capacity 1): FP64 RMS error on 4096 targets and throughput
-    per step (tick + id-gather + Euler).  value = corrected throughput."""
+    trained on --mlp-ticks ticks of residuals at the coarse setting (theta
+    --mlp-theta, leaf capacity --mlp-cap; a smaller theta is coarser here:
+    far <=> d^2 > theta^2 (hw_t + hw_s)^2), validated on another realisation
+    (the plain FMM judges: the correction is used only if it lowers the
+    error), then compared on a third realisation with the plain FMM at the
+    reference setting (theta --theta, leaf capacity 1): FP64 RMS error on
+    4096 targets and throughput per step (tick + id-gather + Euler).
+    value = corrected throughput."""
     from polesitter_amd import api, mlp
     from polesitter_amd import workloads as W
     n = 1 << 22
     mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
-    log(f"bench: {api.device_name()} workload mlp4m: Plummer N={n}, coarse leaf capacity "
-        f"{a.mlp_cap}")
+    th_c = a.mlp_theta if a.mlp_theta else a.theta
+    log(f"bench: {api.device_name()} workload mlp4m: Plummer N={n}, coarse theta {th_c} "
+        f"leaf capacity {a.mlp_cap}")
     zero = [np.zeros(n, np.float32)] * 3
     x, y, z, m = W.plummer(n, seed=1)
     val = W.plummer(n, seed=2) + (zero,)
     t0 = time.time()
     res = mlp.build_correction(x, y, z, m, zero, train_ticks=a.mlp_ticks, val_ticks=10,
-                               samples=4096, theta=a.theta, leaf_cap=a.mlp_cap, steps=3000,
+                               samples=4096, theta=th_c, leaf_cap=a.mlp_cap, steps=3000,
                                batch=8192, lr=2e-3, seed=1, val_system=val, log=log)
     train_s = time.time() - t0
     L = api.hip_lib()
     ex, ey, ez, em = W.plummer(n, seed=3)
     out = {}
-    for name, cap, use in (("plain_reference_setting", 1, False),
-                           ("corrected_coarse", a.mlp_cap, res["enabled"]),
-                           ("plain_coarse", a.mlp_cap, False)):
-        with api.Solver(n, theta=a.theta) as s:
+    for name, th, cap, use in (("plain_reference_setting", a.theta, 1, False),
+                               ("corrected_coarse", th_c, a.mlp_cap, res["enabled"]),
+                               ("plain_coarse", th_c, a.mlp_cap, False)):
+        with api.Solver(n, theta=th) as s:
             api.check("opt", api.ps_set_option(s.ctx, api.OPT_LEAF_CAP, cap))
             if use:
                 api.check("set_mlp", L.ps_hip_engine_set_mlp(s.ctx.engine,
@@ -200,7 +203,7 @@ def bench_mlp(a):
             err = fp64_error(n, f, r["id"], s.direct_f64)
             s.load(ex, ey, ez, em)
             dt = _timed_steps(s, n, zero, a.steps, a.warmup, mmode)
-        out[name] = dict(leaf_cap=cap, correction=bool(use), fp64_rms_err=err[0],
+        out[name] = dict(theta=th, leaf_cap=cap, correction=bool(use), fp64_rms_err=err[0],
                          fp64_max_err=err[1], ms_per_step=round(1e3 * dt, 4),
                          particles_per_s=round(n / dt, 1))
         log(f"bench: {name}: {out[name]}")
@@ -210,9 +213,11 @@ def bench_mlp(a):
                 higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
                 data="synthetic",
                 config=dict(workload="mlp4m", n_particles=n, theta=a.theta, morton=a.morton,
-                            coarse_leaf_cap=a.mlp_cap, train_ticks=a.mlp_ticks,
+                            coarse_theta=th_c, coarse_leaf_cap=a.mlp_cap,
+                            train_ticks=a.mlp_ticks,
                             step="tick + id-gather + semi-implicit Euler dt=1e-3",
-                            net="24-64-64-64-3 GELU, FP32"),
+                            net="24-64-64-64-3 GELU (tanh form), FP32 weights; in-tick "
+                                "on MFMA with bf16 hi/lo operand pairs, FP32 accumulation"),
                 results=out, speedup_vs_plain_reference_setting=round(
                     c["particles_per_s"] / p["particles_per_s"], 3),
                 error_ratio_vs_plain_reference_setting=round(
@@ -298,6 +303,8 @@ def main(argv=None):
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--mlp-cap", type=int, default=16,
                     help="mlp4m: leaf capacity of the corrected coarse setting")
+    ap.add_argument("--mlp-theta", type=float, default=0.0,
+                    help="mlp4m: theta of the corrected coarse setting (0: --theta)")
     ap.add_argument("--mlp-ticks", type=int, default=200, help="mlp4m: training ticks")
     ap.add_argument("--host-ticks", type=int, default=3,
                     help="ticks timed with host (pageable numpy) arrays, PCIe included; 0 = skip")

@@ -3,8 +3,9 @@ tests/ use it to check the HIP kernels; the product path never calls it).
 
 The correction has no counterpart in the reference; what pins this file is
 mathematics: the backward pass is checked against central finite
-differences of the forward pass (tests/test_oracle_mlp.py), and GELU is the
-exact erf form.  Everything is float64.
+differences of the forward pass (tests/test_oracle_mlp.py), and GELU (tanh
+form) against its definition and the erf form it approximates.  Everything
+is float64.
 
 Layout of the packed parameters (same as the kernels):
     W1[64][24] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[3][64] b4[3]
@@ -12,7 +13,6 @@ Layout of the packed parameters (same as the kernels):
 from __future__ import annotations
 
 import numpy as np
-from scipy.special import erf
 
 N_IN, N_H, N_OUT = 24, 64, 3
 SHAPES = [("W1", (N_H, N_IN)), ("b1", (N_H,)), ("W2", (N_H, N_H)), ("b2", (N_H,)),
@@ -36,12 +36,18 @@ def pack(d):
     return np.concatenate([np.asarray(d[name], np.float64).ravel() for name, _ in SHAPES])
 
 
+GELU_C = np.sqrt(2.0 / np.pi)
+GELU_A = 0.044715
+
+
 def gelu(x):
-    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+    """GELU, tanh form: x/2 (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))"""
+    return 0.5 * x * (1.0 + np.tanh(GELU_C * (x + GELU_A * x ** 3)))
 
 
 def gelu_d(x):
-    return 0.5 * (1.0 + erf(x / np.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / np.sqrt(2.0 * np.pi)
+    t = np.tanh(GELU_C * (x + GELU_A * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * GELU_C * (1.0 + 3.0 * GELU_A * x * x)
 
 
 def forward(params, feat):

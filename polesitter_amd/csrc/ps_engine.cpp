@@ -144,6 +144,7 @@ struct ps_hip_engine {
     uint32_t p2p_big = 24;  // leaves of >= this many particles: wavefront per leaf
     int leaf_cap = 1;       // most particles of a leaf above depth 10 (the reference: 1)
     DevBuf mlp_b;           // learned correction parameters (empty: none)
+    DevBuf mlp_pk_b;        // the same as the MFMA kernel's packed operands
     bool mlp_on = false;
     DevBuf mlp_tmp;         // training scratch
 
@@ -233,7 +234,7 @@ struct ps_hip_engine {
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
                          &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,
-                         &small_b,   &block_pairs_b, &tstate_b, &big_b,  &mlp_b, &mlp_tmp};
+                         &small_b,   &block_pairs_b, &tstate_b, &big_b,  &mlp_b, &mlp_pk_b, &mlp_tmp};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
             for (auto& b : es) release(b);
@@ -461,7 +462,7 @@ int run_tick(ps_hip_engine* e) {
     PS_RC(stage_p2p(e, 0, 6));  // records event 6 right before the P2P launch
     e->record(7);
     if (e->mlp_on) {
-        ps_launch_mlp_apply(e->n, reinterpret_cast<const float*>(e->mlp_b.p), e->pos4(),
+        ps_launch_mlp_apply(e->n, e->mlp_pk_b.p, e->pos4(),
                             e->leaf_of(), e->geom(), e->mp(), e->loc(), e->parent(), e->fx, e->fy,
                             e->fz, e->s);
         PS_CK(hipGetLastError());
@@ -602,7 +603,7 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
                   reinterpret_cast<uint32_t*>(e->big_b.p), nullptr, nullptr, s, e->s2, e->fork,
                   e->join);
     if (e->mlp_on)
-        ps_launch_mlp_apply(n, reinterpret_cast<const float*>(e->mlp_b.p), e->pos4(),
+        ps_launch_mlp_apply(n, e->mlp_pk_b.p, e->pos4(),
                             e->leaf_of(), e->geom(), e->mp(), e->loc(), e->parent(), e->fx, e->fy,
                             e->fz, s);
     return map_err(hipGetLastError());
@@ -728,7 +729,10 @@ int ps_hip_engine_set_mlp(ps_hip_engine_t* e, const float* params) {
         return PS_HIP_OK;
     }
     PS_CK(grow(e->mlp_b, 4 * PS_MLP_PARAMS, e->s));
+    PS_CK(grow(e->mlp_pk_b, PS_MLP_PACKED_BYTES, e->s));
     PS_CK(hipMemcpyAsync(e->mlp_b.p, params, 4 * PS_MLP_PARAMS, hipMemcpyDefault, e->s));
+    ps_launch_mlp_pack(reinterpret_cast<const float*>(e->mlp_b.p), e->mlp_pk_b.p, e->s);
+    PS_CK(hipGetLastError());
     PS_CK(hipStreamSynchronize(e->s));
     e->mlp_on = true;
     return PS_HIP_OK;

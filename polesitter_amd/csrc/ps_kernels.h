@@ -209,10 +209,14 @@ void ps_launch_mlp_features(size_t n, const uint32_t* idx, size_t rows, const fl
 // out[n][3] = net(feat)
 void ps_launch_mlp_forward(const float* W, const float* feat, size_t n, float* out,
                            hipStream_t s);
-// f += m * scale * net(features), fused
-void ps_launch_mlp_apply(size_t n, const float* W, const float4* pos4, const uint32_t* leaf_of,
-                         const float4* geom, const float4* mp, const float4* loc,
-                         const uint32_t* parent, float* fx, float* fy, float* fz, hipStream_t s);
+// the parameters W as the MFMA kernel's operands (PS_MLP_PACKED_BYTES)
+#define PS_MLP_PACKED_BYTES 58368
+void ps_launch_mlp_pack(const float* W, void* packed, hipStream_t s);
+// f += m * scale * net(features), fused, on the matrix cores (packed weights)
+void ps_launch_mlp_apply(size_t n, const void* packed, const float4* pos4,
+                         const uint32_t* leaf_of, const float4* geom, const float4* mp,
+                         const float4* loc, const uint32_t* parent, float* fx, float* fy,
+                         float* fz, hipStream_t s);
 // gradient of (1/n) sum |net(feat) - target|^2 into grad[PS_MLP_PARAMS] and the
 // sum of squared errors into *loss (device); tmp: ps_mlp_grad_tmp_floats(n)
 size_t ps_mlp_grad_tmp_floats(size_t n);

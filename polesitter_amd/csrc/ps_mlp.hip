@@ -11,7 +11,7 @@
 //     cell's own M2L field (its local minus its parent's, 3 each), the field
 //     arriving from above G (3), the leaf's mass centroid offset (3); fields
 //     divided by s = |E_L|, the leaf's total far field.
-//   net: 24 -> 64 -> 64 -> 64 -> 3, GELU (exact, erf) on the hidden layers,
+//   net: 24 -> 64 -> 64 -> 64 -> 3, GELU (tanh form) on the hidden layers,
 //     FP32 weights and accumulation.
 //   correction: a_i += s * net(F_i), i.e. f_i += m_i * s * net(F_i).
 //
@@ -33,12 +33,25 @@ static_assert(oB4 + kOut == PS_MLP_PARAMS, "parameter layout");
 constexpr int kFwdBlock = 128;
 constexpr int kRow = kH + 1;  // LDS row stride (no bank conflicts on columns)
 
-__device__ __forceinline__ float gelu(float x) {
-    return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+// GELU in its tanh form (Hendrycks & Gimpel 2016):
+//   gelu(x) = x/2 (1 + tanh(u)) = x sigma(2u),  u = c (x + a x^3),
+//   c = sqrt(2/pi), a = 0.044715,
+//   gelu'(x) = sigma + x sigma (1 - sigma) 2c (1 + 3a x^2).
+// sigma(2u) = 1 / (1 + 2^(-2u log2 e)) with v_exp_f32 and v_rcp_f32: a large
+// negative x gives 2^(+big) = inf and sigma = 0, a large positive one sigma = 1.
+constexpr float kGeluA = 0.044715f;
+constexpr float kGelu2C = 1.5957691216057308f;           // 2 sqrt(2/pi)
+constexpr float kGeluE0 = -2.3022081983492596f;          // -2 sqrt(2/pi) log2(e)
+constexpr float kGeluE1 = kGeluE0 * kGeluA;
+
+__device__ __forceinline__ float gelu_sigma(float x) {
+    const float z = x * fmaf(x * x, kGeluE1, kGeluE0);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z));
 }
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sigma(x); }
 __device__ __forceinline__ float gelu_d(float x) {
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) +
-           x * 0.39894228040143268f * expf(-0.5f * x * x);
+    const float sg = gelu_sigma(x);
+    return fmaf(x * sg * (1.0f - sg), kGelu2C * fmaf(3.0f * kGeluA, x * x, 1.0f), sg);
 }
 
 // ---------------------------------------------------------------------------
@@ -54,22 +67,28 @@ __device__ __forceinline__ float mlp_features(size_t i, const float4* __restrict
     const float4 p = pos4[i];
     const uint32_t L = leaf_of[i];
     const uint32_t P = parent[L];
-    const uint32_t G = P != PS_NONE ? parent[P] : PS_NONE;
+    const bool hasP = P != PS_NONE;
+    const uint32_t Pi = hasP ? P : L;  // always a valid node: no branches around the loads
+    const uint32_t G = hasP ? parent[Pi] : PS_NONE;
+    const bool hasG = G != PS_NONE;
+    const uint32_t Gi = hasG ? G : Pi;
     const float4 gL = geom[L], lL = loc[L], mL = mp[L];
-    const float4 gP = P != PS_NONE ? geom[P] : gL;
-    const float4 lP = P != PS_NONE ? loc[P] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 gG = G != PS_NONE ? geom[G] : gP;
-    const float4 lG = G != PS_NONE ? loc[G] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float s = sqrtf(lL.y * lL.y + lL.z * lL.z + lL.w * lL.w);
-    const float inv = s > 0.f ? 1.0f / s : 0.f;
-    const float hl = 1.0f / gL.w, hp = 1.0f / gP.w, hg = 1.0f / gG.w;
+    const float4 gP = geom[Pi], gG = geom[Gi];
+    float4 lP = loc[Pi], lG = loc[Gi];
+    if (!hasP) lP = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!hasG) lG = make_float4(0.f, 0.f, 0.f, 0.f);
+    // v_sqrt_f32 / v_rcp_f32 (within 1 ulp): the same function computes the
+    // training features, so inference sees exactly what training saw
+    const float s = __builtin_amdgcn_sqrtf(lL.y * lL.y + lL.z * lL.z + lL.w * lL.w);
+    const float inv = s > 0.f ? __builtin_amdgcn_rcpf(s) : 0.f;
+    const float hl = __builtin_amdgcn_rcpf(gL.w), hp = __builtin_amdgcn_rcpf(gP.w),
+                hg = __builtin_amdgcn_rcpf(gG.w);
     f[0] = (p.x - gL.x) * hl;
     f[1] = (p.y - gL.y) * hl;
     f[2] = (p.z - gL.z) * hl;
     f[3] = (lL.y - lP.y) * inv;
     f[4] = (lL.z - lP.z) * inv;
     f[5] = (lL.w - lP.w) * inv;
-    const bool hasP = P != PS_NONE, hasG = G != PS_NONE;
     f[6] = hasP ? (p.x - gP.x) * hp : 0.f;
     f[7] = hasP ? (p.y - gP.y) * hp : 0.f;
     f[8] = hasP ? (p.z - gP.z) * hp : 0.f;
@@ -87,7 +106,7 @@ __device__ __forceinline__ float mlp_features(size_t i, const float4* __restrict
     f[18] = 0.f;
     f[19] = 0.f;
     f[20] = 0.f;
-    const float cm = mL.x != 0.f ? hl / mL.x : 0.f;
+    const float cm = mL.x != 0.f ? hl * __builtin_amdgcn_rcpf(mL.x) : 0.f;
     f[21] = mL.y * cm;
     f[22] = mL.z * cm;
     f[23] = mL.w * cm;
@@ -176,25 +195,177 @@ __global__ __launch_bounds__(kFwdBlock) void mlp_forward_kernel(const float* __r
         for (int c = 0; c < kOut; ++c) out[i * kOut + c] = o[c];
 }
 
-// the corrected tick's last step: f_i += m_i * s_i * net(F_i)
-__global__ __launch_bounds__(kFwdBlock) void mlp_apply_kernel(
-    size_t n, const float* __restrict__ W, const float4* __restrict__ pos4,
+// ---------------------------------------------------------------------------
+// The corrected tick's last step, f_i += m_i * s_i * net(F_i), on the matrix
+// cores.  Each layer is D[out][particle] = W[out][in] x H[in][particle] with
+// v_mfma_f32_16x16x32_bf16: a wavefront takes 16 particles (the 16 columns),
+// 64 outputs are 4 row tiles, an input dimension of 64 two K blocks of 32
+// (the 24 features: one block, zero padded).  FP32 weights and activations
+// enter as bf16 pairs hi + lo (hi = bf16(v), lo = bf16(v - hi): 16 mantissa
+// bits); each product is hi*hi + hi*lo + lo*hi (three MFMAs, the dropped
+// lo*lo is below 2^-16 relative), accumulated in FP32.  The bias is the
+// accumulator's initial value.
+//
+// Lane l holds row i = l % 16 of A and column j = l % 16 of B, both with the
+// eight K elements of group g = l / 16; D element e of lane l is row 4g + e,
+// column j.  Element e of K block kb in lane group g stands for hidden unit
+// 16 (2 kb + e / 4) + 4g + e % 4 -- exactly the D element (tile 2 kb + e / 4,
+// element e % 4) the same lane holds after the previous layer -- so the
+// activations pass from one layer's accumulators to the next layer's B
+// operand without leaving the lane; the packed weights follow the same
+// order.  For the first layer element e of group g is feature 8g + e.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kFrags = 22;      // (layer, row tile, K block): 4 + 8 + 8 + 2
+constexpr int kBiasTiles = 13;  // (layer, row tile): 4 + 4 + 4 + 1
+constexpr int kApplyBlock = 512;
+static_assert(PS_MLP_PACKED_BYTES == kFrags * 2 * 64 * 16 + kBiasTiles * 64 * 16, "packed size");
+
+// hidden unit of element e of K block kb in lane group g (layers 2 to 4)
+__device__ __forceinline__ int mlp_hidden_k(int kb, int g, int e) {
+    return 16 * (2 * kb + (e >> 2)) + 4 * g + (e & 3);
+}
+
+// packed layout: bf16 fragments [frag][hi, lo][lane][8], then float4 biases
+// [tile][lane]; one thread per element
+__global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__ W,
+                                                       uint32_t* __restrict__ pk) {
+    const int tid = blockIdx.x * 256 + threadIdx.x;
+    constexpr int kFragElems = kFrags * 64 * 8;
+    if (tid < kFragElems) {
+        const int e = tid & 7, l = (tid >> 3) & 63, f = tid >> 9;
+        const int i = l & 15, g = l >> 4;
+        float w = 0.f;
+        if (f < 4) {  // layer 1: row tile f, features 8g + e
+            const int o = 16 * f + i, k = 8 * g + e;
+            if (k < kIn) w = W[oW1 + o * kIn + k];
+        } else if (f < 20) {  // layers 2 and 3: row tile t, K block kb
+            const int q = f < 12 ? f - 4 : f - 12;
+            const int o = 16 * (q >> 1) + i, k = mlp_hidden_k(q & 1, g, e);
+            w = W[(f < 12 ? oW2 : oW3) + o * kH + k];
+        } else {  // layer 4: rows 0..2 of one tile
+            if (i < kOut) w = W[oW4 + i * kH + mlp_hidden_k(f - 20, g, e)];
+        }
+        const __bf16 hi = (__bf16)w;
+        const __bf16 lo = (__bf16)(w - (float)hi);
+        __bf16* b = reinterpret_cast<__bf16*>(pk);
+        b[((f * 2 + 0) * 64 + l) * 8 + e] = hi;
+        b[((f * 2 + 1) * 64 + l) * 8 + e] = lo;
+    } else if (tid < kFragElems + kBiasTiles * 64 * 4) {
+        const int r = tid - kFragElems;
+        const int e = r & 3, l = (r >> 2) & 63, T = r >> 8;
+        const int o = 16 * (T & 3) + 4 * (l >> 4) + e;
+        float b = 0.f;
+        if (T < 4) b = W[oB1 + o];
+        else if (T < 8) b = W[oB2 + o];
+        else if (T < 12) b = W[oB3 + o];
+        else if (o < kOut) b = W[oB4 + o];
+        // the fragments take kFragElems words (a hi and a lo bf16 per element)
+        reinterpret_cast<float*>(pk)[kFragElems + r] = b;
+    }
+}
+
+__device__ __forceinline__ void split_bf16(const float v[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 h = (__bf16)v[e];
+        hi[e] = h;
+        lo[e] = (__bf16)(v[e] - (float)h);
+    }
+}
+
+// acc += A(frag) x (bh + bl), A = ah + al: three MFMAs
+__device__ __forceinline__ f32x4 mfma3(const uint4* __restrict__ sfr, int f, int lane, bf16x8 bh,
+                                       bf16x8 bl, f32x4 acc) {
+    const uint4 uh = sfr[(f * 2 + 0) * 64 + lane];
+    const uint4 ul = sfr[(f * 2 + 1) * 64 + lane];
+    const bf16x8 ah = __builtin_bit_cast(bf16x8, uh);
+    const bf16x8 al = __builtin_bit_cast(bf16x8, ul);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+    return acc;
+}
+
+__device__ __forceinline__ f32x4 bias4(const float4* __restrict__ sbi, int T, int lane) {
+    const float4 b = sbi[T * 64 + lane];
+    f32x4 r;
+    r[0] = b.x;
+    r[1] = b.y;
+    r[2] = b.z;
+    r[3] = b.w;
+    return r;
+}
+
+// GELU of the four row tiles, as the next layer's two K blocks (hi, lo)
+__device__ __forceinline__ void hidden_operands(const f32x4 acc[4], bf16x8 hh[2], bf16x8 hl[2]) {
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = gelu(acc[2 * kb + (e >> 2)][e & 3]);
+        split_bf16(v, hh[kb], hl[kb]);
+    }
+}
+
+__global__ __launch_bounds__(kApplyBlock, 4) void mlp_apply_kernel(
+    size_t n, const uint4* __restrict__ pk, const float4* __restrict__ pos4,
     const uint32_t* __restrict__ leaf_of, const float4* __restrict__ geom,
     const float4* __restrict__ mp, const float4* __restrict__ loc,
     const uint32_t* __restrict__ parent, float* __restrict__ fx, float* __restrict__ fy,
     float* __restrict__ fz) {
-    __shared__ float rows[kFwdBlock * kRow];
-    const size_t i = (size_t)blockIdx.x * kFwdBlock + threadIdx.x;
-    const size_t ii = i < n ? i : (n ? n - 1 : 0);
-    float f[kIn];
-    const float s = mlp_features(ii, pos4, leaf_of, geom, mp, loc, parent, f);
-    float o[kOut];
-    mlp_net(W, f, rows + threadIdx.x * kRow, o);
-    if (i >= n || s == 0.f) return;
-    const float w = pos4[i].w * s;
-    fx[i] = fmaf(w, o[0], fx[i]);
-    fy[i] = fmaf(w, o[1], fy[i]);
-    fz[i] = fmaf(w, o[2], fz[i]);
+    __shared__ uint4 sfr[kFrags * 2 * 64];
+    __shared__ float4 sbi[kBiasTiles * 64];
+    for (int k = threadIdx.x; k < kFrags * 2 * 64; k += kApplyBlock) sfr[k] = pk[k];
+    const float4* pb = reinterpret_cast<const float4*>(pk + kFrags * 2 * 64);
+    for (int k = threadIdx.x; k < kBiasTiles * 64; k += kApplyBlock) sbi[k] = pb[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const size_t tiles = (n + 15) / 16;
+    const size_t nw = (size_t)gridDim.x * (kApplyBlock / 64);
+    // wave-uniform loop: every lane is active at each MFMA
+    for (size_t tile = (size_t)blockIdx.x * (kApplyBlock / 64) + (threadIdx.x >> 6); tile < tiles;
+         tile += nw) {
+        const size_t i = tile * 16 + j;
+        const size_t ii = i < n ? i : n - 1;
+        float f[kIn];
+        const float s = mlp_features(ii, pos4, leaf_of, geom, mp, loc, parent, f);
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = g == 0 ? f[e] : g == 1 ? f[8 + e] : g == 2 ? f[16 + e] : 0.f;
+        bf16x8 bh, bl;
+        split_bf16(v, bh, bl);
+        f32x4 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = mfma3(sfr, t, lane, bh, bl, bias4(sbi, t, lane));
+#pragma unroll 1
+        for (int layer = 0; layer < 2; ++layer) {  // layers 2 and 3
+            bf16x8 hh[2], hl[2];
+            hidden_operands(acc, hh, hl);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                f32x4 a = bias4(sbi, 4 + 4 * layer + t, lane);
+                a = mfma3(sfr, 4 + 8 * layer + 2 * t + 0, lane, hh[0], hl[0], a);
+                a = mfma3(sfr, 4 + 8 * layer + 2 * t + 1, lane, hh[1], hl[1], a);
+                acc[t] = a;
+            }
+        }
+        bf16x8 hh[2], hl[2];
+        hidden_operands(acc, hh, hl);
+        f32x4 o = bias4(sbi, 12, lane);
+        o = mfma3(sfr, 20, lane, hh[0], hl[0], o);
+        o = mfma3(sfr, 21, lane, hh[1], hl[1], o);
+        // lanes of group 0 hold the three outputs of particle j
+        if (g == 0 && i < n && s != 0.f) {
+            const float w = pos4[i].w * s;
+            fx[i] = fmaf(w, o[0], fx[i]);
+            fy[i] = fmaf(w, o[1], fy[i]);
+            fz[i] = fmaf(w, o[2], fz[i]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -422,13 +593,23 @@ void ps_launch_mlp_forward(const float* W, const float* feat, size_t n, float* o
         W, feat, n, out);
 }
 
-void ps_launch_mlp_apply(size_t n, const float* W, const float4* pos4, const uint32_t* leaf_of,
+void ps_launch_mlp_pack(const float* W, void* packed, hipStream_t s) {
+    constexpr int total = kFrags * 64 * 8 + kBiasTiles * 64 * 4;
+    mlp_pack_kernel<<<(total + 255) / 256, 256, 0, s>>>(W, reinterpret_cast<uint32_t*>(packed));
+}
+
+void ps_launch_mlp_apply(size_t n, const void* packed, const float4* pos4, const uint32_t* leaf_of,
                          const float4* geom, const float4* mp, const float4* loc,
                          const uint32_t* parent, float* fx, float* fy, float* fz,
                          hipStream_t s) {
     if (n == 0) return;
-    mlp_apply_kernel<<<(unsigned)((n + kFwdBlock - 1) / kFwdBlock), kFwdBlock, 0, s>>>(
-        n, W, pos4, leaf_of, geom, mp, loc, parent, fx, fy, fz);
+    constexpr size_t kWaves = kApplyBlock / 64;
+    const size_t tiles = (n + 15) / 16;
+    size_t blocks = (tiles + kWaves - 1) / kWaves;
+    if (blocks > 1024) blocks = 1024;  // two blocks per CU (LDS), grid-stride beyond
+    mlp_apply_kernel<<<(unsigned)blocks, kApplyBlock, 0, s>>>(
+        n, reinterpret_cast<const uint4*>(packed), pos4, leaf_of, geom, mp, loc, parent, fx, fy,
+        fz);
 }
 
 size_t ps_mlp_grad_tmp_floats(size_t n) {

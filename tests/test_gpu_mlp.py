@@ -104,12 +104,18 @@ def test_correction_applied_in_tick():
         fe = feat.numpy().reshape(n, 24)
         sc = scale.numpy().astype(np.float64)
     net, _ = R.forward(params.astype(np.float64), fe.astype(np.float64))
-    want = np.stack([plain["fx"], plain["fy"], plain["fz"]], 1).astype(np.float64) + \
-        (plain["m"].astype(np.float64) * sc)[:, None] * net
-    got = np.stack([corr["fx"], corr["fy"], corr["fz"]], 1).astype(np.float64)
+    base = np.stack([plain["fx"], plain["fy"], plain["fz"]], 1).astype(np.float64)
+    want = (plain["m"].astype(np.float64) * sc)[:, None] * net
+    got = np.stack([corr["fx"], corr["fy"], corr["fz"]], 1).astype(np.float64) - base
     assert np.array_equal(corr["id"], plain["id"])
+    # the in-tick net runs on the matrix cores with FP32 operands split into
+    # bf16 hi + lo (16-bit mantissas, FP32 accumulation): the correction term
+    # itself agrees with the float64 net to 1e-4
     err = np.linalg.norm(got - want) / np.linalg.norm(want)
-    assert err < 1e-5, err
+    log(f"correction term relative error {err:.2e}")
+    assert err < 1e-4, err
+    # particles without a far field (s = 0) are left alone
+    assert np.array_equal(got[sc == 0], np.zeros_like(got[sc == 0]))
 
 
 def test_training_lowers_the_held_out_error():

@@ -1,6 +1,7 @@
 """The CPU check of the learned correction, pinned by mathematics: its
 backward pass against central finite differences of its forward pass, GELU
-against its definition, and the packing round trip."""
+(tanh form) against its definition and the erf form, and the packing round
+trip."""
 import math
 import os
 import sys
@@ -18,12 +19,18 @@ def test_param_count_and_packing():
     assert np.array_equal(R.pack(R.unpack(p)), p)
 
 
-def test_gelu_is_the_erf_form():
-    for x in (-3.0, -0.5, 0.0, 0.7, 2.5):
-        assert abs(R.gelu(np.array(x)) - 0.5 * x * (1 + math.erf(x / math.sqrt(2)))) < 1e-15
+def test_gelu_tanh_form():
+    c = math.sqrt(2 / math.pi)
+    for x in (-30.0, -3.0, -0.5, 0.0, 0.7, 2.5, 30.0):
+        want = 0.5 * x * (1 + math.tanh(c * (x + 0.044715 * x ** 3)))
+        assert abs(R.gelu(np.array(x)) - want) <= 1e-15 * max(1.0, abs(x))
+        # the tanh form stays within 1e-3 of the exact erf form
+        assert abs(R.gelu(np.array(x)) - 0.5 * x * (1 + math.erf(x / math.sqrt(2)))) < 1e-3
         h = 1e-6
         fd = (R.gelu(np.array(x + h)) - R.gelu(np.array(x - h))) / (2 * h)
         assert abs(R.gelu_d(np.array(x)) - fd) < 1e-8
+    # limits: 0 far left, identity far right
+    assert R.gelu(np.array(-30.0)) == 0.0 and R.gelu(np.array(30.0)) == 30.0
 
 
 def test_backward_matches_finite_differences():

@@ -6,7 +6,8 @@ truncated at 20a, isotropic, equal masses 1/N, float32 SoA), theta = 2.0,
 the reference's threaded-build traversal.  One step = one tick
 (ps_prepare_particles + ps_calc_forces, particles device-resident) followed
 by the caller's frame step on the device (id-gather of velocities,
-semi-implicit Euler dt = 1e-3), so every tick re-sorts and rebuilds.
+semi-implicit Euler dt = 1e-3, ids reset for the next frame; one fused call),
+so every tick re-sorts and rebuilds.
 
     python bench.py [--steps K] [--warmup W]
                     [--workload plummer1m|twoplummer16m|plummer4m|ensemble512|mlp4m]
@@ -141,14 +142,9 @@ def _timed_steps(s, n, v, steps, warmup, mmode):
     vel = [api.DeviceArray.from_numpy(vv) for vv in v]
     tmp = api.DeviceArray(3 * n, np.float32)
 
-    def step():
-        s.reset_ids()
+    def step():  # ids are 0..n-1 on entry (load, or the previous frame step)
         s.tick(mmode)
-        api.check("gather", L.ps_hip_step_gather(s.ctx.engine, s.id.ptr, vel[0].ptr, vel[1].ptr,
-                                                 vel[2].ptr, tmp.ptr, n))
-        api.check("euler", L.ps_hip_step_euler(s.ctx.engine, s.x.ptr, s.y.ptr, s.z.ptr,
-                                               vel[0].ptr, vel[1].ptr, vel[2].ptr, s.fx.ptr,
-                                               s.fy.ptr, s.fz.ptr, s.m.ptr, 1e-3, n))
+        s.frame_step(vel, tmp, 1e-3)
     for _ in range(warmup):
         step()
     api.check("sync", L.ps_hip_synchronize())
@@ -189,7 +185,8 @@ def bench_mlp(a):
     out = {}
     for name, th, cap, use in (("plain_reference_setting", a.theta, 1, False),
                                ("corrected_coarse", th_c, a.mlp_cap, res["enabled"]),
-                               ("plain_coarse", th_c, a.mlp_cap, False)):
+                               ("plain_coarse", th_c, a.mlp_cap, False),
+                               ("plain_reference_theta_coarse_leaf", a.theta, a.mlp_cap, False)):
         with api.Solver(n, theta=th) as s:
             api.check("opt", api.ps_set_option(s.ctx, api.OPT_LEAF_CAP, cap))
             if use:
@@ -203,11 +200,20 @@ def bench_mlp(a):
             err = fp64_error(n, f, r["id"], s.direct_f64)
             s.load(ex, ey, ez, em)
             dt = _timed_steps(s, n, zero, a.steps, a.warmup, mmode)
+            # stage times of one more tick (events only)
+            api.check("opt", api.ps_set_option(s.ctx, api.OPT_STATS, 2))
+            s.reset_ids()
+            s.tick(mmode)
+            st = s.stats()
         out[name] = dict(theta=th, leaf_cap=cap, correction=bool(use), fp64_rms_err=err[0],
                          fp64_max_err=err[1], ms_per_step=round(1e3 * dt, 4),
-                         particles_per_s=round(n / dt, 1))
+                         particles_per_s=round(n / dt, 1),
+                         stages_ms={k: round(st["ms_" + k], 4) for k in
+                                    ("sort", "build", "upward", "traverse", "l2l", "p2p", "mlp",
+                                     "total")})
         log(f"bench: {name}: {out[name]}")
     c, p = out["corrected_coarse"], out["plain_reference_setting"]
+    q = out["plain_reference_theta_coarse_leaf"]
     line = dict(metric=METRIC, value=c["particles_per_s"], unit="particles/s", n_gpus=1,
                 steps=a.steps, warmup=a.warmup, ms_per_step=c["ms_per_step"],
                 higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
@@ -222,6 +228,10 @@ def bench_mlp(a):
                     c["particles_per_s"] / p["particles_per_s"], 3),
                 error_ratio_vs_plain_reference_setting=round(
                     c["fp64_rms_err"] / p["fp64_rms_err"], 4),
+                speedup_vs_plain_reference_theta_coarse_leaf=round(
+                    c["particles_per_s"] / q["particles_per_s"], 3),
+                error_ratio_vs_plain_reference_theta_coarse_leaf=round(
+                    c["fp64_rms_err"] / q["fp64_rms_err"], 4),
                 validation=dict(plain=res["val_plain"], corrected=res["val_corrected"],
                                 enabled=res["enabled"], train_samples=res["train_samples"],
                                 train_seconds=round(train_s, 1)))
@@ -303,7 +313,7 @@ def main(argv=None):
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--mlp-cap", type=int, default=16,
                     help="mlp4m: leaf capacity of the corrected coarse setting")
-    ap.add_argument("--mlp-theta", type=float, default=0.0,
+    ap.add_argument("--mlp-theta", type=float, default=1.9,
                     help="mlp4m: theta of the corrected coarse setting (0: --theta)")
     ap.add_argument("--mlp-ticks", type=int, default=200, help="mlp4m: training ticks")
     ap.add_argument("--host-ticks", type=int, default=3,
@@ -333,14 +343,9 @@ def main(argv=None):
 
     mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
 
-    def step():
-        s.reset_ids()
+    def step():  # ids are 0..n-1 on entry (load, or the previous frame step)
         s.tick(mmode)
-        api.check("gather", L.ps_hip_step_gather(s.ctx.engine, s.id.ptr, vel[0].ptr, vel[1].ptr,
-                                                 vel[2].ptr, tmp.ptr, n))
-        api.check("euler", L.ps_hip_step_euler(s.ctx.engine, s.x.ptr, s.y.ptr, s.z.ptr,
-                                               vel[0].ptr, vel[1].ptr, vel[2].ptr, s.fx.ptr,
-                                               s.fy.ptr, s.fz.ptr, s.m.ptr, dt, n))
+        s.frame_step(vel, tmp, dt)
         return s.stats()
 
     # error of the first tick (initial state), untimed
@@ -352,6 +357,26 @@ def main(argv=None):
         f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
         err = fp64_error(n, f, r["id"], s.direct_f64)
         log(f"bench: GPU FP64 error on 4096 targets rms={err[0]:.3e} max={err[1]:.3e}")
+        # the same tick with the other accumulation (compensated vs plain FP32)
+        # in the far field and P2P sums, on the same targets
+        s.load(x, y, z, m)
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_KAHAN, 0 if a.kahan else 1))
+        s.reset_ids()
+        s.tick(mmode)
+        r2 = s.result()
+        f2 = np.stack([r2["fx"], r2["fy"], r2["fz"]], 1)
+        err2 = fp64_error(n, f2, r2["id"], s.direct_f64)
+        ms_other = s.stats()["ms_total"]
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_KAHAN, 1 if a.kahan else 0))
+        s.reset_ids()
+        s.tick(mmode)
+        ms_this = s.stats()["ms_total"]
+        this, other = ("kahan", "plain") if a.kahan else ("plain", "kahan")
+        accumulation = {this: dict(fp64_rms_err=err[0], fp64_max_err=err[1],
+                                   ms_tick=round(ms_this, 4)),
+                        other: dict(fp64_rms_err=err2[0], fp64_max_err=err2[1],
+                                    ms_tick=round(ms_other, 4))}
+        log(f"bench: {other} accumulation: FP64 rms={err2[0]:.6e} max={err2[1]:.4e}")
         s.load(x, y, z, m)
 
     # warm-up ticks count the P2P pairs (extra reductions); the timed ticks
@@ -414,6 +439,7 @@ def main(argv=None):
                             iterations=int(stats[-1]["iterations"])))
     if err:
         line["fp64_rms_err"], line["fp64_max_err"] = err
+        line["accumulation"] = accumulation
     if a.host_ticks > 0:
         # the same tick through host arrays: ps_calc_forces stages them through
         # device buffers and copies the sorted state and forces back

@@ -170,6 +170,15 @@ int ps_hip_step_euler(ps_hip_engine_t* e, float* x, float* y, float* z,
                       float* vx, float* vy, float* vz, const float* fx,
                       const float* fy, const float* fz, const float* mass,
                       float dt, size_t n);
+/* the three fused, after a tick: gather the velocities by id, semi-implicit
+ * Euler (massless particles keep x and the gathered v), then reset the ids
+ * to 0..n-1 for the next frame (README.md:94-161 from the gather to the top
+ * of the next frame).  tmp3: 3n device floats of scratch.  Equal to
+ * ps_hip_step_gather + ps_hip_step_euler + ps_hip_step_reset_ids. */
+int ps_hip_step_frame(ps_hip_engine_t* e, uint32_t* id, float* vx, float* vy,
+                      float* vz, float* tmp3, float* x, float* y, float* z,
+                      const float* fx, const float* fy, const float* fz,
+                      const float* mass, float dt, size_t n);
 
 /* no reference counterpart (batched entry point): one tick of many
  * independent systems -- for each, ps_prepare_particles (H:1911-2170) and

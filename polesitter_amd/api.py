@@ -123,6 +123,7 @@ def hip_lib():
             "ps_hip_step_reset_ids": ([_vp, _vp, _sz], C.c_int),
             "ps_hip_step_gather": ([_vp, _vp, _vp, _vp, _vp, _vp, _sz], C.c_int),
             "ps_hip_step_euler": ([_vp] * 11 + [_f, _sz], C.c_int),
+            "ps_hip_step_frame": ([_vp] * 13 + [_f, _sz], C.c_int),
             "ps_hip_malloc": ([_sz], _vp),
             "ps_hip_free": ([_vp], C.c_int),
             "ps_hip_memcpy": ([_vp, _vp, _sz, C.c_int], C.c_int),
@@ -362,6 +363,15 @@ class Solver:
     def reset_ids(self):
         check("ps_hip_step_reset_ids",
               hip_lib().ps_hip_step_reset_ids(self.ctx.engine, self.id.ptr, self.n))
+
+    def frame_step(self, vel, tmp, dt):
+        """the caller's frame step after a tick (README.md:94-161): gather the
+        device velocity arrays vel = (vx, vy, vz) by id, semi-implicit Euler
+        with time step dt, reset the ids; tmp: 3n device floats"""
+        check("ps_hip_step_frame", hip_lib().ps_hip_step_frame(
+            self.ctx.engine, self.id.ptr, vel[0].ptr, vel[1].ptr, vel[2].ptr, tmp.ptr,
+            self.x.ptr, self.y.ptr, self.z.ptr, self.fx.ptr, self.fy.ptr, self.fz.ptr,
+            self.m.ptr, dt, self.n))
 
     def tick(self, mode=None):
         """prepare + calc_forces with root = min + range/2 (README.md:104-126)."""

@@ -1210,6 +1210,17 @@ int ps_hip_step_euler(ps_hip_engine_t* e, float* x, float* y, float* z, float* v
     return PS_HIP_OK;
 }
 
+int ps_hip_step_frame(ps_hip_engine_t* e, uint32_t* id, float* vx, float* vy, float* vz,
+                      float* tmp3, float* x, float* y, float* z, const float* fx,
+                      const float* fy, const float* fz, const float* mass, float dt, size_t n) {
+    if (!e || !id || !vx || !vy || !vz || !tmp3 || !x || !y || !z || !fx || !fy || !fz || !mass)
+        return PS_HIP_EINVAL;
+    ps_launch_frame(id, vx, vy, vz, tmp3, x, y, z, fx, fy, fz, mass, dt, n, e->s);
+    PS_CK(hipGetLastError());
+    PS_CK(hipStreamSynchronize(e->s));
+    return PS_HIP_OK;
+}
+
 void* ps_hip_malloc(size_t bytes) {
     void* p = nullptr;
     if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {

@@ -187,6 +187,10 @@ void ps_launch_euler(float* x, float* y, float* z, float* vx, float* vy, float* 
                      const float* fx, const float* fy, const float* fz, const float* m,
                      float dt, size_t n, hipStream_t s);
 void ps_launch_iota(uint32_t* id, size_t n, hipStream_t s);
+// gather + Euler + reset ids in two kernels; tmp3: 3n floats of scratch
+void ps_launch_frame(uint32_t* id, float* vx, float* vy, float* vz, float* tmp3, float* x,
+                     float* y, float* z, const float* fx, const float* fy, const float* fz,
+                     const float* m, float dt, size_t n, hipStream_t s);
 
 // ---- ps_ops.hip: single-operator hooks for the known-answer tests ----------
 hipError_t ps_op_m2l(const float* tgeom, const float* sgeom, const float* smp, float theta,

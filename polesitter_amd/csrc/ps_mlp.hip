@@ -220,7 +220,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kFrags = 22;      // (layer, row tile, K block): 4 + 8 + 8 + 2
 constexpr int kBiasTiles = 13;  // (layer, row tile): 4 + 4 + 4 + 1
-constexpr int kApplyBlock = 512;
+constexpr int kApplyBlock = 1024;
 static_assert(PS_MLP_PACKED_BYTES == kFrags * 2 * 64 * 16 + kBiasTiles * 64 * 16, "packed size");
 
 // hidden unit of element e of K block kb in lane group g (layers 2 to 4)
@@ -310,7 +310,21 @@ __device__ __forceinline__ void hidden_operands(const f32x4 acc[4], bf16x8 hh[2]
     }
 }
 
-__global__ __launch_bounds__(kApplyBlock, 4) void mlp_apply_kernel(
+// One wavefront per 64 particles at a time: lane l computes the features of
+// particle p0 + l into the wave's LDS rows (all 64 lanes busy), then the wave
+// runs the net on the four 16-particle column tiles, reading each lane's B
+// operand (eight features of one particle) from those rows.  The rows are
+// private to the wave, so a wavefront-scope fence and barrier order the
+// writes before the reads (and the reads before the next round's writes).
+constexpr int kApplyWaves = kApplyBlock / 64;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(kApplyBlock) void mlp_apply_kernel(
     size_t n, const uint4* __restrict__ pk, const float4* __restrict__ pos4,
     const uint32_t* __restrict__ leaf_of, const float4* __restrict__ geom,
     const float4* __restrict__ mp, const float4* __restrict__ loc,
@@ -318,53 +332,77 @@ __global__ __launch_bounds__(kApplyBlock, 4) void mlp_apply_kernel(
     float* __restrict__ fz) {
     __shared__ uint4 sfr[kFrags * 2 * 64];
     __shared__ float4 sbi[kBiasTiles * 64];
+    __shared__ float4 sft[kApplyWaves][64 * (kIn / 4)];  // features, 6 float4 per particle
+    __shared__ float ssc[kApplyWaves][64];                // s = |E_L| per particle
     for (int k = threadIdx.x; k < kFrags * 2 * 64; k += kApplyBlock) sfr[k] = pk[k];
     const float4* pb = reinterpret_cast<const float4*>(pk + kFrags * 2 * 64);
     for (int k = threadIdx.x; k < kBiasTiles * 64; k += kApplyBlock) sbi[k] = pb[k];
     __syncthreads();
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, j = lane & 15;
-    const size_t tiles = (n + 15) / 16;
-    const size_t nw = (size_t)gridDim.x * (kApplyBlock / 64);
+    float4* ft = sft[wave];
+    float* sc = ssc[wave];
+    const size_t rounds = (n + 63) / 64;
+    const size_t nw = (size_t)gridDim.x * kApplyWaves;
     // wave-uniform loop: every lane is active at each MFMA
-    for (size_t tile = (size_t)blockIdx.x * (kApplyBlock / 64) + (threadIdx.x >> 6); tile < tiles;
-         tile += nw) {
-        const size_t i = tile * 16 + j;
-        const size_t ii = i < n ? i : n - 1;
-        float f[kIn];
-        const float s = mlp_features(ii, pos4, leaf_of, geom, mp, loc, parent, f);
-        float v[8];
+    for (size_t r = (size_t)blockIdx.x * kApplyWaves + wave; r < rounds; r += nw) {
+        const size_t p0 = r * 64;
+        {
+            const size_t i = p0 + lane;
+            float f[kIn];
+            sc[lane] = mlp_features(i < n ? i : n - 1, pos4, leaf_of, geom, mp, loc, parent, f);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = g == 0 ? f[e] : g == 1 ? f[8 + e] : g == 2 ? f[16 + e] : 0.f;
-        bf16x8 bh, bl;
-        split_bf16(v, bh, bl);
-        f32x4 acc[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = mfma3(sfr, t, lane, bh, bl, bias4(sbi, t, lane));
+            for (int q = 0; q < kIn / 4; ++q)
+                ft[lane * (kIn / 4) + q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2],
+                                                       f[4 * q + 3]);
+        }
+        wave_sync();
 #pragma unroll 1
-        for (int layer = 0; layer < 2; ++layer) {  // layers 2 and 3
+        for (int c = 0; c < 4; ++c) {
+            const int row = 16 * c + j;
+            float v[8];
+            if (g < 3) {
+                const float4 a = ft[row * (kIn / 4) + 2 * g];
+                const float4 b = ft[row * (kIn / 4) + 2 * g + 1];
+                v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+                v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = 0.f;
+            }
+            bf16x8 bh, bl;
+            split_bf16(v, bh, bl);
+            f32x4 acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = mfma3(sfr, t, lane, bh, bl, bias4(sbi, t, lane));
+#pragma unroll 1
+            for (int layer = 0; layer < 2; ++layer) {  // layers 2 and 3
+                bf16x8 hh[2], hl[2];
+                hidden_operands(acc, hh, hl);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    f32x4 a = bias4(sbi, 4 + 4 * layer + t, lane);
+                    a = mfma3(sfr, 4 + 8 * layer + 2 * t + 0, lane, hh[0], hl[0], a);
+                    a = mfma3(sfr, 4 + 8 * layer + 2 * t + 1, lane, hh[1], hl[1], a);
+                    acc[t] = a;
+                }
+            }
             bf16x8 hh[2], hl[2];
             hidden_operands(acc, hh, hl);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                f32x4 a = bias4(sbi, 4 + 4 * layer + t, lane);
-                a = mfma3(sfr, 4 + 8 * layer + 2 * t + 0, lane, hh[0], hl[0], a);
-                a = mfma3(sfr, 4 + 8 * layer + 2 * t + 1, lane, hh[1], hl[1], a);
-                acc[t] = a;
+            f32x4 o = bias4(sbi, 12, lane);
+            o = mfma3(sfr, 20, lane, hh[0], hl[0], o);
+            o = mfma3(sfr, 21, lane, hh[1], hl[1], o);
+            // lanes of group 0 hold the three outputs of particle p0 + row
+            const size_t i = p0 + row;
+            const float s = sc[row];
+            if (g == 0 && i < n && s != 0.f) {
+                const float w = pos4[i].w * s;
+                fx[i] = fmaf(w, o[0], fx[i]);
+                fy[i] = fmaf(w, o[1], fy[i]);
+                fz[i] = fmaf(w, o[2], fz[i]);
             }
         }
-        bf16x8 hh[2], hl[2];
-        hidden_operands(acc, hh, hl);
-        f32x4 o = bias4(sbi, 12, lane);
-        o = mfma3(sfr, 20, lane, hh[0], hl[0], o);
-        o = mfma3(sfr, 21, lane, hh[1], hl[1], o);
-        // lanes of group 0 hold the three outputs of particle j
-        if (g == 0 && i < n && s != 0.f) {
-            const float w = pos4[i].w * s;
-            fx[i] = fmaf(w, o[0], fx[i]);
-            fy[i] = fmaf(w, o[1], fy[i]);
-            fz[i] = fmaf(w, o[2], fz[i]);
-        }
+        wave_sync();  // the rows are rewritten by the next round
     }
 }
 
@@ -603,10 +641,9 @@ void ps_launch_mlp_apply(size_t n, const void* packed, const float4* pos4, const
                          const uint32_t* parent, float* fx, float* fy, float* fz,
                          hipStream_t s) {
     if (n == 0) return;
-    constexpr size_t kWaves = kApplyBlock / 64;
-    const size_t tiles = (n + 15) / 16;
-    size_t blocks = (tiles + kWaves - 1) / kWaves;
-    if (blocks > 1024) blocks = 1024;  // two blocks per CU (LDS), grid-stride beyond
+    const size_t rounds = (n + 63) / 64;
+    size_t blocks = (rounds + kApplyWaves - 1) / kApplyWaves;
+    if (blocks > 512) blocks = 512;  // one block per CU (LDS), grid-stride beyond
     mlp_apply_kernel<<<(unsigned)blocks, kApplyBlock, 0, s>>>(
         n, reinterpret_cast<const uint4*>(packed), pos4, leaf_of, geom, mp, loc, parent, fx, fy,
         fz);

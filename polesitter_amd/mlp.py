@@ -71,8 +71,7 @@ def collect(x, y, z, m, v, ticks: int, samples: int = 4096, dt: float = 1e-3,
         idx = api.DeviceArray(rows, np.uint32)
         feat = api.DeviceArray(rows * N_IN, np.float32)
         scale = api.DeviceArray(rows, np.float32)
-        for k in range(ticks):
-            s.reset_ids()
+        for k in range(ticks):  # ids are 0..n-1 here (load, then every frame step)
             s.tick(mode)
             ids = s.id.numpy()
             tg = W.sample_targets(n, rows, seed=seed * 100003 + k)
@@ -95,11 +94,7 @@ def collect(x, y, z, m, v, ticks: int, samples: int = 4096, dt: float = 1e-3,
             AX.append(fx[ok] / mass[ok, None])
             M.append(mass[ok])
             # the caller's frame step (README.md:94-161) on the device
-            api.check("gather", L.ps_hip_step_gather(s.ctx.engine, s.id.ptr, vel[0].ptr,
-                                                     vel[1].ptr, vel[2].ptr, tmp.ptr, n))
-            api.check("euler", L.ps_hip_step_euler(s.ctx.engine, s.x.ptr, s.y.ptr, s.z.ptr,
-                                                   vel[0].ptr, vel[1].ptr, vel[2].ptr, s.fx.ptr,
-                                                   s.fy.ptr, s.fz.ptr, s.m.ptr, dt, n))
+            s.frame_step(vel, tmp, dt)
             if (k + 1) % 20 == 0 or k + 1 == ticks:
                 _log(log, f"mlp: collected tick {k + 1}/{ticks}")
     S_ = np.concatenate(S)

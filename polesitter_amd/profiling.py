@@ -85,14 +85,9 @@ def run(args):
     s.load(x, y, z, m)
     vel = [api.DeviceArray.from_numpy(vv) for vv in v]
     tmp = api.DeviceArray(3 * n, np.float32)
-    for k in range(args.steps):
-        s.reset_ids()
-        s.tick()
-        api.check("gather", L.ps_hip_step_gather(s.ctx.engine, s.id.ptr, vel[0].ptr, vel[1].ptr,
-                                                 vel[2].ptr, tmp.ptr, n))
-        api.check("euler", L.ps_hip_step_euler(s.ctx.engine, s.x.ptr, s.y.ptr, s.z.ptr,
-                                               vel[0].ptr, vel[1].ptr, vel[2].ptr, s.fx.ptr,
-                                               s.fy.ptr, s.fz.ptr, s.m.ptr, 1e-3, n))
+    for k in range(args.steps):  # the bench's step: tick, then the fused frame step
+        s.tick(api.MORTON_AVX2)
+        s.frame_step(vel, tmp, 1e-3)
         api.check("sync", L.ps_hip_synchronize())
         print(f"profiling: step {k + 1}/{args.steps} done", flush=True)
     s.close()

@@ -31,6 +31,34 @@ def test_hip_library_exports_everything_declared():
     assert not missing, missing
 
 
+def declared_arity(header):
+    """parameter count of every function the header declares"""
+    text = open(os.path.join(INCLUDE, header)).read()
+    text = text.split("#ifdef POLESITTER_IMPLEMENTATION")[0]
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for name in declared(header):
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", text)
+        params = m.group(1).strip()
+        out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_ctypes_signatures_match_the_header():
+    # every ps_hip_* binding of the Python mirror passes as many arguments
+    # as the C declaration takes
+    L = api.hip_lib()
+    arity = declared_arity("polesitter_hip.h")
+    bound = 0
+    for name, k in arity.items():
+        f = getattr(L, name)
+        if f.argtypes is None:
+            continue
+        bound += 1
+        assert len(f.argtypes) == k, (name, len(f.argtypes), k)
+    assert bound >= 20
+
+
 def test_host_library_exports_the_reference_api():
     L = api.host_lib()
     names = declared("polesitter.h")

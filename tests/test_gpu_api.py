@@ -129,6 +129,40 @@ def test_frame_step_on_device(oracle):
                 hv[k] = v_dev
 
 
+def test_fused_frame_step_equals_separate_steps():
+    # ps_hip_step_frame = gather + Euler + reset ids, bit for bit, massless
+    # particles included (they keep x and the gathered velocity)
+    n = 30_000
+    x, y, z, m, vx, vy, vz = W.two_plummers(n, seed=58, sep=2.0)
+    m = m.copy()
+    m[::97] = 0.0
+    dt = 1e-3
+    L = api.hip_lib()
+    outs = []
+    for fused in (False, True):
+        with api.Solver(n, theta=2.0) as s:
+            s.load(x, y, z, m)
+            dv = [api.DeviceArray.from_numpy(v) for v in (vx, vy, vz)]
+            tmp = api.DeviceArray(3 * n, np.float32)
+            for _ in range(3):
+                s.tick()
+                if fused:
+                    s.frame_step(dv, tmp, dt)
+                else:
+                    api.check("gather", L.ps_hip_step_gather(s.ctx.engine, s.id.ptr, dv[0].ptr,
+                                                            dv[1].ptr, dv[2].ptr, tmp.ptr, n))
+                    api.check("euler", L.ps_hip_step_euler(
+                        s.ctx.engine, s.x.ptr, s.y.ptr, s.z.ptr, dv[0].ptr, dv[1].ptr,
+                        dv[2].ptr, s.fx.ptr, s.fy.ptr, s.fz.ptr, s.m.ptr, dt, n))
+                    s.reset_ids()
+            r = s.result()
+            outs.append([r[k] for k in ("x", "y", "z", "m", "fx", "id")] +
+                        [v.numpy() for v in dv])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert np.array_equal(outs[1][5], np.arange(n, dtype=np.uint32))
+
+
 def test_dropin_c_program(oracle, tmp_path):
     # the reference README's frame loop compiled against include/polesitter.h
     exe = os.path.join(api.LIBDIR, "dropin_tick")

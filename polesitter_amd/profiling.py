@@ -23,6 +23,10 @@ Two uses:
     Markdown table of a ``--kernel-trace --stats`` run, with per-tick sums of
     each kernel family.
 
+``python -m polesitter_amd.profiling counters --dirs DIR... --command "..."``
+    Markdown table of ``--pmc`` passes (any counters): each counter summed
+    over every kernel family's dispatches of the run's last tick.
+
 FETCH_SIZE and WRITE_SIZE are KiB per dispatch (derived TCC counters).
 """
 from __future__ import annotations
@@ -45,6 +49,8 @@ FAMILIES = {
     "upward": ("upward_level_kernel",),
     "l2l": ("l2l_level_kernel",),
     "prepare": ("bbox_partial_kernel", "bbox_final_kernel", "morton_kernel"),
+    "mlp": ("mlp_apply_kernel",),
+    "frame": ("frame_kernel", "frame_finish_kernel"),
     "copy": ("op_copy_kernel",),
 }
 TICK_MARK = "trav_init_kernel"
@@ -226,6 +232,53 @@ def trace_summary(args):
     return 0
 
 
+def counters(args):
+    """Markdown table of --pmc passes: every counter summed over each kernel
+    family's dispatches of the last tick of the run (the dispatches after the
+    last trav_init_kernel), plus derived issue figures for the families."""
+    disp = defaultdict(dict)  # (path, dispatch) -> {counter: value}
+    names, order = {}, {}
+    for d in args.dirs:
+        for path in _csvs(d, "counter_collection.csv"):
+            with open(path, newline="") as f:
+                for r in csv.DictReader(f):
+                    key = (path, int(r["Dispatch_Id"]))
+                    c = r["Counter_Name"]
+                    disp[key][c] = disp[key].get(c, 0.0) + float(r["Counter_Value"])
+                    names[key] = r["Kernel_Name"]
+                    order[key] = int(r["Dispatch_Id"])
+    fam = defaultdict(lambda: defaultdict(float))
+    for path in sorted({k[0] for k in disp}):
+        keys = sorted((k for k in disp if k[0] == path), key=lambda k: order[k])
+        marks = [i for i, k in enumerate(keys) if TICK_MARK in names[k]]
+        if not marks:
+            continue
+        for k in keys[marks[-1]:]:
+            f = family_of(names[k])
+            if f:
+                for c, v in disp[k].items():
+                    fam[f][c] += v
+    lines = [f"# Counters: {args.title}", "", "Commands (one pass each, from the repository "
+             "root):", "", "```"] + list(args.command) + ["```", "",
+             "Summed over the last tick's dispatches of each kernel family.", ""]
+    cols = sorted({c for f in fam.values() for c in f})
+    fams = [f for f in FAMILIES if f in fam]
+    lines += ["| counter | " + " | ".join(fams) + " |", "|---|" + "---|" * len(fams)]
+    for c in cols:
+        lines.append(f"| {c} | " + " | ".join(f"{fam[f].get(c, 0.0):.4g}" for f in fams) + " |")
+    if args.ms:
+        lines += ["", "Issue figures (VALU wave-instructions x 4 cycles over the SIMD cycles of "
+                  f"{args.simds} SIMDs at {args.ghz} GHz in the family's time per tick):", ""]
+        for spec in args.ms:
+            f, ms = spec.split("=")
+            v = fam[f].get("SQ_INSTS_VALU", 0.0)
+            cyc = float(ms) * 1e-3 * args.ghz * 1e9 * args.simds
+            lines.append(f"* {f}: {v:.4g} VALU wave-instructions in {float(ms):.3f} ms -> "
+                         f"{100.0 * 4 * v / cyc:.1f} % of the SIMD issue cycles")
+    print("\n".join(lines))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m polesitter_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -246,11 +299,21 @@ def main(argv=None):
     t.add_argument("--title", default="tick")
     t.add_argument("--cmd", default="")
     t.add_argument("--top", type=int, default=16)
+    c = sub.add_parser("counters")
+    c.add_argument("--dirs", nargs="+", required=True)
+    c.add_argument("--title", default="tick")
+    c.add_argument("--command", action="append", default=[])
+    c.add_argument("--ms", action="append", default=[],
+                   help="family=ms per tick, for the issue figures")
+    c.add_argument("--simds", type=int, default=1024)
+    c.add_argument("--ghz", type=float, default=2.4)
     a = ap.parse_args(argv)
     if a.cmd == "run":
         return run(a)
     if a.cmd == "summarize":
         return summarize(a)
+    if a.cmd == "counters":
+        return counters(a)
     return trace_summary(a)
 
 

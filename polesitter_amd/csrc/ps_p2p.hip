@@ -60,6 +60,32 @@ __device__ __forceinline__ void p2p_uniform_range(float4 p, const float4* __rest
     }
 }
 
+// the same for T targets per lane: each scalar-loaded source feeds all T;
+// each target's sum runs over the sources in the same order as above
+template <bool kKahan, int T>
+__device__ __forceinline__ void p2p_uniform_range_t(const float4* p,
+                                                    const float4* __restrict__ pos4, uint32_t j0,
+                                                    uint32_t j1, float* a, float* c) {
+    // unrolled so that about four source loads are in flight per round
+    constexpr int kUnroll = T >= 4 ? 1 : 4 / T;
+#pragma unroll kUnroll
+    for (uint32_t j = j0; j < j1; ++j) {
+        const float4 q = pos4[j];
+#pragma unroll
+        for (int u = 0; u < T; ++u) {
+            if (kKahan) {
+                float tx = 0.f, ty = 0.f, tz = 0.f;
+                ps_pair(p[u], q, tx, ty, tz);
+                ps_kahan_add(a[3 * u + 0], c[3 * u + 0], tx);
+                ps_kahan_add(a[3 * u + 1], c[3 * u + 1], ty);
+                ps_kahan_add(a[3 * u + 2], c[3 * u + 2], tz);
+            } else {
+                ps_pair(p[u], q, a[3 * u + 0], a[3 * u + 1], a[3 * u + 2]);
+            }
+        }
+    }
+}
+
 // the big-leaf kernel's targets: leaves of >= tmin particles (tmin > 0)
 __device__ __forceinline__ bool p2p_is_big(uint32_t lf, uint32_t tmin,
                                            const uint2* __restrict__ link,
@@ -94,6 +120,46 @@ __device__ __forceinline__ void p2p_uniform_list(float4 p, size_t n, uint32_t L,
         }
     }
 }
+
+template <bool kKahan, int T>
+__device__ __forceinline__ void p2p_uniform_list_t(const float4* p, size_t n, uint32_t L,
+                                                   uint32_t o, uint32_t mo,
+                                                   const float4* __restrict__ pos4,
+                                                   const uint32_t* __restrict__ arena,
+                                                   const unsigned long long* __restrict__ masks,
+                                                   const uint2* __restrict__ link,
+                                                   const uint32_t* __restrict__ nend, float* a,
+                                                   float* c, uint32_t& pairs, uint32_t& nsrc) {
+    for (uint32_t k0 = 0; k0 < L; k0 += 64) {
+        unsigned long long near = ~masks[mo + (k0 >> 6)];
+        if (L - k0 < 64) near &= (1ull << (L - k0)) - 1ull;
+        while (near) {
+            const uint32_t b = (uint32_t)__ffsll((long long)near) - 1u;
+            near &= near - 1ull;
+            ++nsrc;
+            const uint32_t s = arena[o + k0 + b];
+            const uint32_t j0 = link[s].x;
+            const uint32_t j1e = nend[s];
+            const uint32_t j1 = j1e < (uint32_t)n ? j1e : (uint32_t)n;
+            pairs += j1e - j0;
+            p2p_uniform_range_t<kKahan, T>(p, pos4, j0, j1, a, c);
+        }
+    }
+}
+
+// one round of T x 64 targets from c0 (targets c0 + 64 u + lane, u < T): the
+// leaf's whole near field, then the stores
+template <int kMode, bool kKahan, int T>
+__device__ __forceinline__ void p2p_big_round(uint32_t c0, uint32_t te, uint32_t lane, size_t n,
+                                              uint32_t L, uint32_t o, uint32_t mo, float4 fl,
+                                              const float4* __restrict__ pos4,
+                                              const uint32_t* __restrict__ arena,
+                                              const unsigned long long* __restrict__ masks,
+                                              const uint2* __restrict__ link,
+                                              const uint32_t* __restrict__ nend,
+                                              float* __restrict__ fx, float* __restrict__ fy,
+                                              float* __restrict__ fz, uint32_t& pairs,
+                                              uint32_t& ws);
 
 // f for particle i from its leaf's field and its P2P sum (mode 0: both;
 // 1: field only; 2: adds the sum to f)
@@ -249,6 +315,47 @@ __global__ __launch_bounds__(kCollect) void p2p_collect_kernel(const uint32_t* _
     }
 }
 
+template <int kMode, bool kKahan, int T>
+__device__ __forceinline__ void p2p_big_round(uint32_t c0, uint32_t te, uint32_t lane, size_t n,
+                                              uint32_t L, uint32_t o, uint32_t mo, float4 fl,
+                                              const float4* __restrict__ pos4,
+                                              const uint32_t* __restrict__ arena,
+                                              const unsigned long long* __restrict__ masks,
+                                              const uint2* __restrict__ link,
+                                              const uint32_t* __restrict__ nend,
+                                              float* __restrict__ fx, float* __restrict__ fy,
+                                              float* __restrict__ fz, uint32_t& pairs,
+                                              uint32_t& ws) {
+    float4 p[T];
+    bool valid[T];
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+        const uint32_t i = c0 + 64 * u + lane;
+        valid[u] = i < te;
+        p[u] = valid[u] ? pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    float a[3 * T], c[3 * T];
+#pragma unroll
+    for (int k = 0; k < 3 * T; ++k) {
+        a[k] = 0.f;
+        c[k] = 0.f;
+    }
+    uint32_t wp = 0;
+    p2p_uniform_list_t<kKahan, T>(p, n, L, o, mo, pos4, arena, masks, link, nend, a, c, wp, ws);
+#pragma unroll
+    for (int u = 0; u < T; ++u) {
+        if (!valid[u]) continue;
+        pairs += wp;
+        float ax = a[3 * u], ay = a[3 * u + 1], az = a[3 * u + 2];
+        if (kKahan) {
+            ax -= c[3 * u];
+            ay -= c[3 * u + 1];
+            az -= c[3 * u + 2];
+        }
+        p2p_store<kMode>(c0 + 64 * u + lane, p[u], fl, ax, ay, az, fx, fy, fz);
+    }
+}
+
 // One wavefront per big leaf (>= tmin particles), 64 targets per
 // round: the whole wave shares the leaf's source list, so the loop runs on
 // scalar control flow and each source particle is one scalar load broadcast
@@ -279,25 +386,31 @@ __global__ __launch_bounds__(kBlock) void p2p_big_kernel(
         const uint32_t o = L ? p2p_off[node] : 0u;
         const uint32_t mo = L ? p2p_moff[node] : 0u;
         const float4 fl = (kMode != 2) ? loc[node] : make_float4(0.f, 0.f, 0.f, 0.f);
-        for (uint32_t c0 = t0; c0 < te; c0 += 64) {
-            const uint32_t i = c0 + lane;
-            const bool valid = i < te;
-            const float4 p = valid ? pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-            float ax = 0.f, ay = 0.f, az = 0.f;
-            float cx = 0.f, cy = 0.f, cz = 0.f;
-            uint32_t wp = 0, ws = 0;
-            p2p_uniform_list<kKahan>(p, n, L, o, mo, pos4, arena, masks, link, nend, ax, ay, az,
-                                     cx, cy, cz, wp, ws);
-            if (valid) {
-                pairs += wp;
-                if (kKahan) {
-                    ax -= cx;
-                    ay -= cy;
-                    az -= cz;
-                }
-                p2p_store<kMode>(i, p, fl, ax, ay, az, fx, fy, fz);
+        // rounds of 64 T targets, T = 1..4 from what remains: each source
+        // load then feeds T pairs per lane
+        bool first = true;  // the leaf pairs are counted once per leaf
+        for (uint32_t c0 = t0; c0 < te;) {
+            const uint32_t rem = te - c0;
+            uint32_t ws = 0;
+            if (rem > 192) {
+                p2p_big_round<kMode, kKahan, 4>(c0, te, lane, n, L, o, mo, fl, pos4, arena, masks,
+                                                link, nend, fx, fy, fz, pairs, ws);
+                c0 += 256;
+            } else if (rem > 128) {
+                p2p_big_round<kMode, kKahan, 3>(c0, te, lane, n, L, o, mo, fl, pos4, arena, masks,
+                                                link, nend, fx, fy, fz, pairs, ws);
+                c0 += 192;
+            } else if (rem > 64) {
+                p2p_big_round<kMode, kKahan, 2>(c0, te, lane, n, L, o, mo, fl, pos4, arena, masks,
+                                                link, nend, fx, fy, fz, pairs, ws);
+                c0 += 128;
+            } else {
+                p2p_big_round<kMode, kKahan, 1>(c0, te, lane, n, L, o, mo, fl, pos4, arena, masks,
+                                                link, nend, fx, fy, fz, pairs, ws);
+                c0 += 64;
             }
-            if (c0 == t0 && lane == 0) nsrc += ws;
+            if (first && lane == 0) nsrc += ws;
+            first = false;
         }
     }
     if (block_pairs) p2p_block_stats(pairs, nsrc, block_pairs, slot0 + blockIdx.x, stride);

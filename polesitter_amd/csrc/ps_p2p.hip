@@ -241,25 +241,51 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
             for (uint32_t k0 = 0; k0 < L; k0 += 64) {
                 unsigned long long near = ~masks[mo + (k0 >> 6)];
                 if (L - k0 < 64) near &= (1ull << (L - k0)) - 1ull;
+                // near sources eight at a time: the eight list words, then the
+                // eight ranges, are requested together (one round trip each
+                // instead of eight); the sums still run source by source in
+                // list order
                 while (near) {
-                    const uint32_t b = (uint32_t)__ffsll((long long)near) - 1u;
-                    near &= near - 1ull;
-                    ++nsrc;
-                    const uint32_t s = arena[o + k0 + b];
-                    const uint32_t j0 = link[s].x;
-                    const uint32_t j1e = nend[s];
-                    const uint32_t j1 = j1e < (uint32_t)n ? j1e : (uint32_t)n;
-                    pairs += j1e - j0;
-                    for (uint32_t j = j0; j < j1; ++j) {
-                        const float4 q = pos4[j];
-                        if (kKahan) {
-                            float tx = 0.f, ty = 0.f, tz = 0.f;
-                            ps_pair(p, q, tx, ty, tz);
-                            ps_kahan_add(ax, cx, tx);
-                            ps_kahan_add(ay, cy, ty);
-                            ps_kahan_add(az, cz, tz);
-                        } else {
-                            ps_pair(p, q, ax, ay, az);
+                    constexpr int kB = 8;
+                    uint32_t sb[kB];
+                    int cnt = 0;
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        sb[u] = 0u;
+                        if (near) {
+                            sb[u] = (uint32_t)__ffsll((long long)near) - 1u;
+                            near &= near - 1ull;
+                            ++cnt;
+                        }
+                    }
+                    uint32_t src[kB];
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) src[u] = u < cnt ? arena[o + k0 + sb[u]] : 0u;
+                    uint32_t r0[kB], r1[kB];
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        r0[u] = u < cnt ? link[src[u]].x : 0u;
+                        r1[u] = u < cnt ? nend[src[u]] : 0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kB; ++u) {
+                        if (u >= cnt) break;
+                        ++nsrc;
+                        const uint32_t j0 = r0[u];
+                        const uint32_t j1e = r1[u];
+                        const uint32_t j1 = j1e < (uint32_t)n ? j1e : (uint32_t)n;
+                        pairs += j1e - j0;
+                        for (uint32_t j = j0; j < j1; ++j) {
+                            const float4 q = pos4[j];
+                            if (kKahan) {
+                                float tx = 0.f, ty = 0.f, tz = 0.f;
+                                ps_pair(p, q, tx, ty, tz);
+                                ps_kahan_add(ax, cx, tx);
+                                ps_kahan_add(ay, cy, ty);
+                                ps_kahan_add(az, cz, tz);
+                            } else {
+                                ps_pair(p, q, ax, ay, az);
+                            }
                         }
                     }
                 }

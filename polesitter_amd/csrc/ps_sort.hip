@@ -8,8 +8,10 @@
 //   scan      exclusive scan of the digit-major table: the start of every
 //             (digit, tile) run in the output -- bucket-major, tile-minor,
 //             the same offset order as H:1661-1667
-//   downsweep the tile is ranked stably inside LDS (wave-wide match of equal
-//             digits with 10 ballots, per-wave counts, in-order rounds), then
+//   downsweep the tile is ranked stably inside LDS (each wavefront ranks its
+//             own quarter in order: wave-wide match of equal digits with 10
+//             ballots and the wave's running digit counts, then one scan of
+//             the counts over the waves -- three barriers per tile), then
 //             written out digit run by digit run, so consecutive lanes store
 //             to consecutive addresses; the payload (id, x, y, z, mass) is
 //             gathered from the tile's own input range, which is L2-resident.
@@ -59,7 +61,6 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
     __shared__ uint16_t sidx[kTile];
     __shared__ uint32_t tstart[kRadix];
     __shared__ uint32_t goff[kRadix];
-    __shared__ uint32_t run[kRadix];
     __shared__ uint16_t wcnt[kWaves][kRadix];
     __shared__ uint32_t wsum[kWaves];
 
@@ -71,7 +72,6 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
 
     for (int d = tid; d < kRadix; d += kBlock) {
         tstart[d] = 0;
-        run[d] = 0;
         goff[d] = offs[(size_t)d * n_tiles + blockIdx.x];
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) wcnt[w][d] = 0;
@@ -113,38 +113,58 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
     }
     __syncthreads();
 
-    // 3. stable ranking, in original order: round r covers keys
-    //    [r*256, r*256+256), lanes in order inside a wave, waves in order
+    // 3. stable ranking.  Wave w ranks the tile's keys [w Q, (w+1) Q),
+    //    Q = kTile / kWaves, in order (rounds of 64): a key's rank among the
+    //    equal digits of its wave so far comes from the round's digit match
+    //    and the wave's own running digit counts in wcnt[w] (no barriers);
+    //    then one exclusive scan over the waves per digit, and the key's
+    //    tile position is tstart[d] + (its digit's count in earlier waves) +
+    //    that rank -- the stable order, since the waves hold consecutive
+    //    ranges.
+    constexpr int kQ = kTile / kWaves;
+    constexpr int kRounds = kQ / 64;
     const uint64_t lt = ps_lanemask_lt();
-    for (int r = 0; r < kItems; ++r) {
-        if ((uint32_t)(r * kBlock) >= valid_n) break;  // block-uniform
-        uint32_t idx = (uint32_t)(r * kBlock + tid);
-        bool valid = idx < valid_n;
-        uint32_t d = (keys[idx] >> shift) & (kRadix - 1);
+    uint32_t dr[kRounds];  // per round: digit << 16 | rank inside the wave
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const uint32_t idx = (uint32_t)(wave * kQ + r * 64 + lane);
+        const bool valid = idx < valid_n;
+        const uint32_t d = (keys[idx] >> shift) & (kRadix - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < kRadixBits; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(valid && bit);
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(valid && bit);
             peers &= bit ? bb : ~bb;
         }
-        uint32_t rank = (uint32_t)__popcll(peers & lt);
-        uint32_t cnt = (uint32_t)__popcll(peers);
-        bool leader = valid && rank == 0;
-        if (leader) wcnt[wave][d] = (uint16_t)cnt;
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = tstart[d] + run[d] + rank;
-            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
-            sidx[pos] = (uint16_t)idx;
-        }
-        __syncthreads();
-        if (leader) {
-            atomicAdd(&run[d], cnt);
-            wcnt[wave][d] = 0;
-        }
-        __syncthreads();
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        const uint32_t cnt = (uint32_t)__popcll(peers);
+        // every lane of digit d reads the wave's count before its leader
+        // (the lowest lane of the digit) adds this round's keys
+        const uint32_t before = valid ? (uint32_t)wcnt[wave][d] : 0u;
+        if (valid && rank == 0) wcnt[wave][d] = (uint16_t)(before + cnt);
+        dr[r] = valid ? (d << 16) | (before + rank) : 0xFFFFFFFFu;
     }
+    __syncthreads();
+    // exclusive scan over the waves, per digit, in place
+    for (int d = tid; d < kRadix; d += kBlock) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = wcnt[w][d];
+            wcnt[w][d] = (uint16_t)acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        if (dr[r] == 0xFFFFFFFFu) continue;
+        const uint32_t d = dr[r] >> 16;
+        const uint32_t pos = tstart[d] + (uint32_t)wcnt[wave][d] + (dr[r] & 0xFFFFu);
+        sidx[pos] = (uint16_t)(wave * kQ + r * 64 + lane);
+    }
+    __syncthreads();
 
     // 4. write out digit runs: position p of the tile goes to
     //    goff[d] + (p - tstart[d])

@@ -164,6 +164,8 @@ struct ps_hip_engine {
     // small device / pinned host scalars
     DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs | u64 leaf pairs
     uint32_t* h_small = nullptr;  // pinned mirror
+    PsTravState* h_tstate = nullptr;  // pinned copy of the traversal's state
+    hipEvent_t ev_counts = nullptr;   // the tree's counts have reached h_small
 
     // arrays of the current tick
     float *x = nullptr, *y = nullptr, *z = nullptr, *m = nullptr;
@@ -240,6 +242,10 @@ struct ps_hip_engine {
             for (auto& b : es) release(b);
         if (h_small) (void)hipHostFree(h_small);
         h_small = nullptr;
+        if (h_tstate) (void)hipHostFree(h_tstate);
+        h_tstate = nullptr;
+        if (ev_counts) (void)hipEventDestroy(ev_counts);
+        ev_counts = nullptr;
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : ev_it)
@@ -278,6 +284,10 @@ hipError_t alloc_fixed(ps_hip_engine* e) {
     PS_AL(small_b, 256);
 #undef PS_AL
     r = hipHostMalloc(reinterpret_cast<void**>(&e->h_small), 256, hipHostMallocDefault);
+    if (r == hipSuccess)
+        r = hipHostMalloc(reinterpret_cast<void**>(&e->h_tstate), sizeof(PsTravState),
+                          hipHostMallocDefault);
+    if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_counts, hipEventDisableTiming);
     return r;
 }
 
@@ -303,12 +313,24 @@ int stage_sort(ps_hip_engine* e) {
     return PS_HIP_OK;
 }
 
+// The counts go to the host while the GPU already writes the tree: the write
+// reads the node count from device memory and stores no node past the
+// arrays' capacity, so the host only has to look at the counts before it
+// sizes the next launches (invalid codes, a tree over the capacity: error,
+// after the stream has drained in the caller).
 int stage_build(ps_hip_engine* e) {
     uint32_t* small = e->small_u32();
     PS_CK(ps_tree_count(e->code, nullptr, e->n, small, small + 12, e->totals(), e->tree_tmp.p,
                         e->leaf_cap, e->s));
     PS_CK(hipMemcpyAsync(e->h_small, e->small_b.p, 96, hipMemcpyDeviceToHost, e->s));
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(hipEventRecord(e->ev_counts, e->s));
+    PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
+                        e->link(), e->nend(), e->parent(), e->leaf_of(), e->node_cap,
+                        e->totals(), e->leaf_cap, e->s));
+    ps_fill_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 0u, e->totals(), 4, e->node_cap,
+                      e->s);
+    ps_fill_u32_count(e->p2p_len(), 0u, e->totals(), 1, e->node_cap, e->s);
+    PS_CK(hipEventSynchronize(e->ev_counts));
     const uint32_t flags = e->h_small[12];
     const unsigned long long total =
         *reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(e->h_small) + 64);
@@ -319,11 +341,6 @@ int stage_build(ps_hip_engine* e) {
     e->max_depth = 0;
     for (int d = 0; d <= 10; ++d)
         if (e->ls[d + 1] > e->ls[d]) e->max_depth = (uint32_t)d;
-    PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
-                        e->link(), e->nend(), e->parent(), e->leaf_of(), e->n_nodes, nullptr,
-                        e->leaf_cap, e->s));
-    ps_fill_u32(reinterpret_cast<uint32_t*>(e->loc()), 0u, 4 * (size_t)e->n_nodes, e->s);
-    ps_fill_u32(e->p2p_len(), 0u, e->n_nodes, e->s);
     e->built = true;
     return PS_HIP_OK;
 }
@@ -378,12 +395,15 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     return map_err(hipGetLastError());
 }
 
+float theta_squared(const ps_hip_engine* e) {
+    volatile float t = e->theta;  // fl(theta * theta), H:1064
+    return t * t;
+}
+
+// Launches the traversal; its state (overflow flag, counts) is copied to the
+// host behind it and looked at by trav_finish once the stream has drained.
 int stage_traverse(ps_hip_engine* e) {
-    float theta2;
-    {
-        volatile float t = e->theta;  // fl(theta * theta), H:1064
-        theta2 = t * t;
-    }
+    const float theta2 = theta_squared(e);
     const bool mt = e->trav == PS_HIP_TRAV_MT;
     const int iters = (int)e->max_depth + 1;
     const size_t M = e->n_nodes;
@@ -396,24 +416,32 @@ int stage_traverse(ps_hip_engine* e) {
     const size_t n0 = std::max<size_t>(e->ls[3], 1) + 16;
     PS_CK(grow(e->arena_b, 4 * std::max<size_t>(96 * e->n + 4096, 1 << 16), e->s));
     PS_CK(grow(e->masks_b, 8 * std::max<size_t>(4 * M + n0 + 4096, n0), e->s));
+    PS_RC(launch_traverse(e, theta2, mt, iters));
+    PS_CK(hipMemcpyAsync(e->h_tstate, e->tstate_b.p, sizeof(PsTravState), hipMemcpyDeviceToHost,
+                         e->s));
+    return PS_HIP_OK;
+}
 
-    PsTravState hst;
-    for (int attempt = 0;; ++attempt) {
-        PS_RC(launch_traverse(e, theta2, mt, iters));
-        PS_CK(hipMemcpyAsync(&hst, e->tstate_b.p, sizeof(hst), hipMemcpyDeviceToHost, e->s));
-        PS_CK(hipStreamSynchronize(e->s));
-        if (!hst.overflow) break;
+// After the stream has drained: the traversal's outcome.  *redo: an arena was
+// too small -- it has been grown and the partial results cleared, and the
+// tick has to run again from the traversal (the sort and the tree stand).
+int trav_finish(ps_hip_engine* e, int attempt, bool* redo) {
+    const PsTravState hst = *e->h_tstate;
+    *redo = false;
+    if (hst.overflow) {
         if (attempt >= 4 || hst.arena_ctr > 0xFFFFFFFFull || hst.mask_ctr > 0xFFFFFFFFull)
             return PS_HIP_EOOM;
-        // grow past what was asked for, clear the partial results, run again
+        // grow past what was asked for, clear the partial results
         PS_CK(grow(e->arena_b, std::max<size_t>(2 * e->arena_b.bytes, 5 * hst.arena_ctr), e->s));
         PS_CK(grow(e->masks_b, std::max<size_t>(2 * e->masks_b.bytes, 10 * hst.mask_ctr), e->s));
         ps_fill_u32(reinterpret_cast<uint32_t*>(e->loc()), 0u, 4 * (size_t)e->n_nodes, e->s);
         ps_fill_u32(e->p2p_len(), 0u, e->n_nodes, e->s);
+        *redo = true;
+        return map_err(hipGetLastError());
     }
     e->st.m2l_pairs = hst.m2l_pairs;
     e->st.list_entries = hst.arena_ctr - 1;
-    e->st.iterations = iters;
+    e->st.iterations = e->max_depth + 1;
     if (e->stats_on) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, e->ev_it[0], e->ev_it[1]) == hipSuccess)
@@ -448,13 +476,8 @@ int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
     return map_err(hipGetLastError());
 }
 
-int run_tick(ps_hip_engine* e) {
-    e->record(1);
-    PS_RC(stage_sort(e));
-    e->record(2);
-    PS_RC(stage_build(e));
-    e->record(3);
-    PS_RC(stage_upward(e));
+// the tick from the traversal on (also its re-run after an arena overflow)
+int run_from_traverse(ps_hip_engine* e) {
     e->record(4);
     PS_RC(stage_traverse(e));
     e->record(5);
@@ -469,6 +492,18 @@ int run_tick(ps_hip_engine* e) {
     }
     e->record(8);
     return PS_HIP_OK;
+}
+
+// Launches the whole tick; one host wait inside (the tree's counts, while
+// the GPU writes the tree).
+int run_tick(ps_hip_engine* e) {
+    e->record(1);
+    PS_RC(stage_sort(e));
+    e->record(2);
+    PS_RC(stage_build(e));
+    e->record(3);
+    PS_RC(stage_upward(e));
+    return run_from_traverse(e);
 }
 
 }  // namespace
@@ -1026,20 +1061,26 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
     PS_RC(bind_all(e, x, y, z, mass, fx, fy, fz, id, n, codes, true));
     e->bound = true;
     int rc = run_tick(e);
-    if (rc != PS_HIP_OK) {
-        (void)hipStreamSynchronize(e->s);
-        return rc;
-    }
-    // copy staged arrays back: the sorted payload and the forces
     void* host[9] = {x, y, z, mass, fx, fy, fz, id, codes};
     void* dev[9] = {e->x, e->y, e->z, e->m, e->fx, e->fy, e->fz, e->id, e->code};
-    for (int k = 0; k < 9; ++k)
-        if (host[k] != dev[k])
-            PS_CK(hipMemcpyAsync(host[k], dev[k], n * 4, hipMemcpyDeviceToHost, e->s));
-    if (e->count_on)
-        PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 16, hipMemcpyDeviceToHost, e->s));
-    if (e->stats_on) PS_CK(hipEventRecord(e->ev[9], e->s));
-    PS_CK(hipStreamSynchronize(e->s));
+    for (int attempt = 0;; ++attempt) {
+        if (rc != PS_HIP_OK) {
+            (void)hipStreamSynchronize(e->s);
+            return rc;
+        }
+        // copy staged arrays back: the sorted payload and the forces
+        for (int k = 0; k < 9; ++k)
+            if (host[k] != dev[k])
+                PS_CK(hipMemcpyAsync(host[k], dev[k], n * 4, hipMemcpyDeviceToHost, e->s));
+        if (e->count_on)
+            PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 16, hipMemcpyDeviceToHost, e->s));
+        if (e->stats_on) PS_CK(hipEventRecord(e->ev[9], e->s));
+        PS_CK(hipStreamSynchronize(e->s));
+        bool redo = false;
+        PS_RC(trav_finish(e, attempt, &redo));
+        if (!redo) break;
+        rc = run_from_traverse(e);
+    }
     e->st.n_particles = n;
     e->st.n_nodes = e->n_nodes;
     e->st.max_depth = e->max_depth;
@@ -1105,7 +1146,16 @@ int ps_hip_debug_stage(ps_hip_engine_t* e, int stage) {
         case kSort: rc = stage_sort(e); break;
         case kBuild: rc = stage_build(e); break;
         case kUpward: rc = stage_upward(e); break;
-        case kInteract: rc = stage_traverse(e); break;
+        case kInteract:
+            for (int attempt = 0;; ++attempt) {
+                rc = stage_traverse(e);
+                if (rc != PS_HIP_OK) break;
+                PS_CK(hipStreamSynchronize(e->s));
+                bool redo = false;
+                rc = trav_finish(e, attempt, &redo);
+                if (rc != PS_HIP_OK || !redo) break;
+            }
+            break;
         case kDownward:
             rc = stage_l2l(e);
             if (rc == PS_HIP_OK) rc = stage_p2p(e, 1);

@@ -81,7 +81,7 @@ hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
 // Phase 2: writes every node (needs phase 1's tmp untouched).  The root cell
 // is `root`, or roots[sys[i]] per system when roots is set.
 // n_nodes_dev (optional): the node count in device memory (then n_nodes is
-// only the capacity).
+// only the capacity).  Nodes at or past n_nodes are not written.
 hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, float4 root,
                          const float4* roots, void* tmp, float4* geom, uint2* link,
                          uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, size_t n_nodes,

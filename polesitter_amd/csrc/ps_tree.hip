@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
     const uint32_t* __restrict__ off, uint32_t nb, float4 root,
     const float4* __restrict__ roots, float4* __restrict__ geom, uint2* __restrict__ link,
     uint32_t* __restrict__ nend, uint32_t* __restrict__ parent, uint32_t* __restrict__ leaf_of,
-    int leaf_cap) {
+    int leaf_cap, uint32_t cap) {
 #pragma clang fp contract(off)
     __shared__ uint32_t ws[kWaves][kDepths];
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -157,7 +157,9 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
             g.z = g.z + ((oct & 4u) ? hw : -hw);
             g.w = hw;
         }
-        if (d >= pd.start_lo) {  // i opens this node
+        // nodes past the arrays' capacity are not written (a tree that big
+        // is reported by the caller from the node count)
+        if (d >= pd.start_lo && node[d] < cap) {  // i opens this node
             const uint32_t nd = node[d];
             geom[nd] = g;
             parent[nd] = d > 0 ? node[d > 0 ? d - 1 : 0] : PS_NONE;
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
             }
             if (d > 0) atomicOr(&link32[2 * (size_t)node[d - 1] + 1], 1u << oct);
         }
-        if (d >= pd.end_lo) nend[node[d]] = (uint32_t)i + 1u;
+        if (d >= pd.end_lo && node[d] < cap) nend[node[d]] = (uint32_t)i + 1u;
     }
     uint32_t lf = node[0];
 #pragma unroll
@@ -342,8 +344,9 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
         ps_fill_u32_count(reinterpret_cast<uint32_t*>(link), 0u, n_nodes_dev, 2, n_nodes, s);
     else
         ps_fill_u32(reinterpret_cast<uint32_t*>(link), 0u, 2 * n_nodes, s);
+    const uint32_t cap = n_nodes < 0xFFFFFFFFull ? (uint32_t)n_nodes : 0xFFFFFFFFu;
     tree_write_kernel<<<nb, kBlock, 0, s>>>(codes, sys, n, off, nb, root, roots, geom, link,
-                                            nend, parent, leaf_of, leaf_cap);
+                                            nend, parent, leaf_of, leaf_cap, cap);
     return hipGetLastError();
 }
 

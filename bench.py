@@ -252,8 +252,12 @@ def bench_ensemble(a):
     n = sum(len(t[0]) for t in systems)
     mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
     log(f"bench: {api.device_name()} workload ensemble512: 512 systems, {n} particles")
-    tuned = autotune.tune_batch(systems, theta=a.theta, mode=mmode, log=log)
-    cfg = tuned["config"]
+    if a.no_tune:  # the reference setting, for profiling runs
+        cfg = dict(leaf_cap=1, p2p_big=24, trav_lanes=8)
+        tuned = dict(config=cfg, reference=None, ms_tick=None, fp64_rms_err=None, table=[])
+    else:
+        tuned = autotune.tune_batch(systems, theta=a.theta, mode=mmode, log=log)
+        cfg = tuned["config"]
     L = api.hip_lib()
     with api.Batch(n, len(systems), theta=a.theta, graph=True, kahan=a.kahan) as b:
         b.set_option(api.OPT_LEAF_CAP, cfg["leaf_cap"])
@@ -288,7 +292,8 @@ def bench_ensemble(a):
                 autotune=dict(reference=tuned["reference"], chosen_ms=tuned["ms_tick"],
                               chosen_fp64_rms_err=tuned["fp64_rms_err"],
                               candidates=[dict(c, ms=round(ms_, 3), err=round(e_, 6))
-                                          for c, ms_, e_ in tuned["table"]]))
+                                          for c, ms_, e_ in tuned["table"]],
+                              skipped=bool(a.no_tune)))
     print(json.dumps(line), flush=True)
     return 0
 
@@ -316,6 +321,8 @@ def main(argv=None):
     ap.add_argument("--mlp-theta", type=float, default=1.9,
                     help="mlp4m: theta of the corrected coarse setting (0: --theta)")
     ap.add_argument("--mlp-ticks", type=int, default=200, help="mlp4m: training ticks")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="ensemble512: skip the autotuner (reference setting)")
     ap.add_argument("--host-ticks", type=int, default=3,
                     help="ticks timed with host (pageable numpy) arrays, PCIe included; 0 = skip")
     a = ap.parse_args(argv)

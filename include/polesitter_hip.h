@@ -50,6 +50,8 @@ extern "C" {
 #define PS_HIP_OPT_TRAV_LANES 5 /* lanes per traversal entry: 8 (default), 16, 32, 64 */
 #define PS_HIP_OPT_P2P_BIG   6 /* P2P wavefront-per-leaf threshold (particles; 0 = off; 24) */
 #define PS_HIP_OPT_LEAF_CAP  7 /* most particles of a leaf above depth 10: 1 (the reference) .. 64 */
+#define PS_HIP_OPT_ARENA_WORDS 8 /* traversal list arena at first use, words per particle
+                                    (96; grown and the tick re-run when too small) */
 
 typedef struct ps_hip_engine ps_hip_engine_t;
 

@@ -143,6 +143,7 @@ struct ps_hip_engine {
     int trav_lanes = 8;  // lanes per traversal entry
     uint32_t p2p_big = 24;  // leaves of >= this many particles: wavefront per leaf
     int leaf_cap = 1;       // most particles of a leaf above depth 10 (the reference: 1)
+    uint32_t arena_words = 96;  // list arena at first use, words per particle
     DevBuf mlp_b;           // learned correction parameters (empty: none)
     DevBuf mlp_pk_b;        // the same as the MFMA kernel's packed operands
     bool mlp_on = false;
@@ -411,10 +412,10 @@ int stage_traverse(ps_hip_engine* e) {
         for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[b][k], 4 * M, e->s));
     PS_CK(grow(e->far_b, 8 * (size_t)ps_trav_blocks(0xFFFFFFFFu, 8) * (size_t)iters, e->s));
     PS_CK(grow(e->tstate_b, sizeof(PsTravState), e->s));
-    // first use: room for ~96 list words per particle; a larger need is met
-    // by growing and running the traversal again
+    // first use: room for arena_words list words per particle (96); a larger
+    // need is met by growing and running the traversal again
     const size_t n0 = std::max<size_t>(e->ls[3], 1) + 16;
-    PS_CK(grow(e->arena_b, 4 * std::max<size_t>(96 * e->n + 4096, 1 << 16), e->s));
+    PS_CK(grow(e->arena_b, 4 * ((size_t)e->arena_words * e->n + 4096), e->s));
     PS_CK(grow(e->masks_b, 8 * std::max<size_t>(4 * M + n0 + 4096, n0), e->s));
     PS_RC(launch_traverse(e, theta2, mt, iters));
     PS_CK(hipMemcpyAsync(e->h_tstate, e->tstate_b.p, sizeof(PsTravState), hipMemcpyDeviceToHost,
@@ -562,7 +563,7 @@ int batch_ensure(ps_hip_batch* b, size_t n, size_t cap) {
         for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[g][k], 4 * cap, s));
     PS_CK(grow(e->far_b, 8 * (size_t)ps_trav_blocks(0xFFFFFFFFu, 8) * PS_MAX_DEPTH_ITERS, s));
     PS_CK(grow(e->tstate_b, sizeof(PsTravState), s));
-    PS_CK(grow(e->arena_b, 4 * std::max<size_t>(96 * n + 4096, 1 << 16), s));
+    PS_CK(grow(e->arena_b, 4 * ((size_t)e->arena_words * n + 4096), s));
     PS_CK(grow(e->masks_b, 8 * (4 * cap + 4096), s));
     PS_CK(grow(e->big_b, 4 * (cap + 1), s));
     PS_CK(grow(b->sys_b, 4 * std::max<size_t>(n, 1), s));
@@ -748,6 +749,11 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
         case PS_HIP_OPT_P2P_BIG:
             if (!(value >= 0.0 && value <= 4294967295.0)) return PS_HIP_EINVAL;
             e->p2p_big = (uint32_t)value;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_ARENA_WORDS:
+            if (!(value >= 1.0 && value <= 65536.0) || value != (double)(int)value)
+                return PS_HIP_EINVAL;
+            e->arena_words = (uint32_t)value;
             return PS_HIP_OK;
         default:
             return PS_HIP_EINVAL;

@@ -22,7 +22,8 @@ def log(msg):
 
 
 def gpu_tick(x, y, z, m, theta=2.0, trav=api.TRAV_MT, mode=api.MORTON_TRUNC, host=False,
-             kahan=False, ids=None, tree=True, lanes=None, p2p_big=None, leaf_cap=None):
+             kahan=False, ids=None, tree=True, lanes=None, p2p_big=None, leaf_cap=None,
+             arena_words=None):
     """ps_prepare_particles + ps_calc_forces on the device, root = min +
     range/2 (tests/accuracy.c:493-503).  host=True passes numpy (host)
     arrays, otherwise device arrays."""
@@ -37,6 +38,9 @@ def gpu_tick(x, y, z, m, theta=2.0, trav=api.TRAV_MT, mode=api.MORTON_TRUNC, hos
             api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_P2P_BIG, p2p_big))
         if leaf_cap is not None:
             api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_LEAF_CAP, leaf_cap))
+        if arena_words is not None:
+            api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_ARENA_WORDS,
+                                                         arena_words))
         if host:
             fx, fy, fz = (np.zeros(n, f32) for _ in range(3))
             codes = np.zeros(n, np.uint32)

@@ -115,6 +115,18 @@ def test_other_theta(oracle, theta):
     assert_tick_parity(g, o)
 
 
+@pytest.mark.parametrize("trav", [api.TRAV_MT, api.TRAV_ST])
+def test_arena_overflow_reruns_the_traversal(oracle, trav):
+    # a list arena of one word per particle is far too small: the traversal
+    # overflows, the arenas grow and the tick runs again from the traversal
+    # (once the stream has drained) -- with exactly the oracle's result
+    x, y, z, m = W.plummer(30_000, seed=45)
+    o = oracle.tick(x, y, z, m, theta=2.0, trav=trav)
+    g = gpu_tick(x, y, z, m, theta=2.0, trav=trav, arena_words=1)
+    assert g["stats"]["list_entries"] > 30_000 + 4096  # more than the first arena held
+    assert_tick_parity(g, o)
+
+
 def test_avx2_morton_mode(oracle):
     x, y, z, m = W.plummer(50_003, seed=42)
     o = oracle.tick(x, y, z, m, theta=2.0, mode=oracle.MORTON_AVX2)

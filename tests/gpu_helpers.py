@@ -10,8 +10,9 @@ import numpy as np
 
 from polesitter_amd import api
 
-# Stated tolerances (north star: forces <= 1e-5 RMS relative to the reference,
-# order-of-summation differences only).
+# Stated tolerances: forces and locals within 1e-5 RMS relative to the oracle
+# (the GPU sums M2L and P2P terms in another order than the reference's
+# depth-first recursion; everything discrete is compared bit for bit).
 FORCE_RMS_TOL = 1e-5
 LOCAL_RMS_TOL = 1e-5
 

@@ -44,6 +44,7 @@ constexpr int kBlock = 256;            // init kernel
 constexpr int kStep = 1024;            // threads of an iteration block
 constexpr uint32_t kMaxBlocks = 512;   // grid cap of an iteration (2 blocks per CU)
 constexpr uint32_t kSeg = 2048;        // entries per LDS segment (one reservation each)
+constexpr uint32_t kBins = 4;          // list-length classes of the segment order
 
 // entries per block range: a multiple of the groups per block, the ranges
 // of nblk blocks covering n_e
@@ -398,6 +399,8 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
     __shared__ uint32_t sbase[4];  // list, entry and mask bases; 1 = reserved
     __shared__ uint32_t queue[2];  // work queues of the count and write phases
     __shared__ unsigned long long sfar[kGroups];
+    __shared__ uint32_t sbin[kBins][kSeg / 64];  // per 64 entries: counts, then offsets
+    __shared__ uint16_t sord[kSeg];              // the segment's entries, longest lists first
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t grp = tid / G;
     const uint32_t gl = tid & (G - 1);
@@ -424,11 +427,53 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
     for (uint32_t seg = b0; seg < b1; seg += kSeg) {  // uniform per block
         const uint32_t segn = b1 - seg < kSeg ? b1 - seg : kSeg;
         if (tid == 0) queue[1] = 0u;  // the write phase of the last segment is over
+        // order: a group's work is the chunk count of its entry's list, and a
+        // wavefront runs as many chunk rounds as the longest list among its
+        // groups, so the queues hand out the entries by class of chunk count
+        // (3+, 2, 1, 0), each class in entry order.  Which group takes an
+        // entry changes nothing in what the entry computes.
+        uint32_t cls[kSeg / kStep], rk[kSeg / kStep];
+#pragma unroll
+        for (uint32_t r = 0; r < kSeg / kStep; ++r) {
+            const uint32_t j = r * kStep + tid;
+            uint32_t c = kBins;  // past the segment: no class
+            if (j < segn) {
+                const uint32_t ch = (ent.len[seg + j] + 63u) >> 6;
+                c = ch < kBins - 1 ? kBins - 1 - ch : 0u;
+            }
+            cls[r] = c;
+            rk[r] = 0u;
+#pragma unroll
+            for (uint32_t b = 0; b < kBins; ++b) {
+                const uint64_t m = __ballot(c == b);
+                if (c == b) rk[r] = (uint32_t)__popcll(m & ps_lanemask_lt());
+                if (lane == 0) sbin[b][j >> 6] = (uint32_t)__popcll(m);
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {  // exclusive prefix over (class, 64-entry block), class major
+            constexpr uint32_t kW = kSeg / 64;
+            static_assert(kBins * kW == 128, "two counts per lane");
+            const uint32_t a0 = sbin[(2 * lane) / kW][(2 * lane) % kW];
+            const uint32_t a1 = sbin[(2 * lane + 1) / kW][(2 * lane + 1) % kW];
+            uint32_t tot;
+            const uint32_t ex = ps_wave_excl_scan(a0 + a1, &tot);
+            sbin[(2 * lane) / kW][(2 * lane) % kW] = ex;
+            sbin[(2 * lane + 1) / kW][(2 * lane + 1) % kW] = ex + a0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t r = 0; r < kSeg / kStep; ++r) {
+            const uint32_t j = r * kStep + tid;
+            if (cls[r] < kBins) sord[sbin[cls[r]][j >> 6] + rk[r]] = (uint16_t)j;
+        }
+        __syncthreads();
         // count
         for (uint32_t i = group_take<G>(&queue[0], gl); i < segn; i = group_take<G>(&queue[0], gl)) {
-            const uint4 c = count_entry<kKahan, G>(t, theta2, ent, arena, masks, mask_cap, seg + i,
+            const uint32_t k = sord[i];
+            const uint4 c = count_entry<kKahan, G>(t, theta2, ent, arena, masks, mask_cap, seg + k,
                                                    gl, gshift);
-            if (gl == 0) scnt[i] = c;
+            if (gl == 0) scnt[k] = c;
             far_acc += c.w;
         }
         __syncthreads();
@@ -495,9 +540,10 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
             const uint32_t la = sbase[0], le = sbase[1], lm = sbase[2];
             for (uint32_t i = group_take<G>(&queue[1], gl); i < segn;
                  i = group_take<G>(&queue[1], gl)) {
-                const uint4 q = scnt[i];
+                const uint32_t k = sord[i];
+                const uint4 q = scnt[k];
                 if (q.w == 0) continue;  // final or empty: nothing to pass on
-                write_entry<G>(t, ent, arena, masks, seg + i, q.w,
+                write_entry<G>(t, ent, arena, masks, seg + k, q.w,
                                make_uint4(la + q.x, le + q.y, lm + q.z, 0u), next, gl);
             }
         }

@@ -263,6 +263,10 @@ def bench_ensemble(a):
         b.set_option(api.OPT_LEAF_CAP, cfg["leaf_cap"])
         b.set_option(api.OPT_P2P_BIG, cfg["p2p_big"])
         b.set_option(api.OPT_TRAV_LANES, cfg["trav_lanes"])
+        if a.trav_blocks:
+            b.set_option(api.OPT_TRAV_BLOCKS, a.trav_blocks)
+        if a.trav_segments:
+            b.set_option(api.OPT_TRAV_SEGMENTS, a.trav_segments)
         b.load(systems)
         a.warmup = max(a.warmup, 1)
         for _ in range(a.warmup):  # the first one records the graph
@@ -307,6 +311,10 @@ def main(argv=None):
     ap.add_argument("--theta", type=float, default=2.0)
     ap.add_argument("--kahan", action="store_true")
     ap.add_argument("--trav-lanes", type=int, default=8, choices=(8, 16, 32, 64))
+    ap.add_argument("--trav-blocks", type=int, default=0,
+                    help="workgroups of one traversal iteration (a multiple of 8; 0: one per CU)")
+    ap.add_argument("--trav-segments", type=int, default=0,
+                    help="traversal work pieces per workgroup and iteration (0: the default)")
     ap.add_argument("--morton", default="avx2", choices=("avx2", "trunc"),
                     help="quantization of ps_prepare_particles: the reference's AVX2 path "
                          "(round to nearest, the shipped build the CPU baseline runs) or its "
@@ -343,6 +351,11 @@ def main(argv=None):
     s = api.Solver(n, theta=a.theta, kahan=a.kahan, stats=True)
     api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_LANES, a.trav_lanes))
     api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_P2P_BIG, a.p2p_big))
+    if a.trav_blocks:
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_BLOCKS, a.trav_blocks))
+    if a.trav_segments:
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_SEGMENTS,
+                                                     a.trav_segments))
     s.load(x, y, z, m)
     vel = [api.DeviceArray.from_numpy(vv) for vv in v]
     tmp = api.DeviceArray(3 * n, np.float32)

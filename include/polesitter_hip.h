@@ -52,6 +52,11 @@ extern "C" {
 #define PS_HIP_OPT_LEAF_CAP  7 /* most particles of a leaf above depth 10: 1 (the reference) .. 64 */
 #define PS_HIP_OPT_ARENA_WORDS 8 /* traversal list arena at first use, words per particle
                                     (96; grown and the tick re-run when too small) */
+#define PS_HIP_OPT_TRAV_BLOCKS 9 /* workgroups of one traversal iteration: a multiple of 8
+                                    from 8 to 8192 (the device's CU count) */
+#define PS_HIP_OPT_TRAV_SEGMENTS 10 /* traversal work pieces per workgroup and iteration,
+                                       1 to 64 (2): more balance the load, each costs a few
+                                       block-wide synchronisations */
 
 typedef struct ps_hip_engine ps_hip_engine_t;
 

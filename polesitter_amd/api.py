@@ -32,6 +32,8 @@ MORTON_TRUNC, MORTON_AVX2 = 0, 1
 OPT_TRAVERSAL, OPT_KAHAN, OPT_THETA, OPT_STATS, OPT_TRAV_LANES, OPT_P2P_BIG, OPT_LEAF_CAP = \
     1, 2, 3, 4, 5, 6, 7
 OPT_ARENA_WORDS = 8
+OPT_TRAV_BLOCKS = 9
+OPT_TRAV_SEGMENTS = 10
 BATCH_OPT_GRAPH = 100
 
 

@@ -141,6 +141,8 @@ struct ps_hip_engine {
     int stats_on = 0;   // stage events
     int count_on = 0;   // pair counts (extra reductions)
     int trav_lanes = 8;  // lanes per traversal entry
+    uint32_t trav_blocks = 256;  // most workgroups of a traversal iteration (one per CU)
+    uint32_t trav_spb = 2;       // traversal segments per workgroup
     uint32_t p2p_big = 24;  // leaves of >= this many particles: wavefront per leaf
     int leaf_cap = 1;       // most particles of a leaf above depth 10 (the reference: 1)
     uint32_t arena_words = 96;  // list arena at first use, words per particle
@@ -388,8 +390,9 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
         const int cur = k & 1, nxt = cur ^ 1;
         const uint32_t ub = std::min(entries_bound(e, mt, k), n_nodes);
         ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
-                            e->entries(nxt), far_part + far_n, e->kahan, e->trav_lanes, e->s);
-        far_n += ps_trav_blocks(ub, e->trav_lanes);
+                            e->entries(nxt), far_part + far_n, e->kahan, e->trav_lanes,
+                            e->trav_blocks, e->trav_spb, e->s);
+        far_n += ps_trav_blocks(ub, e->trav_lanes, e->trav_blocks);
     }
     if (e->stats_on) (void)hipEventRecord(e->ev_it[1], e->s);
     ps_reduce_u64(far_part, far_n, &st->m2l_pairs, e->s);
@@ -410,7 +413,7 @@ int stage_traverse(ps_hip_engine* e) {
     const size_t M = e->n_nodes;
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[b][k], 4 * M, e->s));
-    PS_CK(grow(e->far_b, 8 * (size_t)ps_trav_blocks(0xFFFFFFFFu, 8) * (size_t)iters, e->s));
+    PS_CK(grow(e->far_b, 8 * (size_t)e->trav_blocks * (size_t)iters, e->s));
     PS_CK(grow(e->tstate_b, sizeof(PsTravState), e->s));
     // first use: room for arena_words list words per particle (96); a larger
     // need is met by growing and running the traversal again
@@ -561,7 +564,7 @@ int batch_ensure(ps_hip_batch* b, size_t n, size_t cap) {
     for (DevBuf* d : node4) PS_CK(grow(*d, 4 * cap, s));
     for (int g = 0; g < 2; ++g)
         for (int k = 0; k < 4; ++k) PS_CK(grow(e->ent[g][k], 4 * cap, s));
-    PS_CK(grow(e->far_b, 8 * (size_t)ps_trav_blocks(0xFFFFFFFFu, 8) * PS_MAX_DEPTH_ITERS, s));
+    PS_CK(grow(e->far_b, 8 * (size_t)e->trav_blocks * PS_MAX_DEPTH_ITERS, s));
     PS_CK(grow(e->tstate_b, sizeof(PsTravState), s));
     PS_CK(grow(e->arena_b, 4 * ((size_t)e->arena_words * n + 4096), s));
     PS_CK(grow(e->masks_b, 8 * (4 * cap + 4096), s));
@@ -628,8 +631,9 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
     for (int k = 0; k < PS_MAX_DEPTH_ITERS; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
         ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
-                            e->entries(nxt), far_part + far_n, e->kahan, e->trav_lanes, s);
-        far_n += ps_trav_blocks(ub, e->trav_lanes);
+                            e->entries(nxt), far_part + far_n, e->kahan, e->trav_lanes,
+                            e->trav_blocks, e->trav_spb, s);
+        far_n += ps_trav_blocks(ub, e->trav_lanes, e->trav_blocks);
     }
     ps_reduce_u64(far_part, far_n, &st->m2l_pairs, s);
     ps_launch_l2l_dev(ls, e->parent(), e->loc(), s);
@@ -695,6 +699,10 @@ int ps_hip_engine_create(ps_hip_engine_t** out, size_t max_particles, size_t max
     e->theta = theta > 0.0f ? theta : 1.0f;
     e->trav = traversal == PS_HIP_TRAV_ST ? PS_HIP_TRAV_ST : PS_HIP_TRAV_MT;
     hipError_t r = hipGetDevice(&e->device);
+    int cus = 0;
+    if (r == hipSuccess)
+        r = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device);
+    if (r == hipSuccess && cus >= 8) e->trav_blocks = (uint32_t)cus & ~7u;
     if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking);
     if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking);
     if (r == hipSuccess) r = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
@@ -749,6 +757,17 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
         case PS_HIP_OPT_P2P_BIG:
             if (!(value >= 0.0 && value <= 4294967295.0)) return PS_HIP_EINVAL;
             e->p2p_big = (uint32_t)value;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_TRAV_BLOCKS:
+            if (!(value >= 8.0 && value <= 8192.0) || value != (double)(int)value ||
+                ((int)value & 7))
+                return PS_HIP_EINVAL;
+            e->trav_blocks = (uint32_t)value;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_TRAV_SEGMENTS:
+            if (!(value >= 1.0 && value <= 64.0) || value != (double)(int)value)
+                return PS_HIP_EINVAL;
+            e->trav_spb = (uint32_t)value;
             return PS_HIP_OK;
         case PS_HIP_OPT_ARENA_WORDS:
             if (!(value >= 1.0 && value <= 65536.0) || value != (double)(int)value)

@@ -128,6 +128,7 @@ struct PsTravState {
     unsigned long long arena_ctr;  // list words reserved (or asked for, at an overflow)
     unsigned long long mask_ctr;   // mask words reserved (or asked for)
     unsigned long long m2l_pairs;  // filled by a reduction of the per-block far counts
+    uint32_t seg_ctr[16][8];       // iteration k: entries handed out of each eighth
 };
 // Iteration-0 entries from the level starts ls[0..11] in device memory.  MT:
 // every node of depth <= 2 is an entry, with the list {its system's root} at
@@ -139,17 +140,18 @@ void ps_launch_trav_init(const uint32_t* ls, int mt, uint32_t n_sys, const uint3
                          PsTravState* st, uint32_t* arena, uint32_t arena_cap, uint32_t mask_cap,
                          uint32_t max_ent, hipStream_t s);
 // Blocks of the iteration launch for max_ent entries (length of its far_part
-// slice; never more than ps_trav_blocks(UINT32_MAX, 8)).
-uint32_t ps_trav_blocks(uint32_t max_ent, int lanes);
+// slice; never more than max_blocks, a multiple of 8).
+uint32_t ps_trav_blocks(uint32_t max_ent, int lanes, uint32_t max_blocks);
 // Iteration k: MAC + M2L for every (entry target, listed source), far masks,
 // P2P lists of final leaves, next lists appended to the arena and next
 // entries (count in st->n_ent[k + 1]); far pairs per block into far_part.
 // lanes: lanes per entry (8, 16, 32 or 64); max_ent bounds the launch (the
-// count itself is st->n_ent[k]).
+// count itself is st->n_ent[k]); spb: segments per block the entries are
+// cut into (more: finer balance, more per-segment overhead).
 void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t* arena,
                          unsigned long long* masks, PsTravState* st, int k, uint32_t max_ent,
                          PsTravEntries next, unsigned long long* far_part, int kahan, int lanes,
-                         hipStream_t s);
+                         uint32_t max_blocks, uint32_t spb, hipStream_t s);
 
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------
 // mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);

@@ -16,9 +16,12 @@
 // and every target owns its accumulations: no atomics on the results.
 //
 // One iteration = one kernel, a group of G lanes (8 to 64; several groups per
-// wavefront) per entry.  The grid is fixed (at most kMaxBlocks blocks); block
-// b owns one contiguous range of the iteration's entries, sized on the device
-// from the entry count, and takes it in LDS segments of kSeg entries:
+// wavefront) per entry.  The grid is fixed (one block per CU by default); the
+// entries are cut into parts, one per XCD (eight, or fewer for a small
+// iteration), and each block takes segments (kStep / G to kSeg entries, sized
+// on the device from the entry count) from its XCD's part -- the first one
+// fixed by the block index, the others through an atomic counter -- then
+// from the other parts once its own is used up.  Per segment, in LDS:
 //   count  MAC for each listed source (the group takes the list in strides),
 //          M2L for far ones, a 64-bit far mask per 64 list entries, the
 //          entry's output sizes into LDS; a final leaf records (list, masks)
@@ -42,16 +45,79 @@ namespace {
 
 constexpr int kBlock = 256;            // init kernel
 constexpr int kStep = 1024;            // threads of an iteration block
-constexpr uint32_t kMaxBlocks = 512;   // grid cap of an iteration (2 blocks per CU)
 constexpr uint32_t kSeg = 2048;        // entries per LDS segment (one reservation each)
 constexpr uint32_t kBins = 4;          // list-length classes of the segment order
 
-// entries per block range: a multiple of the groups per block, the ranges
-// of nblk blocks covering n_e
-__device__ __forceinline__ uint32_t trav_range(uint32_t n_e, uint32_t nblk, uint32_t groups) {
-    uint32_t r = (uint32_t)(((unsigned long long)n_e + nblk - 1) / nblk);
-    r = (r + groups - 1) / groups * groups;
-    return r > groups ? r : groups;
+// Segment length of an iteration of n_e entries over nblk blocks: about spb
+// segments per block, a multiple of the groups per block up to kSeg (small
+// iterations spread over many blocks: one entry per group).
+__device__ __forceinline__ uint32_t trav_seg(uint32_t n_e, uint32_t nblk, uint32_t spb,
+                                             uint32_t groups) {
+    const unsigned long long d = (unsigned long long)spb * nblk;
+    uint32_t s = (uint32_t)(((unsigned long long)n_e + d - 1) / d);
+    s = (s + groups - 1) / groups * groups;
+    return s < groups ? groups : (s > kSeg ? kSeg : s);
+}
+
+// First entry of part x of n_e entries cut into np parts (x = 0..np).
+__device__ __forceinline__ uint32_t part_start(uint32_t n_e, uint32_t np, uint32_t x) {
+    return (uint32_t)((unsigned long long)n_e * x / np);
+}
+
+// Segments past the first nbx of each part are handed out by its counter:
+// segment c of part x starts at part_start(x) + (nbx + c) S.  Called by the
+// lanes of one wavefront.  The block's own part is tried first with an
+// atomic add alone (own_done once it gave nothing); then the lanes read the
+// np counters at once and take a segment from the first part, in order from
+// the block's own, that has one left (when another block took the last one
+// first, that part reads as used up next time round, so the loop ends).
+// Every lane gets the first entry (PS_NONE: no work left) and the length.
+__device__ uint32_t trav_grab(PsTravState* st, int it, uint32_t n_e, uint32_t np, uint32_t S,
+                              uint32_t nbx, uint32_t xcd, uint32_t lane, bool& own_done,
+                              uint32_t& len) {
+    const uint32_t x = (xcd + lane) % np;
+    const uint32_t lo = part_start(n_e, np, x), hi = part_start(n_e, np, x + 1u);
+    const unsigned long long first = (unsigned long long)lo + (unsigned long long)nbx * S;
+    uint32_t got = PS_NONE, n = 0u;
+    if (!own_done) {
+        if (lane == 0 && first < hi) {
+            const unsigned long long start = first + atomicAdd(&st->seg_ctr[it][x], S);
+            if (start < hi) {
+                got = (uint32_t)start;
+                n = hi - got < S ? hi - got : S;
+            }
+        }
+        got = __shfl(got, 0, 64);
+        if (got != PS_NONE) {
+            len = __shfl(n, 0, 64);
+            return got;
+        }
+        own_done = true;
+    }
+    for (;;) {
+        bool left = false;
+        if (lane < np)
+            left = first + __atomic_load_n(&st->seg_ctr[it][x], __ATOMIC_RELAXED) < hi;
+        const uint64_t m = __ballot(left);
+        if (m == 0) {
+            len = 0u;
+            return PS_NONE;
+        }
+        const uint32_t pick = (uint32_t)__ffsll((long long)m) - 1u;
+        got = PS_NONE;
+        if (lane == pick) {
+            const unsigned long long start = first + atomicAdd(&st->seg_ctr[it][x], S);
+            if (start < hi) {
+                got = (uint32_t)start;
+                n = hi - got < S ? hi - got : S;
+            }
+        }
+        got = __shfl(got, (int)pick, 64);
+        if (got != PS_NONE) {
+            len = __shfl(n, (int)pick, 64);
+            return got;
+        }
+    }
 }
 
 // Iteration-0 entries.  The level starts come from device memory.  MT (the
@@ -392,7 +458,7 @@ template <bool kKahan, int G>
 __global__ __launch_bounds__(kStep) void trav_step_kernel(
     PsTravTree t, float theta2, PsTravEntries ent, uint32_t* __restrict__ arena,
     unsigned long long* __restrict__ masks, PsTravState* __restrict__ st, int it,
-    PsTravEntries next, unsigned long long* __restrict__ far_part) {
+    PsTravEntries next, unsigned long long* __restrict__ far_part, uint32_t spb) {
     constexpr uint32_t kGroups = kStep / G;
     __shared__ uint4 scnt[kSeg];  // sizes of the segment's entries, then their offsets
     __shared__ uint4 wtot[kStep / 64];
@@ -401,38 +467,67 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
     __shared__ unsigned long long sfar[kGroups];
     __shared__ uint32_t sbin[kBins][kSeg / 64];  // per 64 entries: counts, then offsets
     __shared__ uint16_t sord[kSeg];              // the segment's entries, longest lists first
+    __shared__ uint32_t sseg[2];                 // the next segment: first entry, length
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t grp = tid / G;
     const uint32_t gl = tid & (G - 1);
     const uint32_t gshift = lane & ~(uint32_t)(G - 1);
-    // consecutive ranges to the blocks of one XCD (blocks go round-robin over
-    // the 8 XCDs), so neighbouring targets share an L2
+    // Blocks go round-robin over the 8 XCDs: the blocks of XCD x take
+    // segments of the x-th part of the entries first, so neighbouring
+    // targets share an L2, then help with the other parts (the work per
+    // entry varies with the density of the region).  An iteration of few
+    // segments keeps to few XCDs: it is a chain of memory round trips, and
+    // its input was just written through those L2s.
     const uint32_t nblk = gridDim.x;
-    const uint32_t bid = (nblk & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (nblk >> 3) + (blockIdx.x >> 3);
+    const uint32_t xcd = blockIdx.x & 7u;
     // another block may raise the flag meanwhile: the traversal is then redone
     const uint32_t ovf = __atomic_load_n(&st->overflow, __ATOMIC_RELAXED);
     const uint32_t n_e = ovf ? 0u : st->n_ent[it];
     const uint32_t arena_cap = st->arena_cap;
     const uint32_t mask_cap = st->mask_cap;
-    const uint32_t R = trav_range(n_e, nblk, kGroups);
-    const unsigned long long r0 = (unsigned long long)bid * R;
-    const uint32_t b0 = (uint32_t)(r0 < n_e ? r0 : n_e);
-    const uint32_t b1 = (uint32_t)(r0 + R < n_e ? r0 + R : n_e);
+    const uint32_t S = trav_seg(n_e, nblk, spb, kGroups);
+    const uint32_t n_seg = (uint32_t)(((unsigned long long)n_e + S - 1) / S);
+    const uint32_t per_x = nblk >= 8u ? nblk >> 3 : 1u;  // blocks per XCD
+    uint32_t np = (n_seg + per_x - 1) / per_x;
+    np = np < 1u ? 1u : (np > 8u ? 8u : np);
+    if (np > nblk) np = nblk;  // every part has the block of its first segment
+    uint32_t nbx = (n_seg + np - 1) / np;  // segments of a part fixed by the block index
+    nbx = nbx < per_x ? nbx : per_x;
+    bool own_done = false;  // of wave 0: the own part has no segment left
     unsigned long long far_acc = 0;
+    if (wave == 0 && xcd < np && (blockIdx.x >> 3) < nbx) {
+        const uint32_t lo = part_start(n_e, np, xcd), hi = part_start(n_e, np, xcd + 1u);
+        const unsigned long long start =
+            (unsigned long long)lo + (unsigned long long)(blockIdx.x >> 3) * S;
+        uint32_t first = PS_NONE, len = 0u;
+        if (start < hi) {
+            first = (uint32_t)start;
+            len = hi - first < S ? hi - first : S;
+        } else {
+            first = trav_grab(st, it, n_e, np, S, nbx, xcd, lane, own_done, len);
+        }
+        if (lane == 0) {
+            sseg[0] = first;
+            sseg[1] = len;
+        }
+    } else if (tid == 0) {
+        sseg[0] = PS_NONE;
+    }
     if (tid == 0) {
         queue[0] = 0u;
         queue[1] = 0u;
     }
     __syncthreads();
-    for (uint32_t seg = b0; seg < b1; seg += kSeg) {  // uniform per block
-        const uint32_t segn = b1 - seg < kSeg ? b1 - seg : kSeg;
+    for (;;) {
+        const uint32_t seg = sseg[0], segn = sseg[1];
+        if (seg == PS_NONE) break;  // uniform per block
         if (tid == 0) queue[1] = 0u;  // the write phase of the last segment is over
         // order: a group's work is the chunk count of its entry's list, and a
         // wavefront runs as many chunk rounds as the longest list among its
         // groups, so the queues hand out the entries by class of chunk count
         // (3+, 2, 1, 0), each class in entry order.  Which group takes an
         // entry changes nothing in what the entry computes.
-        uint32_t cls[kSeg / kStep], rk[kSeg / kStep];
+        uint32_t crk[kSeg / kStep];  // class << 16 | rank among the class in its 64 entries
 #pragma unroll
         for (uint32_t r = 0; r < kSeg / kStep; ++r) {
             const uint32_t j = r * kStep + tid;
@@ -441,12 +536,11 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
                 const uint32_t ch = (ent.len[seg + j] + 63u) >> 6;
                 c = ch < kBins - 1 ? kBins - 1 - ch : 0u;
             }
-            cls[r] = c;
-            rk[r] = 0u;
+            crk[r] = c << 16;
 #pragma unroll
             for (uint32_t b = 0; b < kBins; ++b) {
                 const uint64_t m = __ballot(c == b);
-                if (c == b) rk[r] = (uint32_t)__popcll(m & ps_lanemask_lt());
+                if (c == b) crk[r] |= (uint32_t)__popcll(m & ps_lanemask_lt());
                 if (lane == 0) sbin[b][j >> 6] = (uint32_t)__popcll(m);
             }
         }
@@ -465,7 +559,8 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
 #pragma unroll
         for (uint32_t r = 0; r < kSeg / kStep; ++r) {
             const uint32_t j = r * kStep + tid;
-            if (cls[r] < kBins) sord[sbin[cls[r]][j >> 6] + rk[r]] = (uint16_t)j;
+            const uint32_t c = crk[r] >> 16;
+            if (c < kBins) sord[sbin[c][j >> 6] + (crk[r] & 0xFFFFu)] = (uint16_t)j;
         }
         __syncthreads();
         // count
@@ -520,9 +615,15 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
                 if (lane == 1) b = atomicAdd(&st->mask_ctr, (unsigned long long)carry.z);
                 if (lane == 2) b = atomicAdd(&st->n_ent[it + 1], carry.y);
             }
+            uint32_t nlen = 0u;  // the next segment, asked for beside the reservation
+            const uint32_t nseg = trav_grab(st, it, n_e, np, S, nbx, xcd, lane, own_done, nlen);
             const unsigned long long ba = __shfl(b, 0, 64);
             const unsigned long long bm = __shfl(b, 1, 64);
             const unsigned long long be = __shfl(b, 2, 64);
+            if (lane == 0) {  // every thread has read sseg (barriers since)
+                sseg[0] = nseg;
+                sseg[1] = nlen;
+            }
             if (lane == 0) {
                 const uint32_t ok =
                     (carry.y && ba + carry.x <= arena_cap && bm + carry.z <= mask_cap) ? 1u : 0u;
@@ -577,20 +678,20 @@ namespace {
 int lane_group(int lanes) { return (lanes == 8 || lanes == 32 || lanes == 64) ? lanes : 16; }
 }  // namespace
 
-uint32_t ps_trav_blocks(uint32_t max_ent, int lanes) {
+uint32_t ps_trav_blocks(uint32_t max_ent, int lanes, uint32_t max_blocks) {
     const uint32_t groups = kStep / lane_group(lanes);
     const uint32_t b = (uint32_t)(((unsigned long long)max_ent + groups - 1) / groups);
-    return b < 1u ? 1u : (b < kMaxBlocks ? b : kMaxBlocks);
+    return b < 1u ? 1u : (b < max_blocks ? b : max_blocks);
 }
 
 void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t* arena,
                          unsigned long long* masks, PsTravState* st, int k, uint32_t max_ent,
                          PsTravEntries next, unsigned long long* far_part, int kahan, int lanes,
-                         hipStream_t s) {
-    const uint32_t nb = ps_trav_blocks(max_ent, lanes);
+                         uint32_t max_blocks, uint32_t spb, hipStream_t s) {
+    const uint32_t nb = ps_trav_blocks(max_ent, lanes, max_blocks);
 #define PS_STEP(KH, G)                                                                    \
     trav_step_kernel<KH, G><<<nb, kStep, 0, s>>>(t, theta2, ent, arena, masks, st, k, next, \
-                                                  far_part)
+                                                  far_part, spb)
     switch (lane_group(lanes) + (kahan ? 1 : 0)) {
         case 8: PS_STEP(false, 8); break;
         case 9: PS_STEP(true, 8); break;

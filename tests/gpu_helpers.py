@@ -24,7 +24,7 @@ def log(msg):
 
 def gpu_tick(x, y, z, m, theta=2.0, trav=api.TRAV_MT, mode=api.MORTON_TRUNC, host=False,
              kahan=False, ids=None, tree=True, lanes=None, p2p_big=None, leaf_cap=None,
-             arena_words=None):
+             arena_words=None, trav_blocks=None, trav_segments=None):
     """ps_prepare_particles + ps_calc_forces on the device, root = min +
     range/2 (tests/accuracy.c:493-503).  host=True passes numpy (host)
     arrays, otherwise device arrays."""
@@ -42,6 +42,12 @@ def gpu_tick(x, y, z, m, theta=2.0, trav=api.TRAV_MT, mode=api.MORTON_TRUNC, hos
         if arena_words is not None:
             api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_ARENA_WORDS,
                                                          arena_words))
+        if trav_blocks is not None:
+            api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_BLOCKS,
+                                                         trav_blocks))
+        if trav_segments is not None:
+            api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_SEGMENTS,
+                                                         trav_segments))
         if host:
             fx, fy, fz = (np.zeros(n, f32) for _ in range(3))
             codes = np.zeros(n, np.uint32)

@@ -94,6 +94,29 @@ def dense_clusters(n, seed):
     return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), m
 
 
+@pytest.mark.parametrize("blocks,segments", [(8, 1), (16, 64), (64, 3), (1024, 2), (256, 16)])
+def test_traversal_work_distribution(oracle, blocks, segments):
+    # any grid and segment size visits the same pairs: 8 workgroups leave
+    # most parts to be taken from other XCDs, 16 segments per workgroup hand
+    # out most of the entries through the counters
+    log(f"traversal blocks={blocks} segments={segments}")
+    x, y, z, m = W.two_plummers(120_000, seed=35)[:4]
+    o = oracle.tick(x, y, z, m, theta=2.0)
+    g = gpu_tick(x, y, z, m, theta=2.0, trav_blocks=blocks, trav_segments=segments)
+    assert_tick_parity(g, o)
+
+
+def test_traversal_work_options_are_checked():
+    with api.Solver(1000, theta=2.0) as s:
+        for key, bad in ((api.OPT_TRAV_BLOCKS, 0), (api.OPT_TRAV_BLOCKS, 12),
+                         (api.OPT_TRAV_BLOCKS, 8200), (api.OPT_TRAV_BLOCKS, 64.5),
+                         (api.OPT_TRAV_SEGMENTS, 0), (api.OPT_TRAV_SEGMENTS, 65),
+                         (api.OPT_TRAV_SEGMENTS, 1.5)):
+            assert api.ps_set_option(s.ctx, key, bad) != api.PS_OK, (key, bad)
+        assert api.ps_set_option(s.ctx, api.OPT_TRAV_BLOCKS, 64) == api.PS_OK
+        assert api.ps_set_option(s.ctx, api.OPT_TRAV_SEGMENTS, 5) == api.PS_OK
+
+
 @pytest.mark.parametrize("p2p_big", [0, 1, 32, 200])
 def test_p2p_big_leaf_kernel(oracle, p2p_big):
     # the wavefront-per-leaf P2P path and the thread-per-particle path give the

@@ -469,7 +469,7 @@ int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
         PS_CK(grow(e->block_pairs_b, 16 * (size_t)ps_p2p_blocks(e->n, bound), e->s));
     if (!e->arena_b.p) PS_CK(grow(e->arena_b, 256, e->s));
     if (!e->masks_b.p) PS_CK(grow(e->masks_b, 256, e->s));
-    PS_CK(grow(e->big_b, 4 * ((size_t)bound + 1), e->s));
+    PS_CK(grow(e->big_b, 4 * ((size_t)bound + 2), e->s));
     if (ev_before >= 0) e->record(ev_before);
     ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
                   e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
@@ -568,7 +568,7 @@ int batch_ensure(ps_hip_batch* b, size_t n, size_t cap) {
     PS_CK(grow(e->tstate_b, sizeof(PsTravState), s));
     PS_CK(grow(e->arena_b, 4 * ((size_t)e->arena_words * n + 4096), s));
     PS_CK(grow(e->masks_b, 8 * (4 * cap + 4096), s));
-    PS_CK(grow(e->big_b, 4 * (cap + 1), s));
+    PS_CK(grow(e->big_b, 4 * (cap + 2), s));
     PS_CK(grow(b->sys_b, 4 * std::max<size_t>(n, 1), s));
     PS_CK(grow(b->aux_b, 8 * std::max<size_t>(e->max_p, 1), s));
     e->node_cap = std::max(e->node_cap, cap);

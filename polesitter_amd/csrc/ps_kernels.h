@@ -159,8 +159,8 @@ void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t
 // its final traversal list.  Leaves of >= tmin particles go to a
 // wavefront-per-leaf kernel, the rest to a thread-per-particle kernel (tmin =
 // 0: all of them); ls = level starts in device memory (ls[11] = node count),
-// bound >= the node count (grid sizes only), big: scratch of 1 + bound
-// words for the list of the big leaves.  With s_big, fork and join set, the
+// bound >= the node count (grid sizes only), big: scratch of 2 + bound
+// words for the list of the big leaves (count, queue counter, leaves).  With s_big, fork and join set, the
 // big-leaf kernel runs on s_big beside the other; all work is ordered after
 // earlier work on s and before later work on s.
 // pair_count (stats, may be null): per-block partial sums in block_pairs

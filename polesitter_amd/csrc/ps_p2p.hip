@@ -336,7 +336,7 @@ __global__ __launch_bounds__(kCollect) void p2p_collect_kernel(const uint32_t* _
             base = t ? atomicAdd(&big[0], t) : 0u;
         }
         __syncthreads();
-        if (take) big[1 + base + wcnt[wave] + (uint32_t)__popcll(b & ps_lanemask_lt())] = node;
+        if (take) big[2 + base + wcnt[wave] + (uint32_t)__popcll(b & ps_lanemask_lt())] = node;
         __syncthreads();  // wcnt and base are rewritten by the next round
     }
 }
@@ -385,10 +385,13 @@ __device__ __forceinline__ void p2p_big_round(uint32_t c0, uint32_t te, uint32_t
 // One wavefront per big leaf (>= tmin particles), 64 targets per
 // round: the whole wave shares the leaf's source list, so the loop runs on
 // scalar control flow and each source particle is one scalar load broadcast
-// to the 64 lanes.  Wave w takes list entries w, w + waves, ...; the grid is
-// large (up to kBigBlocks), so the hardware balances the uneven leaves.
+// to the 64 lanes.  The grid is fixed (kBigBlocks); wave k starts with
+// leaf k of the list and then takes the next one from a counter (big[1])
+// when it is done, so the uneven leaves balance without launching a wave
+// per possible leaf.  The next leaf is asked for when a leaf starts, so the
+// atomic's round trip overlaps the leaf's own loads.
 constexpr int kBigWaves = kBlock / 64;
-constexpr uint32_t kBigBlocks = 65536;
+constexpr uint32_t kBigBlocks = 1024;
 constexpr uint32_t kCollectBlocks = 2048;
 
 template <int kMode, bool kKahan>
@@ -398,14 +401,18 @@ __global__ __launch_bounds__(kBlock) void p2p_big_kernel(
     const uint32_t* __restrict__ p2p_moff, const uint32_t* __restrict__ arena,
     const unsigned long long* __restrict__ masks, const uint2* __restrict__ link,
     const uint32_t* __restrict__ nend, float* __restrict__ fx, float* __restrict__ fy,
-    float* __restrict__ fz, const uint32_t* __restrict__ big, unsigned long long* block_pairs,
+    float* __restrict__ fz, uint32_t* big, unsigned long long* block_pairs,
     uint32_t slot0, uint32_t stride) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t nbig = big[0];
     uint32_t pairs = 0, nsrc = 0;
-    for (uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * kBigWaves + (threadIdx.x >> 6));
-         w < nbig; w += gridDim.x * kBigWaves) {
-        const uint32_t node = big[1 + w];
+    // wave k starts with leaf k; later leaves come from the counter
+    const uint32_t nw = gridDim.x * kBigWaves;
+    uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * kBigWaves + (threadIdx.x >> 6));
+    while (w < nbig) {
+        uint32_t wn = 0;
+        if (lane == 0) wn = nw + atomicAdd(&big[1], 1u);
+        const uint32_t node = big[2 + w];
         const uint32_t t0 = link[node].x;
         const uint32_t te = nend[node] < (uint32_t)n ? nend[node] : (uint32_t)n;
         const uint32_t L = (kMode != 1) ? p2p_len[node] : 0u;
@@ -438,12 +445,13 @@ __global__ __launch_bounds__(kBlock) void p2p_big_kernel(
             if (first && lane == 0) nsrc += ws;
             first = false;
         }
+        w = __builtin_amdgcn_readfirstlane(wn);
     }
     if (block_pairs) p2p_block_stats(pairs, nsrc, block_pairs, slot0 + blockIdx.x, stride);
 }
 
-uint32_t big_blocks(uint32_t n_nodes10) {
-    const uint32_t b = (n_nodes10 + kBigWaves - 1) / kBigWaves;
+uint32_t big_blocks(uint32_t bound) {
+    const uint32_t b = (bound + kBigWaves - 1) / kBigWaves;
     return b < kBigBlocks ? b : kBigBlocks;
 }
 
@@ -452,7 +460,7 @@ void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const fl
                  const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                  const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                  const uint32_t* nend, float* fx, float* fy, float* fz, uint32_t bound,
-                 uint32_t tmin, const uint32_t* big, unsigned long long* bp,
+                 uint32_t tmin, uint32_t* big, unsigned long long* bp,
                  hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join) {
     const uint32_t nb1 = (uint32_t)((n + kBlock - 1) / kBlock);
     const uint32_t nb2 = tmin ? big_blocks(bound) : 0u;
@@ -494,7 +502,7 @@ void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const 
     if (!big || bound == 0) tmin = 0;  // no big-leaf kernel
     unsigned long long* bp = pair_count ? block_pairs : nullptr;
     if (tmin) {
-        ps_fill_u32(big, 0u, 1, s);
+        ps_fill_u32(big, 0u, 2, s);  // count, queue
         uint32_t cb = (bound + kCollect - 1) / kCollect;
         p2p_collect_kernel<<<cb < kCollectBlocks ? cb : kCollectBlocks, kCollect, 0, s>>>(
             ls, tmin, link, nend, big);

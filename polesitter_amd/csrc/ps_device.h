@@ -171,3 +171,102 @@ __device__ __forceinline__ uint32_t ps_child_count(uint2 link) {
 __device__ __forceinline__ uint64_t ps_lanemask_lt() {
     return (__lane_id() == 0) ? 0ull : ((~0ull) >> (64 - __lane_id()));
 }
+
+// ---------------------------------------------------------------------------
+// Lane groups inside a wavefront.
+// ---------------------------------------------------------------------------
+// DPP lane exchanges inside a row of 16 lanes (one VALU op, no LDS round
+// trip): quad_perm [1,0,3,2] and [2,3,0,1] are xor 1 and xor 2; the half-row
+// and row mirrors pair lane i with 7 - i and 15 - i.  In a butterfly sum taken
+// in the order 1, 2, 4 (8) every lane of a 4 (8) lane block holds the same
+// value after the earlier steps, so the mirror partner gives exactly what
+// xor 4 (xor 8) would.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __builtin_bit_cast(float, dpp_u32<CTRL>(__builtin_bit_cast(uint32_t, v)));
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+constexpr int kDppShr1 = 0x111, kDppShr2 = 0x112, kDppShr4 = 0x114, kDppShr8 = 0x118;
+
+// sum over the G lanes of one group (G a power of two <= 64); groups of 8 and
+// 16 with DPP, wider ones with shuffles; the same value in every lane
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+    if (G <= 16) {
+        v += dpp_f32<kDppXor1>(v);
+        v += dpp_f32<kDppXor2>(v);
+        v += dpp_f32<kDppHalfMirror>(v);
+        if (G == 16) v += dpp_f32<kDppMirror>(v);
+        return v;
+    }
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
+    return v;
+}
+
+// inclusive prefix sum over the lanes of one group (gl: lane in the group)
+// and the group's total, valid in every lane
+template <int G>
+__device__ __forceinline__ uint32_t group_scan_u32(uint32_t v, uint32_t gl, uint32_t& total) {
+    uint32_t inc = v;
+    if (G <= 16) {
+        // row shifts cross into the neighbouring group for the first lanes:
+        // those take nothing
+        uint32_t t = dpp_u32<kDppShr1>(inc);
+        inc += gl >= 1u ? t : 0u;
+        t = dpp_u32<kDppShr2>(inc);
+        inc += gl >= 2u ? t : 0u;
+        t = dpp_u32<kDppShr4>(inc);
+        inc += gl >= 4u ? t : 0u;
+        if (G == 16) {
+            t = dpp_u32<kDppShr8>(inc);
+            inc += gl >= 8u ? t : 0u;
+        }
+        uint32_t s = v;
+        s += dpp_u32<kDppXor1>(s);
+        s += dpp_u32<kDppXor2>(s);
+        s += dpp_u32<kDppHalfMirror>(s);
+        if (G == 16) s += dpp_u32<kDppMirror>(s);
+        total = s;
+        return inc;
+    }
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        const uint32_t o = __shfl_up(inc, d, G);
+        if (gl >= (uint32_t)d) inc += o;
+    }
+    total = __shfl(inc, G - 1, G);
+    return inc;
+}
+
+template <int G>
+__device__ __forceinline__ unsigned long long group_bits(bool pred, uint32_t gshift) {
+    const unsigned long long b = __ballot(pred) >> gshift;
+    return G == 64 ? b : (b & ((1ull << G) - 1ull));
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t group_sum_u32(uint32_t v) {
+    if (G <= 16) {
+        v += dpp_u32<kDppXor1>(v);
+        v += dpp_u32<kDppXor2>(v);
+        v += dpp_u32<kDppHalfMirror>(v);
+        if (G == 16) v += dpp_u32<kDppMirror>(v);
+        return v;
+    }
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
+    return v;
+}
+
+// orders the LDS accesses of one wavefront (its lanes run in step: no
+// hardware barrier is needed, only that the compiler keeps the order)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}

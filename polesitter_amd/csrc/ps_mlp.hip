@@ -318,11 +318,6 @@ __device__ __forceinline__ void hidden_operands(const f32x4 acc[4], bf16x8 hh[2]
 // writes before the reads (and the reads before the next round's writes).
 constexpr int kApplyWaves = kApplyBlock / 64;
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __global__ __launch_bounds__(kApplyBlock) void mlp_apply_kernel(
     size_t n, const uint4* __restrict__ pk, const float4* __restrict__ pos4,

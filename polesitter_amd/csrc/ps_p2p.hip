@@ -267,6 +267,9 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
                         r0[u] = u < cnt ? link[src[u]].x : 0u;
                         r1[u] = u < cnt ? nend[src[u]] : 0u;
                     }
+                    // the first particle of each source is loaded while the
+                    // previous source is summed (one load ahead)
+                    float4 qn = cnt ? pos4[r0[0]] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
                     for (int u = 0; u < kB; ++u) {
                         if (u >= cnt) break;
@@ -275,7 +278,20 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
                         const uint32_t j1e = r1[u];
                         const uint32_t j1 = j1e < (uint32_t)n ? j1e : (uint32_t)n;
                         pairs += j1e - j0;
-                        for (uint32_t j = j0; j < j1; ++j) {
+                        const float4 qf = qn;
+                        if (u + 1 < kB && u + 1 < cnt) qn = pos4[r0[u + 1]];
+                        if (j0 < j1) {
+                            if (kKahan) {
+                                float tx = 0.f, ty = 0.f, tz = 0.f;
+                                ps_pair(p, qf, tx, ty, tz);
+                                ps_kahan_add(ax, cx, tx);
+                                ps_kahan_add(ay, cy, ty);
+                                ps_kahan_add(az, cz, tz);
+                            } else {
+                                ps_pair(p, qf, ax, ay, az);
+                            }
+                        }
+                        for (uint32_t j = j0 + 1; j < j1; ++j) {
                             const float4 q = pos4[j];
                             if (kKahan) {
                                 float tx = 0.f, ty = 0.f, tz = 0.f;

@@ -60,6 +60,21 @@ __device__ __forceinline__ void p2p_uniform_range(float4 p, const float4* __rest
     }
 }
 
+// one pair into target p's sum (plain or compensated)
+template <bool kKahan>
+__device__ __forceinline__ void p2p_acc(float4 p, float4 q, float& ax, float& ay, float& az,
+                                        float& cx, float& cy, float& cz) {
+    if (kKahan) {
+        float tx = 0.f, ty = 0.f, tz = 0.f;
+        ps_pair(p, q, tx, ty, tz);
+        ps_kahan_add(ax, cx, tx);
+        ps_kahan_add(ay, cy, ty);
+        ps_kahan_add(az, cz, tz);
+    } else {
+        ps_pair(p, q, ax, ay, az);
+    }
+}
+
 // the same for T targets per lane: each scalar-loaded source feeds all T;
 // each target's sum runs over the sources in the same order as above
 template <bool kKahan, int T>
@@ -267,9 +282,16 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
                         r0[u] = u < cnt ? link[src[u]].x : 0u;
                         r1[u] = u < cnt ? nend[src[u]] : 0u;
                     }
-                    // the first particle of each source is loaded while the
-                    // previous source is summed (one load ahead)
-                    float4 qn = cnt ? pos4[r0[0]] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    // the first two particles of each source are loaded while
+                    // the previous source is summed (one source ahead); the
+                    // loads are unconditional (index 0 past cnt, the first
+                    // particle again for a source of one), their use is not
+                    auto second = [&](int u) {
+                        const uint32_t e = r1[u] < (uint32_t)n ? r1[u] : (uint32_t)n;
+                        return r0[u] + 1u < e ? r0[u] + 1u : r0[u];
+                    };
+                    float4 qn = pos4[r0[0]];
+                    float4 qn2 = pos4[second(0)];
 #pragma unroll
                     for (int u = 0; u < kB; ++u) {
                         if (u >= cnt) break;
@@ -278,20 +300,14 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
                         const uint32_t j1e = r1[u];
                         const uint32_t j1 = j1e < (uint32_t)n ? j1e : (uint32_t)n;
                         pairs += j1e - j0;
-                        const float4 qf = qn;
-                        if (u + 1 < kB && u + 1 < cnt) qn = pos4[r0[u + 1]];
-                        if (j0 < j1) {
-                            if (kKahan) {
-                                float tx = 0.f, ty = 0.f, tz = 0.f;
-                                ps_pair(p, qf, tx, ty, tz);
-                                ps_kahan_add(ax, cx, tx);
-                                ps_kahan_add(ay, cy, ty);
-                                ps_kahan_add(az, cz, tz);
-                            } else {
-                                ps_pair(p, qf, ax, ay, az);
-                            }
+                        const float4 qa = qn, qb = qn2;
+                        if (u + 1 < kB) {
+                            qn = pos4[r0[u + 1]];
+                            qn2 = pos4[second(u + 1)];
                         }
-                        for (uint32_t j = j0 + 1; j < j1; ++j) {
+                        if (j0 < j1) p2p_acc<kKahan>(p, qa, ax, ay, az, cx, cy, cz);
+                        if (j0 + 1u < j1) p2p_acc<kKahan>(p, qb, ax, ay, az, cx, cy, cz);
+                        for (uint32_t j = j0 + 2; j < j1; ++j) {
                             const float4 q = pos4[j];
                             if (kKahan) {
                                 float tx = 0.f, ty = 0.f, tz = 0.f;

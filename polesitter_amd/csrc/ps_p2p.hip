@@ -228,7 +228,7 @@ __device__ __forceinline__ void p2p_block_stats(uint32_t pairs, uint32_t nsrc,
 // One thread per target particle of a small leaf (particles of big leaves
 // are left to p2p_big_kernel).
 template <int kMode, bool kKahan>
-__global__ __launch_bounds__(kBlock) void p2p_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void p2p_kernel(
     size_t n, const float4* __restrict__ pos4, const uint32_t* __restrict__ leaf_of,
     const float4* __restrict__ loc, const uint32_t* __restrict__ p2p_off,
     const uint32_t* __restrict__ p2p_len, const uint32_t* __restrict__ p2p_moff,
@@ -307,18 +307,13 @@ __global__ __launch_bounds__(kBlock) void p2p_kernel(
                         }
                         if (j0 < j1) p2p_acc<kKahan>(p, qa, ax, ay, az, cx, cy, cz);
                         if (j0 + 1u < j1) p2p_acc<kKahan>(p, qb, ax, ay, az, cx, cy, cz);
-                        for (uint32_t j = j0 + 2; j < j1; ++j) {
-                            const float4 q = pos4[j];
-                            if (kKahan) {
-                                float tx = 0.f, ty = 0.f, tz = 0.f;
-                                ps_pair(p, q, tx, ty, tz);
-                                ps_kahan_add(ax, cx, tx);
-                                ps_kahan_add(ay, cy, ty);
-                                ps_kahan_add(az, cz, tz);
-                            } else {
-                                ps_pair(p, q, ax, ay, az);
-                            }
+                        uint32_t j = j0 + 2;
+                        for (; j + 2 <= j1; j += 2) {  // two loads in flight
+                            const float4 q0 = pos4[j], q1 = pos4[j + 1];
+                            p2p_acc<kKahan>(p, q0, ax, ay, az, cx, cy, cz);
+                            p2p_acc<kKahan>(p, q1, ax, ay, az, cx, cy, cz);
                         }
+                        for (; j < j1; ++j) p2p_acc<kKahan>(p, pos4[j], ax, ay, az, cx, cy, cz);
                     }
                 }
             }

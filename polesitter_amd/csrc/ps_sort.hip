@@ -9,14 +9,16 @@
 //             (digit, tile) run in the output -- bucket-major, tile-minor,
 //             the same offset order as H:1661-1667
 //   downsweep the tile is ranked stably inside LDS (each wavefront ranks its
-//             own quarter in order: wave-wide match of equal digits with 10
+//             own sixteenth in order: wave-wide match of equal digits with 10
 //             ballots and the wave's running digit counts, then one scan of
 //             the counts over the waves -- three barriers per tile), then
 //             written out digit run by digit run, so consecutive lanes store
 //             to consecutive addresses; the payload (id, x, y, z, mass) is
 //             gathered from the tile's own input range, which is L2-resident.
-// Tiles are 4096 keys (256 threads x 16).  Ping-pong: io -> t0 -> t1 -> io,
-// so the sorted data ends in the caller's arrays (H:1688-1690).  Many systems
+// Tiles are 4096 keys (upsweep: 256 threads x 16; downsweep: 1024 threads
+// x 4, sixteen wavefronts of 256 keys each ranked in four rounds).
+// Ping-pong: io -> t0 -> t1 -> io, so the sorted data ends in the caller's
+// arrays (H:1688-1690).  Many systems
 // sorted at once carry their system id as a second key: a fourth, stable pass
 // by it groups the systems again, each in code order.
 #include "ps_device.h"
@@ -26,10 +28,14 @@ namespace {
 
 constexpr int kRadixBits = 10;
 constexpr int kRadix = 1 << kRadixBits;
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;   // upsweep
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;
-constexpr int kWaves = kBlock / 64;
+constexpr int kDBlock = 1024;  // downsweep: sixteen wavefronts rank one tile
+constexpr int kDItems = kTile / kDBlock;
+constexpr int kWaves = kDBlock / 64;
+constexpr int kDigitsPer = kRadix / kDBlock;  // digits of the tile histogram per thread
+static_assert(kDigitsPer >= 1 && kRadix % kDBlock == 0, "digits per thread");
 
 __global__ __launch_bounds__(kBlock) void sort_upsweep_kernel(const uint32_t* __restrict__ key,
                                                               size_t n, int shift,
@@ -49,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void sort_upsweep_kernel(const uint32_t* __
         table[(size_t)d * n_tiles + blockIdx.x] = h[d];
 }
 
-__global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
+__global__ __launch_bounds__(kDBlock) void sort_downsweep_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ iin,
     const float* __restrict__ xin, const float* __restrict__ yin, const float* __restrict__ zin,
     const float* __restrict__ min_, uint32_t* __restrict__ kout, uint32_t* __restrict__ iout,
@@ -70,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
     const size_t base = (size_t)blockIdx.x * kTile;
     const uint32_t valid_n = (uint32_t)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
 
-    for (int d = tid; d < kRadix; d += kBlock) {
+    for (int d = tid; d < kRadix; d += kDBlock) {
         tstart[d] = 0;
         goff[d] = offs[(size_t)d * n_tiles + blockIdx.x];
 #pragma unroll
@@ -79,9 +85,9 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
     __syncthreads();
 
     // 1. load the tile, count digits
-#pragma unroll 4
-    for (int r = 0; r < kItems; ++r) {
-        uint32_t idx = (uint32_t)(r * kBlock + tid);
+#pragma unroll
+    for (int r = 0; r < kDItems; ++r) {
+        uint32_t idx = (uint32_t)(r * kDBlock + tid);
         uint32_t k = 0xFFFFFFFFu;
         if (idx < valid_n) {
             k = kin[base + idx];
@@ -91,11 +97,15 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
     }
     __syncthreads();
 
-    // 2. exclusive scan of the tile histogram (4 digits per thread)
+    // 2. exclusive scan of the tile histogram (kDigitsPer digits per thread)
     {
-        uint32_t a0 = tstart[4 * tid + 0], a1 = tstart[4 * tid + 1];
-        uint32_t a2 = tstart[4 * tid + 2], a3 = tstart[4 * tid + 3];
-        uint32_t s = a0 + a1 + a2 + a3;
+        uint32_t a[kDigitsPer];
+        uint32_t s = 0;
+#pragma unroll
+        for (int q = 0; q < kDigitsPer; ++q) {
+            a[q] = tstart[kDigitsPer * tid + q];
+            s += a[q];
+        }
         uint32_t inc = s;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -106,10 +116,11 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
         __syncthreads();
         uint32_t ex = inc - s;
         for (int w = 0; w < wave; ++w) ex += wsum[w];
-        tstart[4 * tid + 0] = ex;
-        tstart[4 * tid + 1] = ex + a0;
-        tstart[4 * tid + 2] = ex + a0 + a1;
-        tstart[4 * tid + 3] = ex + a0 + a1 + a2;
+#pragma unroll
+        for (int q = 0; q < kDigitsPer; ++q) {
+            tstart[kDigitsPer * tid + q] = ex;
+            ex += a[q];
+        }
     }
     __syncthreads();
 
@@ -147,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
     }
     __syncthreads();
     // exclusive scan over the waves, per digit, in place
-    for (int d = tid; d < kRadix; d += kBlock) {
+    for (int d = tid; d < kRadix; d += kDBlock) {
         uint32_t acc = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
@@ -168,9 +179,9 @@ __global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(
 
     // 4. write out digit runs: position p of the tile goes to
     //    goff[d] + (p - tstart[d])
-#pragma unroll 4
-    for (int r = 0; r < kItems; ++r) {
-        uint32_t p = (uint32_t)(r * kBlock + tid);
+#pragma unroll
+    for (int r = 0; r < kDItems; ++r) {
+        uint32_t p = (uint32_t)(r * kDBlock + tid);
         if (p >= valid_n) break;
         uint32_t sidx_p = sidx[p];
         uint32_t k = keys[sidx_p];
@@ -224,7 +235,7 @@ hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, floa
         sort_upsweep_kernel<<<tiles, kBlock, 0, s>>>(kin, n, shift, table, tiles);
         hipError_t e = ps_scan_u32(table, table, table_n, nullptr, scan_tmp, s);
         if (e != hipSuccess) return e;
-        sort_downsweep_kernel<<<tiles, kBlock, 0, s>>>(
+        sort_downsweep_kernel<<<tiles, kDBlock, 0, s>>>(
             kin, a.id, a.x, a.y, a.z, a.m, kout, b.id, b.x, b.y, b.z, b.m,
             pass == passes - 1 ? pos4 : nullptr, ain, aout, n, shift, table, tiles);
     }

@@ -400,13 +400,15 @@ def main(argv=None):
         s.load(x, y, z, m)
 
     # warm-up ticks count the P2P pairs (extra reductions); the timed ticks
-    # record the stage events only, and the roofline takes its pair counts
-    # from the last warm-up tick
+    # record four events each (the traversal and the P2P stage, for the
+    # roofline), and the roofline takes its pair counts from the last warm-up
+    # tick; the stage breakdown comes from a few more ticks after the timed
+    # ones, with every stage event recorded
     counted = None
     a.warmup = max(a.warmup, 1)
     for _ in range(a.warmup):
         counted = step()
-    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 2))
+    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 3))
     api.check("sync", L.ps_hip_synchronize())
     stats = []
     t0 = time.perf_counter()
@@ -416,8 +418,11 @@ def main(argv=None):
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / a.steps
     value = n * a.steps / wall
+    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 2))
+    staged = [step() for _ in range(min(a.steps, 5))]
 
     mean = lambda k: float(np.mean([st[k] for st in stats]))
+    mean_staged = lambda k: float(np.mean([st[k] for st in staged]))
     kern = {
         "trav_step": dict(ms=mean("ms_trav_steps"), flop=FLOP_M2L * mean("m2l_pairs")),
         "p2p": dict(ms=mean("ms_p2p"), flop=FLOP_P2P * counted["p2p_pairs"]),
@@ -451,7 +456,7 @@ def main(argv=None):
                             morton=a.morton,
                             pointers="device-resident"),
                 roofline=roof,
-                stages_ms={k: round(mean("ms_" + k), 4) for k in
+                stages_ms={k: round(mean_staged("ms_" + k), 4) for k in
                            ("sort", "build", "upward", "traverse", "l2l", "p2p", "total",
                             "trav_steps")},
                 counts=dict(nodes=int(stats[-1]["n_nodes"]), m2l_pairs=int(mean("m2l_pairs")),

@@ -46,7 +46,8 @@ extern "C" {
 #define PS_HIP_OPT_TRAVERSAL 1 /* PS_HIP_TRAV_MT / PS_HIP_TRAV_ST */
 #define PS_HIP_OPT_KAHAN     2 /* 1: compensated far-field and P2P sums */
 #define PS_HIP_OPT_THETA     3 /* MAC theta, like ps_config_t.theta */
-#define PS_HIP_OPT_STATS     4 /* 1: time each stage and count P2P pairs; 2: time only */
+#define PS_HIP_OPT_STATS     4 /* 1: time each stage and count P2P pairs; 2: time only;
+                                  3: time the traversal only (two events per tick) */
 #define PS_HIP_OPT_TRAV_LANES 5 /* lanes per traversal entry: 8 (default), 16, 32, 64 */
 #define PS_HIP_OPT_P2P_BIG   6 /* P2P wavefront-per-leaf threshold (particles; 0 = off; 24) */
 #define PS_HIP_OPT_LEAF_CAP  7 /* most particles of a leaf above depth 10: 1 (the reference) .. 64 */

@@ -138,7 +138,9 @@ struct ps_hip_engine {
     float theta = 1.0f;
     int trav = PS_HIP_TRAV_MT;
     int kahan = 0;
-    int stats_on = 0;   // stage events
+    int stats_on = 0;          // stage events (all of them)
+    uint32_t ev_mask = 0;      // stage events recorded (bit k: ev[k])
+    bool trav_ev = false;      // events around the traversal iterations
     int count_on = 0;   // pair counts (extra reductions)
     int trav_lanes = 8;  // lanes per traversal entry
     uint32_t trav_blocks = 256;  // most workgroups of a traversal iteration (one per CU)
@@ -226,11 +228,13 @@ struct ps_hip_engine {
     }
 
     void record(int k) {
-        if (stats_on) (void)hipEventRecord(ev[k], s);
+        if ((ev_mask >> k) & 1u) (void)hipEventRecord(ev[k], s);
     }
     float elapsed(int a, int b) {
         float ms = 0.f;
-        if (stats_on && hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess) return ms;
+        if (((ev_mask >> a) & (ev_mask >> b) & 1u) &&
+            hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess)
+            return ms;
         return 0.f;
     }
 
@@ -385,7 +389,7 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     t.p2p_moff = e->p2p_moff();
     unsigned long long* far_part = reinterpret_cast<unsigned long long*>(e->far_b.p);
     size_t far_n = 0;
-    if (e->stats_on) (void)hipEventRecord(e->ev_it[0], e->s);
+    if (e->trav_ev) (void)hipEventRecord(e->ev_it[0], e->s);
     for (int k = 0; k < iters; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
         const uint32_t ub = std::min(entries_bound(e, mt, k), n_nodes);
@@ -394,7 +398,7 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
                             e->trav_blocks, e->trav_spb, e->s);
         far_n += ps_trav_blocks(ub, e->trav_lanes, e->trav_blocks);
     }
-    if (e->stats_on) (void)hipEventRecord(e->ev_it[1], e->s);
+    if (e->trav_ev) (void)hipEventRecord(e->ev_it[1], e->s);
     ps_reduce_u64(far_part, far_n, &st->m2l_pairs, e->s);
     return map_err(hipGetLastError());
 }
@@ -446,7 +450,7 @@ int trav_finish(ps_hip_engine* e, int attempt, bool* redo) {
     e->st.m2l_pairs = hst.m2l_pairs;
     e->st.list_entries = hst.arena_ctr - 1;
     e->st.iterations = e->max_depth + 1;
-    if (e->stats_on) {
+    if (e->trav_ev) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, e->ev_it[0], e->ev_it[1]) == hipSuccess)
             e->st.ms_trav_steps = ms;
@@ -742,9 +746,13 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
             e->theta = value > 0.0 ? (float)value : 1.0f;
             return PS_HIP_OK;
         case PS_HIP_OPT_STATS:
-            if (value != 0.0 && value != 1.0 && value != 2.0) return PS_HIP_EINVAL;
-            e->stats_on = value != 0.0;
+            if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0)
+                return PS_HIP_EINVAL;
+            e->stats_on = value == 1.0 || value == 2.0;
             e->count_on = value == 1.0;
+            e->trav_ev = value != 0.0;
+            // mode 3: the traversal and the P2P stage only (four events)
+            e->ev_mask = e->stats_on ? 0x3FFu : (value == 3.0 ? (1u << 6) | (1u << 7) : 0u);
             return PS_HIP_OK;
         case PS_HIP_OPT_TRAV_LANES:
             if (value != 8 && value != 16 && value != 32 && value != 64) return PS_HIP_EINVAL;
@@ -925,6 +933,8 @@ int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z, float* ma
     std::vector<uint32_t> off(offsets, offsets + n_systems + 1);
     e->stats_on = 0;
     e->count_on = 0;
+    e->trav_ev = false;
+    e->ev_mask = 0;
     e->n = n;
     e->x = x;
     e->y = y;
@@ -1099,7 +1109,7 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
                 PS_CK(hipMemcpyAsync(host[k], dev[k], n * 4, hipMemcpyDeviceToHost, e->s));
         if (e->count_on)
             PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 16, hipMemcpyDeviceToHost, e->s));
-        if (e->stats_on) PS_CK(hipEventRecord(e->ev[9], e->s));
+        e->record(9);
         PS_CK(hipStreamSynchronize(e->s));
         bool redo = false;
         PS_RC(trav_finish(e, attempt, &redo));
@@ -1114,7 +1124,7 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
         e->st.p2p_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 32);
         e->st.p2p_leaf_pairs = *reinterpret_cast<unsigned long long*>(e->h_small + 34);
     }
-    if (e->stats_on) {
+    if (e->ev_mask) {
         e->st.ms_sort = e->elapsed(1, 2);
         e->st.ms_build = e->elapsed(2, 3);
         e->st.ms_upward = e->elapsed(3, 4);

@@ -180,3 +180,26 @@ def test_dropin_c_program(oracle, tmp_path):
     assert np.array_equal(f32[0], o["x"]) and np.array_equal(f32[3], o["m"])
     f = np.stack(f32[4:7], 1).astype(np.float64)
     assert rms_rel(f, forces(o)) <= 1e-5
+
+
+def test_stats_modes_time_what_they_say():
+    # 2: every stage event; 3: the traversal and the P2P stage only; 0: none
+    x, y, z, m = W.plummer(50_000, seed=41)
+    with api.Solver(len(x), theta=2.0) as s:
+        got = {}
+        for mode in (2, 3, 0):
+            api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, mode))
+            s.load(x, y, z, m)
+            s.tick(api.MORTON_TRUNC)
+            got[mode] = s.stats()
+        assert api.ps_set_option(s.ctx, api.OPT_STATS, 4) != api.PS_OK
+    full, light, off = got[2], got[3], got[0]
+    for k in ("ms_sort", "ms_build", "ms_upward", "ms_traverse", "ms_p2p", "ms_total",
+              "ms_trav_steps"):
+        assert full[k] > 0.0, k
+    assert light["ms_trav_steps"] > 0.0 and light["ms_p2p"] > 0.0
+    for k in ("ms_sort", "ms_build", "ms_upward", "ms_total"):
+        assert light[k] == 0.0, k
+    for k in ("ms_sort", "ms_p2p", "ms_total", "ms_trav_steps"):
+        assert off[k] == 0.0, k
+    assert full["m2l_pairs"] == light["m2l_pairs"] == off["m2l_pairs"] > 0

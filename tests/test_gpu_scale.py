@@ -93,3 +93,18 @@ def test_ticks_with_euler_stay_consistent(oracle):
             api.check("euler", L.ps_hip_step_euler(s.ctx.engine, s.x.ptr, s.y.ptr, s.z.ptr,
                                                   vel[0].ptr, vel[1].ptr, vel[2].ptr, s.fx.ptr,
                                                   s.fy.ptr, s.fz.ptr, s.m.ptr, 1e-3, n))
+
+
+@pytest.mark.parametrize("n", [(1 << 22) + 12_345])
+def test_large_sort_is_the_stable_order(n):
+    # at 4M keys (many tiles, a partial last tile) the permutation is the
+    # stable one (equal codes keep their input order) and the payload follows
+    # it
+    x, y, z, m = W.plummer(n, seed=5)
+    log(f"sort {n}")
+    g = gpu_tick(x, y, z, m, theta=2.0, tree=False)
+    perm = np.argsort(g["prep_codes"], kind="stable")
+    assert np.array_equal(g["id"], perm.astype(np.uint32))
+    assert np.array_equal(g["codes"], g["prep_codes"][perm])
+    for k, a in (("x", x), ("y", y), ("z", z), ("m", m)):
+        assert np.array_equal(g[k], np.asarray(a, np.float32)[perm]), k

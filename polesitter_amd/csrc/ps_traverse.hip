@@ -238,12 +238,13 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
             }
             const unsigned long long fb = group_bits<G>(far, gshift);
             word |= fb << (j * G);
-            nfar += (uint32_t)__popcll(fb);
-            nleaf += (uint32_t)__popcll(group_bits<G>(near_leaf, gshift));
+            nleaf += near_leaf ? 1u : 0u;  // this lane's; summed over the group below
         }
         if (gl == 0 && mo + (k0 >> 6) < mask_cap) masks[mo + (k0 >> 6)] = word;
+        nfar += (uint32_t)__popcll(word);
     }
     nchild = group_sum_u32<G>(nchild);
+    nleaf = group_sum_u32<G>(nleaf);
     if (kKahan) {
 #pragma unroll
         for (int d = G / 2; d > 0; d >>= 1) {

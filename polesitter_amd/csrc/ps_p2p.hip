@@ -418,7 +418,7 @@ __device__ __forceinline__ void p2p_big_round(uint32_t c0, uint32_t te, uint32_t
 // per possible leaf.  The next leaf is asked for when a leaf starts, so the
 // atomic's round trip overlaps the leaf's own loads.
 constexpr int kBigWaves = kBlock / 64;
-constexpr uint32_t kBigBlocks = 1024;
+constexpr uint32_t kBigBlocks = 4096;
 constexpr uint32_t kCollectBlocks = 2048;
 
 template <int kMode, bool kKahan>

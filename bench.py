@@ -310,7 +310,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="plummer1m")
     ap.add_argument("--theta", type=float, default=2.0)
     ap.add_argument("--kahan", action="store_true")
-    ap.add_argument("--trav-lanes", type=int, default=8, choices=(8, 16, 32, 64))
+    ap.add_argument("--trav-lanes", type=int, default=0, choices=(0, 8, 16, 32, 64))
     ap.add_argument("--trav-blocks", type=int, default=0,
                     help="workgroups of one traversal iteration (a multiple of 8; 0: one per CU)")
     ap.add_argument("--trav-segments", type=int, default=0,
@@ -452,7 +452,7 @@ def main(argv=None):
                 config=dict(workload=a.workload, description=desc, n_particles=n,
                             theta=a.theta, traversal="MT depth-2 jobs (PS_MULTITHREADING)",
                             step="tick + id-gather + semi-implicit Euler dt=1e-3",
-                            kahan=bool(a.kahan), trav_lanes=a.trav_lanes, p2p_big=a.p2p_big,
+                            kahan=bool(a.kahan), trav_lanes=a.trav_lanes or "auto", p2p_big=a.p2p_big,
                             morton=a.morton,
                             pointers="device-resident"),
                 roofline=roof,

@@ -48,7 +48,9 @@ extern "C" {
 #define PS_HIP_OPT_THETA     3 /* MAC theta, like ps_config_t.theta */
 #define PS_HIP_OPT_STATS     4 /* 1: time each stage and count P2P pairs; 2: time only;
                                   3: time the traversal only (two events per tick) */
-#define PS_HIP_OPT_TRAV_LANES 5 /* lanes per traversal entry: 8 (default), 16, 32, 64 */
+#define PS_HIP_OPT_TRAV_LANES 5 /* lanes per traversal entry: 8, 16, 32, 64, or 0 (default):
+                                  per iteration, the widest group that gives every entry
+                                  a group at once, 8 when none does */
 #define PS_HIP_OPT_P2P_BIG   6 /* P2P wavefront-per-leaf threshold (particles; 0 = off; 24) */
 #define PS_HIP_OPT_LEAF_CAP  7 /* most particles of a leaf above depth 10: 1 (the reference) .. 64 */
 #define PS_HIP_OPT_ARENA_WORDS 8 /* traversal list arena at first use, words per particle

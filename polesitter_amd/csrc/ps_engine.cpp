@@ -142,7 +142,7 @@ struct ps_hip_engine {
     uint32_t ev_mask = 0;      // stage events recorded (bit k: ev[k])
     bool trav_ev = false;      // events around the traversal iterations
     int count_on = 0;   // pair counts (extra reductions)
-    int trav_lanes = 8;  // lanes per traversal entry
+    int trav_lanes = 0;  // lanes per traversal entry (0: chosen per iteration)
     uint32_t trav_blocks = 256;  // most workgroups of a traversal iteration (one per CU)
     uint32_t trav_spb = 2;       // traversal segments per workgroup
     uint32_t p2p_big = 24;  // leaves of >= this many particles: wavefront per leaf
@@ -369,6 +369,20 @@ uint32_t entries_bound(const ps_hip_engine* e, bool mt, int k) {
 // memory, no host synchronisation in between.  An internal source at
 // iteration k sits at depth k (each iteration expands every near internal
 // source), so a tree of depth D needs D + 1 iterations.
+// Lanes per entry of a traversal iteration of at most ub entries: the fixed
+// setting, or (0, the default) the widest group (64, 32, 16) with which every
+// entry has its group at once on the iteration's grid of trav_blocks blocks of
+// 1024 threads, else 8.  The small top iterations are chains of memory round
+// trips: with wide groups they spread over the whole chip, and each lane has
+// fewer of the entry's sources to test and expand.
+int trav_lanes_for(const ps_hip_engine* e, uint32_t ub) {
+    if (e->trav_lanes) return e->trav_lanes;
+    const unsigned long long lanes = (unsigned long long)e->trav_blocks * 1024ull;
+    for (int g = 64; g > 8; g >>= 1)
+        if ((unsigned long long)ub * (unsigned)g <= lanes) return g;
+    return 8;
+}
+
 int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     const uint32_t n_nodes = e->n_nodes;
     PsTravState* st = reinterpret_cast<PsTravState*>(e->tstate_b.p);
@@ -393,10 +407,11 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     for (int k = 0; k < iters; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
         const uint32_t ub = std::min(entries_bound(e, mt, k), n_nodes);
+        const int lanes = trav_lanes_for(e, ub);
         ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
-                            e->entries(nxt), far_part + far_n, e->kahan, e->trav_lanes,
+                            e->entries(nxt), far_part + far_n, e->kahan, lanes,
                             e->trav_blocks, e->trav_spb, e->s);
-        far_n += ps_trav_blocks(ub, e->trav_lanes, e->trav_blocks);
+        far_n += ps_trav_blocks(ub, lanes, e->trav_blocks);
     }
     if (e->trav_ev) (void)hipEventRecord(e->ev_it[1], e->s);
     ps_reduce_u64(far_part, far_n, &st->m2l_pairs, e->s);
@@ -634,10 +649,11 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
     size_t far_n = 0;
     for (int k = 0; k < PS_MAX_DEPTH_ITERS; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
+        const int lanes = trav_lanes_for(e, ub);
         ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
-                            e->entries(nxt), far_part + far_n, e->kahan, e->trav_lanes,
+                            e->entries(nxt), far_part + far_n, e->kahan, lanes,
                             e->trav_blocks, e->trav_spb, s);
-        far_n += ps_trav_blocks(ub, e->trav_lanes, e->trav_blocks);
+        far_n += ps_trav_blocks(ub, lanes, e->trav_blocks);
     }
     ps_reduce_u64(far_part, far_n, &st->m2l_pairs, s);
     ps_launch_l2l_dev(ls, e->parent(), e->loc(), s);
@@ -755,7 +771,8 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
             e->ev_mask = e->stats_on ? 0x3FFu : (value == 3.0 ? (1u << 6) | (1u << 7) : 0u);
             return PS_HIP_OK;
         case PS_HIP_OPT_TRAV_LANES:
-            if (value != 8 && value != 16 && value != 32 && value != 64) return PS_HIP_EINVAL;
+            if (value != 0 && value != 8 && value != 16 && value != 32 && value != 64)
+                return PS_HIP_EINVAL;
             e->trav_lanes = (int)value;
             return PS_HIP_OK;
         case PS_HIP_OPT_LEAF_CAP:

@@ -140,7 +140,9 @@ int ps_hip_engine_export_tree(ps_hip_engine_t* e, size_t cap, float* geom4,
  * reference's own tests call its static passes the same way,
  * tests/main.c:125-476).  bind, then any prefix of:
  * 1 sort, 2 build, 3 upward, 4 interaction (M2L), 5 downward (L2L + L2P,
- * f = m*field), 6 p2p (f += near field). */
+ * f = m*field), 6 p2p (f += near field); or 7 after 3: 4 to 6 as the tick
+ * runs them (the leaves' summed locals in one launch, the node locals
+ * summed when they are read). */
 int ps_hip_debug_bind(ps_hip_engine_t* e, float* x, float* y, float* z,
                       float* mass, float* fx, float* fy, float* fz,
                       uint32_t* id, size_t n, uint32_t* codes, float cx,

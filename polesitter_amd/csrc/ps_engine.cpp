@@ -122,7 +122,10 @@ size_t worst_case_nodes(size_t n) {
     return total;
 }
 
-enum Stage { kSort = 1, kBuild = 2, kUpward = 3, kInteract = 4, kDownward = 5, kP2P = 6 };
+enum Stage {
+    kSort = 1, kBuild = 2, kUpward = 3, kInteract = 4, kDownward = 5, kP2P = 6,
+    kTickTail = 7  // 4..6 as the tick runs them
+};
 
 constexpr int PS_MAX_DEPTH_ITERS = 11;  // traversal iterations of a depth-10 tree
 
@@ -151,6 +154,9 @@ struct ps_hip_engine {
     DevBuf mlp_b;           // learned correction parameters (empty: none)
     DevBuf mlp_pk_b;        // the same as the MFMA kernel's packed operands
     bool mlp_on = false;
+    // loc holds each node's own local (the tick gave the leaves their sums
+    // in ltot): the level-by-level L2L is still to be applied before loc is read
+    bool l2l_pending = false;
     DevBuf mlp_tmp;         // training scratch
 
     // host-pointer staging, one buffer per array (allocated on first use)
@@ -161,6 +167,7 @@ struct ps_hip_engine {
     DevBuf pos4_b, leaf_of_b;
     // nodes
     DevBuf geom_b, mp_b, loc_b, link_b, nend_b, parent_b, p2p_off_b, p2p_len_b, p2p_moff_b;
+    DevBuf ltot_b;  // summed locals of the leaves (the tick's L2L)
     // traversal: entries (tgt, off, len, moff) ping-pong; the list arena and
     // the far-mask arena fill over the whole tick
     DevBuf ent[2][4];
@@ -197,6 +204,7 @@ struct ps_hip_engine {
     float4* geom() { return reinterpret_cast<float4*>(geom_b.p); }
     float4* mp() { return reinterpret_cast<float4*>(mp_b.p); }
     float4* loc() { return reinterpret_cast<float4*>(loc_b.p); }
+    float4* ltot() { return reinterpret_cast<float4*>(ltot_b.p); }
     uint2* link() { return reinterpret_cast<uint2*>(link_b.p); }
     uint32_t* nend() { return reinterpret_cast<uint32_t*>(nend_b.p); }
     uint32_t* parent() { return reinterpret_cast<uint32_t*>(parent_b.p); }
@@ -242,7 +250,7 @@ struct ps_hip_engine {
         for (auto& b : stage) release(b);
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
-                         &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,
+                         &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,  &ltot_b,
                          &small_b,   &block_pairs_b, &tstate_b, &big_b,  &mlp_b, &mlp_pk_b, &mlp_tmp};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
@@ -282,6 +290,7 @@ hipError_t alloc_fixed(ps_hip_engine* e) {
     PS_AL(geom_b, 16 * M);
     PS_AL(mp_b, 16 * M);
     PS_AL(loc_b, 16 * M);
+    PS_AL(ltot_b, 16 * M);
     PS_AL(link_b, 8 * M);
     PS_AL(nend_b, 4 * M);
     PS_AL(parent_b, 4 * M);
@@ -337,6 +346,7 @@ int stage_build(ps_hip_engine* e) {
     ps_fill_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 0u, e->totals(), 4, e->node_cap,
                       e->s);
     ps_fill_u32_count(e->p2p_len(), 0u, e->totals(), 1, e->node_cap, e->s);
+    e->l2l_pending = false;
     PS_CK(hipEventSynchronize(e->ev_counts));
     const uint32_t flags = e->h_small[12];
     const unsigned long long total =
@@ -426,6 +436,7 @@ float theta_squared(const ps_hip_engine* e) {
 // Launches the traversal; its state (overflow flag, counts) is copied to the
 // host behind it and looked at by trav_finish once the stream has drained.
 int stage_traverse(ps_hip_engine* e) {
+    e->l2l_pending = false;
     const float theta2 = theta_squared(e);
     const bool mt = e->trav == PS_HIP_TRAV_MT;
     const int iters = (int)e->max_depth + 1;
@@ -476,10 +487,26 @@ int trav_finish(ps_hip_engine* e, int attempt, bool* redo) {
 int stage_l2l(ps_hip_engine* e) {
     for (uint32_t d = 1; d <= e->max_depth; ++d)
         ps_launch_l2l_level(e->ls[d], e->ls[d + 1], e->parent(), e->loc(), e->s);
+    e->l2l_pending = false;
     return map_err(hipGetLastError());
 }
 
-int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
+// The tick's L2L: only the leaves' sums feed the forces, so one launch puts
+// them in ltot and the level-by-level pass over loc waits until something
+// reads the node locals (materialize_locals).
+int stage_leaf_locals(ps_hip_engine* e) {
+    ps_launch_leaf_locals(e->n_nodes, e->link(), e->parent(), e->loc(), e->ltot(), e->s);
+    e->l2l_pending = true;
+    return map_err(hipGetLastError());
+}
+
+// Applies the pending L2L to loc (after a tick, before the locals are read).
+int materialize_locals(ps_hip_engine* e) {
+    if (!e->l2l_pending) return PS_HIP_OK;
+    return stage_l2l(e);
+}
+
+int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1, const float4* leaf_loc = nullptr) {
     unsigned long long* pc = e->count_on ? e->pairs() : nullptr;
     // candidates for the wavefront-per-leaf kernel: every node (leaves of
     // more than one particle sit at depth 10 unless leaf_cap > 1)
@@ -490,9 +517,9 @@ int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1) {
     if (!e->masks_b.p) PS_CK(grow(e->masks_b, 256, e->s));
     PS_CK(grow(e->big_b, 4 * ((size_t)bound + 2), e->s));
     if (ev_before >= 0) e->record(ev_before);
-    ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
-                  e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
-                  e->fz, mode, e->kahan, e->small_u32(), bound, e->p2p_big,
+    ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), leaf_loc ? leaf_loc : e->loc(), e->p2p_off(),
+                  e->p2p_len(), e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(),
+                  e->fx, e->fy, e->fz, mode, e->kahan, e->small_u32(), bound, e->p2p_big,
                   reinterpret_cast<uint32_t*>(e->big_b.p),
                   reinterpret_cast<unsigned long long*>(e->block_pairs_b.p), pc, e->s, e->s2,
                   e->fork, e->join);
@@ -504,8 +531,13 @@ int run_from_traverse(ps_hip_engine* e) {
     e->record(4);
     PS_RC(stage_traverse(e));
     e->record(5);
-    PS_RC(stage_l2l(e));
-    PS_RC(stage_p2p(e, 0, 6));  // records event 6 right before the P2P launch
+    if (e->mlp_on) {  // the correction's features read internal nodes' locals
+        PS_RC(stage_l2l(e));
+        PS_RC(stage_p2p(e, 0, 6));  // records event 6 right before the P2P launch
+    } else {
+        PS_RC(stage_leaf_locals(e));
+        PS_RC(stage_p2p(e, 0, 6, e->ltot()));
+    }
     e->record(7);
     if (e->mlp_on) {
         ps_launch_mlp_apply(e->n, e->mlp_pk_b.p, e->pos4(),
@@ -826,6 +858,7 @@ int ps_hip_engine_set_mlp(ps_hip_engine_t* e, const float* params) {
 int ps_hip_mlp_features(ps_hip_engine_t* e, float* feat, float* scale) {
     if (!e || !feat || !scale || !e->built) return PS_HIP_EINVAL;
     if (!is_device_ptr(feat) || !is_device_ptr(scale)) return PS_HIP_EINVAL;
+    PS_RC(materialize_locals(e));
     ps_launch_mlp_features(e->n, nullptr, e->n, e->pos4(), e->leaf_of(), e->geom(), e->mp(),
                            e->loc(), e->parent(), feat, scale, e->s);
     PS_CK(hipGetLastError());
@@ -839,6 +872,7 @@ int ps_hip_mlp_features_at(ps_hip_engine_t* e, const uint32_t* idx, size_t rows,
     if (rows == 0) return PS_HIP_OK;
     if (!is_device_ptr(idx) || !is_device_ptr(feat) || !is_device_ptr(scale))
         return PS_HIP_EINVAL;
+    PS_RC(materialize_locals(e));
     ps_launch_mlp_features(e->n, idx, rows, e->pos4(), e->leaf_of(), e->geom(), e->mp(),
                            e->loc(), e->parent(), feat, scale, e->s);
     PS_CK(hipGetLastError());
@@ -1163,6 +1197,7 @@ int ps_hip_engine_export_tree(ps_hip_engine_t* e, size_t cap, float* geom4, floa
     if (n_nodes) *n_nodes = e->n_nodes;
     const size_t M = e->n_nodes;
     if (cap < M) return PS_HIP_EOOM;
+    if (loc4) PS_RC(materialize_locals(e));
     PS_CK(hipStreamSynchronize(e->s));
     if (geom4) PS_CK(hipMemcpy(geom4, e->geom(), 16 * M, hipMemcpyDeviceToHost));
     if (mp4) PS_CK(hipMemcpy(mp4, e->mp(), 16 * M, hipMemcpyDeviceToHost));
@@ -1213,6 +1248,16 @@ int ps_hip_debug_stage(ps_hip_engine_t* e, int stage) {
             if (rc == PS_HIP_OK) rc = stage_p2p(e, 1);
             break;
         case kP2P: rc = stage_p2p(e, 2); break;
+        case kTickTail:
+            for (int attempt = 0;; ++attempt) {
+                rc = run_from_traverse(e);
+                if (rc != PS_HIP_OK) break;
+                PS_CK(hipStreamSynchronize(e->s));
+                bool redo = false;
+                rc = trav_finish(e, attempt, &redo);
+                if (rc != PS_HIP_OK || !redo) break;
+            }
+            break;
         default: return PS_HIP_EINVAL;
     }
     if (rc != PS_HIP_OK) return rc;

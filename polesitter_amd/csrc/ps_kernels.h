@@ -96,6 +96,10 @@ void ps_launch_upward_level(uint32_t begin, uint32_t end, const float4* pos4,
                             float4* mp, hipStream_t s);
 void ps_launch_l2l_level(uint32_t begin, uint32_t end, const uint32_t* parent, float4* loc,
                          hipStream_t s);
+// The summed local of every leaf (what the level passes leave there) into
+// ltot, in one launch; loc is not modified.
+void ps_launch_leaf_locals(uint32_t n_nodes, const uint2* link, const uint32_t* parent,
+                           const float4* loc, float4* ltot, hipStream_t s);
 
 // ---- ps_traverse.hip: dual-tree traversal, M2L, P2P lists -------------------
 struct PsTravTree {

@@ -235,6 +235,57 @@ __global__ __launch_bounds__(kBlock) void l2l_level_kernel(uint32_t begin, uint3
     loc[nidx] = cl;
 }
 
+// L2L for the leaves alone, in one launch.  The level-by-level pass makes a
+// node's local its own plus its parent's summed local, so a leaf's is the sum
+// of the locals on its path from the root, added from the root down; this
+// kernel adds them in that order (the bits agree with the level pass).  The
+// path comes from the parent links (at most PS_MAX_DEPTH steps, all loads of
+// the locals in flight at once); loc is only read, the leaves' sums go to
+// ltot (internal nodes' entries are not written).
+__global__ __launch_bounds__(kBlock) void leaf_locals_kernel(uint32_t n_nodes,
+                                                             const uint2* __restrict__ link,
+                                                             const uint32_t* __restrict__ parent,
+                                                             const float4* __restrict__ loc,
+                                                             float4* __restrict__ ltot) {
+    const uint32_t nidx = blockIdx.x * kBlock + threadIdx.x;
+    if (nidx >= n_nodes || !(link[nidx].y & PS_LEAF)) return;
+    uint32_t a[PS_MAX_DEPTH];  // ancestors, the parent first; PS_NONE past the root
+    uint32_t cur = nidx;
+#pragma unroll
+    for (int k = 0; k < PS_MAX_DEPTH; ++k) {
+        cur = cur != PS_NONE ? parent[cur] : PS_NONE;
+        a[k] = cur;
+    }
+    const float4 own = loc[nidx];
+    float4 v[PS_MAX_DEPTH];
+#pragma unroll
+    for (int k = 0; k < PS_MAX_DEPTH; ++k)
+        v[k] = a[k] != PS_NONE ? loc[a[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    bool have = false;  // the root's own local starts the sum
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+    for (int k = PS_MAX_DEPTH - 1; k >= 0; --k) {
+        if (a[k] == PS_NONE) continue;
+        if (have) {
+            sx = v[k].y + sx;
+            sy = v[k].z + sy;
+            sz = v[k].w + sz;
+        } else {
+            sx = v[k].y;
+            sy = v[k].z;
+            sz = v[k].w;
+            have = true;
+        }
+    }
+    float4 r = own;
+    if (have) {
+        r.y = own.y + sx;
+        r.z = own.z + sy;
+        r.w = own.w + sz;
+    }
+    ltot[nidx] = r;
+}
+
 // the same two passes with the level range read from device memory
 // (ls[d], ls[d+1]) and a fixed grid: nothing depends on host-side sizes, so
 // the launches can be recorded in a graph
@@ -356,6 +407,13 @@ void ps_launch_upward_level(uint32_t begin, uint32_t end, const float4* pos4,
     if (end <= begin) return;
     uint32_t blocks = (end - begin + kBlock - 1) / kBlock;
     upward_level_kernel<<<blocks, kBlock, 0, s>>>(begin, end, pos4, geom, link, nend, mp);
+}
+
+void ps_launch_leaf_locals(uint32_t n_nodes, const uint2* link, const uint32_t* parent,
+                           const float4* loc, float4* ltot, hipStream_t s) {
+    if (n_nodes == 0) return;
+    const uint32_t blocks = (n_nodes + kBlock - 1) / kBlock;
+    leaf_locals_kernel<<<blocks, kBlock, 0, s>>>(n_nodes, link, parent, loc, ltot);
 }
 
 void ps_launch_l2l_level(uint32_t begin, uint32_t end, const uint32_t* parent, float4* loc,

@@ -187,3 +187,42 @@ def test_fmm_p2p_pass():
     np.testing.assert_allclose(r["fx"], [e, -e], atol=1e-4)
     assert np.all(r["fy"] == 0) and np.all(r["fz"] == 0)
     st.close()
+
+
+def morton_codes(x, y, z, hw):
+    # 10-bit quantisation of [-hw, hw)^3 about the origin, interleaved x, y, z
+    out = np.zeros(len(x), np.uint32)
+    for k, v in enumerate((x, y, z)):
+        q = np.clip(np.floor((v.astype(np.float64) + hw) / (2 * hw) * 1024), 0, 1023)
+        q = q.astype(np.uint32)
+        for b in range(10):
+            out |= ((q >> np.uint32(b)) & np.uint32(1)) << np.uint32(3 * b + k)
+    return out
+
+
+@pytest.mark.parametrize("seed", [3, 11])
+def test_tick_tail_matches_the_staged_passes(seed):
+    # the tick's L2L (the leaves' summed locals in one launch, the node locals
+    # summed when read) against the level-by-level downward pass: the same
+    # forces and the same node locals, bit for bit
+    from polesitter_amd import workloads as W
+    x, y, z, m = W.two_plummers(60_000, seed=seed)[:4]
+    hw = float(max(np.abs(x).max(), np.abs(y).max(), np.abs(z).max())) * 1.01
+    codes = morton_codes(x, y, z, hw)
+    root = (0.0, 0.0, 0.0, hw)
+    out = []
+    for stages in ((1, 2, 3, 4, 5, 6), (1, 2, 3, 7)):
+        log(f"  seed {seed} stages {stages}")
+        st = Staged(x, y, z, m, codes, root, theta=2.0)
+        st.stage(*stages)
+        r = st.s.result()
+        t1 = st.tree(4 * len(x))
+        t2 = st.tree(4 * len(x))  # a second read sums nothing again
+        assert np.array_equal(t1["loc"].view(np.uint32), t2["loc"].view(np.uint32))
+        out.append((r, t1))
+        st.close()
+    (ra, ta), (rb, tb) = out
+    for k in ("fx", "fy", "fz"):
+        assert np.array_equal(ra[k].view(np.uint32), rb[k].view(np.uint32)), k
+    assert np.array_equal(ta["loc"].view(np.uint32), tb["loc"].view(np.uint32))
+    assert np.abs(ta["loc"][:, 1:]).max() > 0  # the far field is there

@@ -2,8 +2,11 @@
 device arrays, run as one forest and replayed from a recorded HIP graph.
 
 Bar: every system's sorted state (codes, ids, positions, masses) and forces
-are bit-identical to a single-system tick of the same particles (the same
-kernels visit the same lists in the same order), the forest's M2L pair count
+are bit-identical to a single-system tick of the same particles at the same
+lanes per traversal entry (the same kernels visit the same lists in the same
+order; the lane count sets the order in which an entry's M2L terms are
+summed, and the single-system tick otherwise picks it per iteration from its
+entry counts), the forest's M2L pair count
 is the sum of the oracle's per-system counts, and every graph replay
 reproduces the directly run tick bit for bit -- with other device work
 between the replays."""
@@ -17,6 +20,7 @@ from polesitter_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("codes", "id", "x", "y", "z", "m", "fx", "fy", "fz")
+LANES = 8  # lanes per traversal entry on both sides of a comparison
 
 
 def ragged_systems(seed=61):
@@ -37,7 +41,7 @@ def ragged_systems(seed=61):
 
 def single(sys_, mode, trav):
     x, y, z, m = sys_
-    return gpu_tick(x, y, z, m, theta=2.0, trav=trav, mode=mode, tree=False)
+    return gpu_tick(x, y, z, m, theta=2.0, trav=trav, mode=mode, tree=False, lanes=LANES)
 
 
 @pytest.mark.parametrize("mode", [api.MORTON_TRUNC, api.MORTON_AVX2])
@@ -46,6 +50,7 @@ def test_batch_equals_single_ticks(mode, trav):
     systems = ragged_systems()
     n = sum(len(s[0]) for s in systems)
     with api.Batch(n, len(systems), theta=2.0, traversal=trav, graph=False) as b:
+        b.set_option(api.OPT_TRAV_LANES, LANES)
         b.load(systems)
         b.tick(mode)
         res = b.result()
@@ -63,11 +68,12 @@ def test_batch_leaf_cap_equals_single_ticks():
     n = sum(len(s[0]) for s in systems)
     with api.Batch(n, len(systems), theta=2.0, graph=True) as b:
         b.set_option(api.OPT_LEAF_CAP, 8)
+        b.set_option(api.OPT_TRAV_LANES, LANES)
         b.load(systems)
         b.tick()
         res = b.result()
     for k, (x, y, z, m) in enumerate(systems):
-        o = gpu_tick(x, y, z, m, theta=2.0, tree=False, leaf_cap=8)
+        o = gpu_tick(x, y, z, m, theta=2.0, tree=False, leaf_cap=8, lanes=LANES)
         g = b.system(res, k)
         for f in FIELDS:
             assert np.array_equal(g[f], o[f]), (k, f)

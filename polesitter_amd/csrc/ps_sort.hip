@@ -5,9 +5,11 @@
 // (the reference uses four 8-bit passes over 32 bits; any stable sort yields
 // the same permutation).  Each pass:
 //   upsweep   per-tile digit histogram in LDS -> table[digit][tile]
-//   scan      exclusive scan of the digit-major table: the start of every
-//             (digit, tile) run in the output -- bucket-major, tile-minor,
-//             the same offset order as H:1661-1667
+//   row scan  one block per digit: exclusive scan of its row (where each
+//             tile's run starts inside the digit's output) and the digit's
+//             total; each downsweep block adds the exclusive scan of the
+//             1024 totals (where the digit's output starts) -- bucket-major,
+//             tile-minor, the same offset order as H:1661-1667
 //   downsweep the tile is ranked stably inside LDS (each wavefront ranks its
 //             own sixteenth in order: wave-wide match of equal digits with 10
 //             ballots and the wave's running digit counts, then one scan of
@@ -36,6 +38,8 @@ constexpr int kDItems = kTile / kDBlock;
 constexpr int kWaves = kDBlock / 64;
 constexpr int kDigitsPer = kRadix / kDBlock;  // digits of the tile histogram per thread
 static_assert(kDigitsPer >= 1 && kRadix % kDBlock == 0, "digits per thread");
+static_assert(kRadix == kDBlock, "the downsweep scans the digit totals one per thread");
+constexpr int kRowBlock = 256;  // row scan
 
 __global__ __launch_bounds__(kBlock) void sort_upsweep_kernel(const uint32_t* __restrict__ key,
                                                               size_t n, int shift,
@@ -55,6 +59,41 @@ __global__ __launch_bounds__(kBlock) void sort_upsweep_kernel(const uint32_t* __
         table[(size_t)d * n_tiles + blockIdx.x] = h[d];
 }
 
+// Row d of the digit-major table (the digit's count in every tile) becomes
+// its exclusive scan, in place; dtot[d] gets the row's total.  Thread k sums
+// a contiguous run of the row, the block scans the sums, and the run is
+// rewritten from there.
+__global__ __launch_bounds__(kRowBlock) void sort_rowscan_kernel(uint32_t* __restrict__ table,
+                                                                 uint32_t n_tiles,
+                                                                 uint32_t* __restrict__ dtot) {
+    __shared__ uint32_t wsum[kRowBlock / 64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t* row = table + (size_t)blockIdx.x * n_tiles;
+    const uint32_t per = (n_tiles + kRowBlock - 1) / kRowBlock;
+    const uint32_t lo = threadIdx.x * per < n_tiles ? threadIdx.x * per : n_tiles;
+    const uint32_t hi = lo + per < n_tiles ? lo + per : n_tiles;
+    uint32_t sum = 0;
+#pragma unroll 8
+    for (uint32_t t = lo; t < hi; ++t) sum += row[t];
+    uint32_t wtot;
+    uint32_t ex = ps_wave_excl_scan(sum, &wtot);
+    if (lane == 0) wsum[wave] = wtot;
+    __syncthreads();
+    uint32_t all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kRowBlock / 64; ++w) {
+        if (w < wave) ex += wsum[w];
+        all += wsum[w];
+    }
+#pragma unroll 8
+    for (uint32_t t = lo; t < hi; ++t) {
+        const uint32_t c = row[t];
+        row[t] = ex;
+        ex += c;
+    }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = all;
+}
+
 __global__ __launch_bounds__(kDBlock) void sort_downsweep_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ iin,
     const float* __restrict__ xin, const float* __restrict__ yin, const float* __restrict__ zin,
@@ -62,7 +101,7 @@ __global__ __launch_bounds__(kDBlock) void sort_downsweep_kernel(
     float* __restrict__ xout, float* __restrict__ yout, float* __restrict__ zout,
     float* __restrict__ mout, float4* __restrict__ pos4, const uint32_t* __restrict__ ain,
     uint32_t* __restrict__ aout, size_t n, int shift, const uint32_t* __restrict__ offs,
-    uint32_t n_tiles) {
+    const uint32_t* __restrict__ dtot, uint32_t n_tiles) {
     __shared__ uint32_t keys[kTile];
     __shared__ uint16_t sidx[kTile];
     __shared__ uint32_t tstart[kRadix];
@@ -76,11 +115,21 @@ __global__ __launch_bounds__(kDBlock) void sort_downsweep_kernel(
     const size_t base = (size_t)blockIdx.x * kTile;
     const uint32_t valid_n = (uint32_t)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
 
-    for (int d = tid; d < kRadix; d += kDBlock) {
+    // where digit tid's run of this tile goes: the digit's start (exclusive
+    // scan of the totals, one digit per thread) + the tile's offset in it
+    {
+        const uint32_t d = (uint32_t)tid;
+        const uint32_t own = offs[(size_t)d * n_tiles + blockIdx.x];
+        const uint32_t c = dtot[d];
+        uint32_t wt;
+        uint32_t ex = ps_wave_excl_scan(c, &wt);
+        if (lane == 0) wsum[wave] = wt;
         tstart[d] = 0;
-        goff[d] = offs[(size_t)d * n_tiles + blockIdx.x];
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) wcnt[w][d] = 0;
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) ex += wsum[w];
+        goff[d] = ex + own;
     }
     __syncthreads();
 
@@ -207,7 +256,7 @@ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 size_t ps_sort_tmp_bytes(size_t n) {
     size_t tiles = (n + kTile - 1) / kTile;
     size_t table = (size_t)kRadix * (tiles ? tiles : 1);
-    return align256(table * sizeof(uint32_t)) + ps_scan_tmp_bytes_u32(table);
+    return align256(table * sizeof(uint32_t)) + align256(kRadix * sizeof(uint32_t));
 }
 
 hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
@@ -216,7 +265,8 @@ hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, floa
     const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
     const size_t table_n = (size_t)kRadix * tiles;
     uint32_t* table = reinterpret_cast<uint32_t*>(tmp);
-    void* scan_tmp = reinterpret_cast<char*>(tmp) + align256(table_n * sizeof(uint32_t));
+    uint32_t* dtot =
+        reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(tmp) + align256(table_n * sizeof(uint32_t)));
     // three passes over the code (io -> t0 -> t1 -> io); with a second key,
     // io -> t0 -> t1 -> t0 by code, then t0 -> io by aux (code carried)
     const bool two = io.aux && t0.aux && t1.aux;
@@ -233,11 +283,10 @@ hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, floa
         const uint32_t* ain = two ? (by_aux ? a.key : a.aux) : nullptr;
         uint32_t* aout = two ? (by_aux ? b.key : b.aux) : nullptr;
         sort_upsweep_kernel<<<tiles, kBlock, 0, s>>>(kin, n, shift, table, tiles);
-        hipError_t e = ps_scan_u32(table, table, table_n, nullptr, scan_tmp, s);
-        if (e != hipSuccess) return e;
+        sort_rowscan_kernel<<<kRadix, kRowBlock, 0, s>>>(table, tiles, dtot);
         sort_downsweep_kernel<<<tiles, kDBlock, 0, s>>>(
             kin, a.id, a.x, a.y, a.z, a.m, kout, b.id, b.x, b.y, b.z, b.m,
-            pass == passes - 1 ? pos4 : nullptr, ain, aout, n, shift, table, tiles);
+            pass == passes - 1 ? pos4 : nullptr, ain, aout, n, shift, table, dtot, tiles);
     }
     return hipGetLastError();
 }

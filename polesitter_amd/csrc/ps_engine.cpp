@@ -343,9 +343,8 @@ int stage_build(ps_hip_engine* e) {
     PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
                         e->link(), e->nend(), e->parent(), e->leaf_of(), e->node_cap,
                         e->totals(), e->leaf_cap, e->s));
-    ps_fill_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 0u, e->totals(), 4, e->node_cap,
-                      e->s);
-    ps_fill_u32_count(e->p2p_len(), 0u, e->totals(), 1, e->node_cap, e->s);
+    ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
+                       e->totals(), e->node_cap, e->s);
     e->l2l_pending = false;
     PS_CK(hipEventSynchronize(e->ev_counts));
     const uint32_t flags = e->h_small[12];
@@ -651,8 +650,8 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
     PS_CK(ps_tree_write(e->code, sys, n, make_float4(0.f, 0.f, 0.f, 0.f), roots, e->tree_tmp.p,
                         e->geom(), e->link(), e->nend(), e->parent(), e->leaf_of(), cap,
                         e->totals(), e->leaf_cap, s));
-    ps_fill_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 0u, e->totals(), 4, cap, s);
-    ps_fill_u32_count(e->p2p_len(), 0u, e->totals(), 1, cap, s);
+    ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
+                       e->totals(), cap, s);
     ps_launch_upward_dev(ls, e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), s);
 
     float theta2;

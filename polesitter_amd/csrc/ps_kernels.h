@@ -29,7 +29,11 @@ void ps_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 // p[0 .. per * min(*count, cap)) = v with the count in device memory
 void ps_fill_u32_count(uint32_t* p, uint32_t v, const unsigned long long* count, uint32_t per,
                        size_t cap, hipStream_t s);
-// *out = sum of in[0..n) (cleared by a kernel, then one atomic add per block)
+// p[0 .. per * m) = v and p2[0 .. per2 * m) = v, m = min(*count, cap): one launch
+void ps_fill2_u32_count(uint32_t* p, uint32_t per, uint32_t* p2, uint32_t per2, uint32_t v,
+                        const unsigned long long* count, size_t cap, hipStream_t s);
+// *out = sum of in[0..n) (one block: stored; more: cleared by a kernel, then
+// one atomic add per block)
 void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* out,
                    hipStream_t s);
 

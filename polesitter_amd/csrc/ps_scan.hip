@@ -216,18 +216,22 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n) {
 }
 
 // p[0 .. per * min(*count, cap)) = v, the count read on the device
-__global__ void fill_u32_count_kernel(uint32_t* p, uint32_t v, const unsigned long long* count,
-                                      uint32_t per, size_t cap) {
+// p[0 .. per * m) = v and p2[0 .. per2 * m) = v (p2 may be null), with
+// m = min(*count, cap)
+__global__ void fill_u32_count_kernel(uint32_t* p, uint32_t per, uint32_t* p2, uint32_t per2,
+                                      uint32_t v, const unsigned long long* count, size_t cap) {
     const unsigned long long c = *count;
-    const size_t n = (size_t)per * (size_t)(c < cap ? c : cap);
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = v;
+    const size_t m = (size_t)(c < cap ? c : cap);
+    const size_t n = (size_t)per * m, n2 = p2 ? (size_t)per2 * m : 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride)
+        p2[i] = v;
 }
 
 // Sum of n u64 values into *out: each block sums its slice (kRedV
-// independent loads per thread per round) and adds it to *out, which the
-// launcher clears first.
+// independent loads per thread per round); a single block stores the sum,
+// several add theirs to *out, which the launcher clears first.
 constexpr int kRedThreads = 1024;
 constexpr int kRedV = 8;
 constexpr size_t kRedSlice = (size_t)kRedThreads * kRedV * 4;  // values per block
@@ -258,7 +262,10 @@ __global__ __launch_bounds__(kRedThreads) void reduce_u64_kernel(const unsigned 
     if (threadIdx.x == 0) {
         unsigned long long s = 0;
         for (int w = 0; w < kRedThreads / 64; ++w) s += ws[w];
-        atomicAdd(out, s);
+        if (gridDim.x == 1)
+            *out = s;
+        else
+            atomicAdd(out, s);
     }
 }
 
@@ -369,12 +376,20 @@ void ps_fill_u32_count(uint32_t* p, uint32_t v, const unsigned long long* count,
     if (cap == 0 || per == 0) return;
     size_t blocks = (per * cap + 255) / 256;
     if (blocks > 4096) blocks = 4096;
-    fill_u32_count_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, v, count, per, cap);
+    fill_u32_count_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, per, nullptr, 0u, v, count, cap);
+}
+
+void ps_fill2_u32_count(uint32_t* p, uint32_t per, uint32_t* p2, uint32_t per2, uint32_t v,
+                        const unsigned long long* count, size_t cap, hipStream_t s) {
+    if (cap == 0 || per + per2 == 0) return;
+    size_t blocks = ((size_t)(per > per2 ? per : per2) * cap + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    fill_u32_count_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, per, p2, per2, v, count, cap);
 }
 
 void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* out,
                    hipStream_t s) {
     const unsigned blocks = (unsigned)((n + kRedSlice - 1) / kRedSlice);
-    clear_u64_kernel<<<1, 1, 0, s>>>(out);
-    if (blocks) reduce_u64_kernel<<<blocks, kRedThreads, 0, s>>>(in, n, out);
+    if (blocks > 1) clear_u64_kernel<<<1, 1, 0, s>>>(out);
+    reduce_u64_kernel<<<blocks > 1 ? blocks : 1u, kRedThreads, 0, s>>>(in, n, out);
 }

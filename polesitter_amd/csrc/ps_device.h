@@ -101,19 +101,6 @@ __device__ __forceinline__ bool ps_far_d(float4 t, float4 s, float theta2, float
     return d2 > theta2 * (hs * hs);
 }
 
-// wave-wide sums (64 lanes), result valid in every lane
-__device__ __forceinline__ float ps_wave_sum(float v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
-__device__ __forceinline__ uint32_t ps_wave_sum_u32(uint32_t v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
 // exclusive prefix over the wave; *total gets the wave-wide sum
 __device__ __forceinline__ uint32_t ps_wave_excl_scan(uint32_t v, uint32_t* total) {
     const int lane = __lane_id();

@@ -19,47 +19,6 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// accumulate the pairs of target p with the particles [j0, j1) of one source
-// leaf; j0, j1 wave-uniform: the source loads become scalar loads shared by
-// the wave, four particles per round
-template <bool kKahan>
-__device__ __forceinline__ void p2p_uniform_range(float4 p, const float4* __restrict__ pos4,
-                                                  uint32_t j0, uint32_t j1, float& ax, float& ay,
-                                                  float& az, float& cx, float& cy, float& cz) {
-    uint32_t j = j0;
-    for (; j + 4 <= j1; j += 4) {
-        const float4 q0 = pos4[j], q1 = pos4[j + 1], q2 = pos4[j + 2], q3 = pos4[j + 3];
-        if (kKahan) {
-            const float4 qs[4] = {q0, q1, q2, q3};
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                float tx = 0.f, ty = 0.f, tz = 0.f;
-                ps_pair(p, qs[u], tx, ty, tz);
-                ps_kahan_add(ax, cx, tx);
-                ps_kahan_add(ay, cy, ty);
-                ps_kahan_add(az, cz, tz);
-            }
-        } else {
-            ps_pair(p, q0, ax, ay, az);
-            ps_pair(p, q1, ax, ay, az);
-            ps_pair(p, q2, ax, ay, az);
-            ps_pair(p, q3, ax, ay, az);
-        }
-    }
-    for (; j < j1; ++j) {
-        const float4 q = pos4[j];
-        if (kKahan) {
-            float tx = 0.f, ty = 0.f, tz = 0.f;
-            ps_pair(p, q, tx, ty, tz);
-            ps_kahan_add(ax, cx, tx);
-            ps_kahan_add(ay, cy, ty);
-            ps_kahan_add(az, cz, tz);
-        } else {
-            ps_pair(p, q, ax, ay, az);
-        }
-    }
-}
-
 // one pair into target p's sum (plain or compensated)
 template <bool kKahan>
 __device__ __forceinline__ void p2p_acc(float4 p, float4 q, float& ax, float& ay, float& az,
@@ -75,8 +34,9 @@ __device__ __forceinline__ void p2p_acc(float4 p, float4 q, float& ax, float& ay
     }
 }
 
-// the same for T targets per lane: each scalar-loaded source feeds all T;
-// each target's sum runs over the sources in the same order as above
+// accumulate T targets per lane (p[0..T)) with the particles [j0, j1) of one
+// source leaf; j0, j1 wave-uniform, so each source particle is a scalar load
+// shared by the wave and feeds all T; each target sums in particle order
 template <bool kKahan, int T>
 __device__ __forceinline__ void p2p_uniform_range_t(const float4* p,
                                                     const float4* __restrict__ pos4, uint32_t j0,
@@ -108,34 +68,8 @@ __device__ __forceinline__ bool p2p_is_big(uint32_t lf, uint32_t tmin,
     return tmin && nend[lf] - link[lf].x >= tmin;
 }
 
-// accumulate target p against the near sources of one final list (wave-uniform
-// list: scalar loads); pairs += particle pairs, nsrc += source leaves
-template <bool kKahan>
-__device__ __forceinline__ void p2p_uniform_list(float4 p, size_t n, uint32_t L, uint32_t o,
-                                                 uint32_t mo, const float4* __restrict__ pos4,
-                                                 const uint32_t* __restrict__ arena,
-                                                 const unsigned long long* __restrict__ masks,
-                                                 const uint2* __restrict__ link,
-                                                 const uint32_t* __restrict__ nend, float& ax,
-                                                 float& ay, float& az, float& cx, float& cy,
-                                                 float& cz, uint32_t& pairs, uint32_t& nsrc) {
-    for (uint32_t k0 = 0; k0 < L; k0 += 64) {
-        unsigned long long near = ~masks[mo + (k0 >> 6)];
-        if (L - k0 < 64) near &= (1ull << (L - k0)) - 1ull;
-        while (near) {
-            const uint32_t b = (uint32_t)__ffsll((long long)near) - 1u;
-            near &= near - 1ull;
-            ++nsrc;
-            const uint32_t s = arena[o + k0 + b];
-            const uint32_t j0 = link[s].x;
-            const uint32_t j1e = nend[s];
-            const uint32_t j1 = j1e < (uint32_t)n ? j1e : (uint32_t)n;
-            pairs += j1e - j0;
-            p2p_uniform_range<kKahan>(p, pos4, j0, j1, ax, ay, az, cx, cy, cz);
-        }
-    }
-}
-
+// accumulate T targets per lane against the near sources of one final list
+// (wave-uniform: scalar loads); pairs += particle pairs, nsrc += source leaves
 template <bool kKahan, int T>
 __device__ __forceinline__ void p2p_uniform_list_t(const float4* p, size_t n, uint32_t L,
                                                    uint32_t o, uint32_t mo,

@@ -330,13 +330,14 @@ __device__ __forceinline__ void write_entry(const PsTravTree& t, const PsTravEnt
             uint32_t total;
             const uint32_t inc = group_scan_u32<G>(nout, gl, total);
             const uint32_t w = base + inc - nout;
-            if (nout) {
-                if (sl[j].y & PS_LEAF) {
-                    if (w < limit) arena[w] = s[j];
-                } else {
-                    for (uint32_t r = 0; r < nout; ++r)
-                        if (w + r < limit) arena[w + r] = sl[j].x + r;
-                }
+            // a near leaf is one word (itself), a near internal node its
+            // nout <= 8 children, consecutive ids from its first child
+            if (nout && w + nout <= limit) {
+                uint32_t* dst = arena + w;
+                const uint32_t v0 = (sl[j].y & PS_LEAF) ? s[j] : sl[j].x;
+#pragma unroll
+                for (uint32_t r = 0; r < 8; ++r)
+                    if (r < nout) dst[r] = v0 + r;
             }
             base += total;
         }

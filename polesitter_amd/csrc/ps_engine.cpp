@@ -168,6 +168,7 @@ struct ps_hip_engine {
     // nodes
     DevBuf geom_b, mp_b, loc_b, link_b, nend_b, parent_b, p2p_off_b, p2p_len_b, p2p_moff_b;
     DevBuf ltot_b;  // summed locals of the leaves (the tick's L2L)
+    DevBuf oct_b;   // each node's slot in its parent (tree write scratch)
     // traversal: entries (tgt, off, len, moff) ping-pong; the list arena and
     // the far-mask arena fill over the whole tick
     DevBuf ent[2][4];
@@ -205,6 +206,7 @@ struct ps_hip_engine {
     float4* mp() { return reinterpret_cast<float4*>(mp_b.p); }
     float4* loc() { return reinterpret_cast<float4*>(loc_b.p); }
     float4* ltot() { return reinterpret_cast<float4*>(ltot_b.p); }
+    uint8_t* oct() { return reinterpret_cast<uint8_t*>(oct_b.p); }
     uint2* link() { return reinterpret_cast<uint2*>(link_b.p); }
     uint32_t* nend() { return reinterpret_cast<uint32_t*>(nend_b.p); }
     uint32_t* parent() { return reinterpret_cast<uint32_t*>(parent_b.p); }
@@ -250,7 +252,7 @@ struct ps_hip_engine {
         for (auto& b : stage) release(b);
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
-                         &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,  &ltot_b,
+                         &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,  &ltot_b, &oct_b,
                          &small_b,   &block_pairs_b, &tstate_b, &big_b,  &mlp_b, &mlp_pk_b, &mlp_tmp};
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
@@ -291,6 +293,7 @@ hipError_t alloc_fixed(ps_hip_engine* e) {
     PS_AL(mp_b, 16 * M);
     PS_AL(loc_b, 16 * M);
     PS_AL(ltot_b, 16 * M);
+    PS_AL(oct_b, M);
     PS_AL(link_b, 8 * M);
     PS_AL(nend_b, 4 * M);
     PS_AL(parent_b, 4 * M);
@@ -341,8 +344,8 @@ int stage_build(ps_hip_engine* e) {
     PS_CK(hipMemcpyAsync(e->h_small, e->small_b.p, 96, hipMemcpyDeviceToHost, e->s));
     PS_CK(hipEventRecord(e->ev_counts, e->s));
     PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
-                        e->link(), e->nend(), e->parent(), e->leaf_of(), e->node_cap,
-                        e->totals(), e->leaf_cap, e->s));
+                        e->link(), e->nend(), e->parent(), e->leaf_of(), e->oct(),
+                        e->node_cap, e->totals(), e->leaf_cap, e->s));
     ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
                        e->totals(), e->node_cap, e->s);
     e->l2l_pending = false;
@@ -610,6 +613,7 @@ int batch_ensure(ps_hip_batch* b, size_t n, size_t cap) {
     DevBuf* node16[] = {&e->geom_b, &e->mp_b, &e->loc_b};
     for (DevBuf* d : node16) PS_CK(grow(*d, 16 * cap, s));
     PS_CK(grow(e->link_b, 8 * cap, s));
+    PS_CK(grow(e->oct_b, cap, s));
     DevBuf* node4[] = {&e->nend_b, &e->parent_b, &e->p2p_off_b, &e->p2p_len_b, &e->p2p_moff_b};
     for (DevBuf* d : node4) PS_CK(grow(*d, 4 * cap, s));
     for (int g = 0; g < 2; ++g)
@@ -648,8 +652,8 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
     PS_CK(ps_tree_count(e->code, sys, n, ls, ls + 12, e->totals(), e->tree_tmp.p, e->leaf_cap,
                         s));
     PS_CK(ps_tree_write(e->code, sys, n, make_float4(0.f, 0.f, 0.f, 0.f), roots, e->tree_tmp.p,
-                        e->geom(), e->link(), e->nend(), e->parent(), e->leaf_of(), cap,
-                        e->totals(), e->leaf_cap, s));
+                        e->geom(), e->link(), e->nend(), e->parent(), e->leaf_of(), e->oct(),
+                        cap, e->totals(), e->leaf_cap, s));
     ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
                        e->totals(), cap, s);
     ps_launch_upward_dev(ls, e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), s);

@@ -86,10 +86,13 @@ hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
 // is `root`, or roots[sys[i]] per system when roots is set.
 // n_nodes_dev (optional): the node count in device memory (then n_nodes is
 // only the capacity).  Nodes at or past n_nodes are not written.
+// octant: one byte per node (the node's slot in its parent), scratch for the
+// child masks, which are stored without atomics
 hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, float4 root,
                          const float4* roots, void* tmp, float4* geom, uint2* link,
-                         uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, size_t n_nodes,
-                         const unsigned long long* n_nodes_dev, int leaf_cap, hipStream_t s);
+                         uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, uint8_t* octant,
+                         size_t n_nodes, const unsigned long long* n_nodes_dev, int leaf_cap,
+                         hipStream_t s);
 // Upward (depths 10..0) and L2L (depths 1..10) with the level starts read
 // from device memory ls[0..11]; fixed grids, no host-side sizes.
 void ps_launch_upward_dev(const uint32_t* ls, const float4* pos4, const float4* geom,

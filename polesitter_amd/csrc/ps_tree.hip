@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
     const uint32_t* __restrict__ off, uint32_t nb, float4 root,
     const float4* __restrict__ roots, float4* __restrict__ geom, uint2* __restrict__ link,
     uint32_t* __restrict__ nend, uint32_t* __restrict__ parent, uint32_t* __restrict__ leaf_of,
-    int leaf_cap, uint32_t cap) {
+    uint8_t* __restrict__ octant, int leaf_cap, uint32_t cap) {
 #pragma clang fp contract(off)
     __shared__ uint32_t ws[kWaves][kDepths];
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
             } else {
                 link32[2 * (size_t)nd] = node[d + 1 < kDepths ? d + 1 : d];
             }
-            if (d > 0) atomicOr(&link32[2 * (size_t)node[d - 1] + 1], 1u << oct);
+            if (d > 0) octant[nd] = (uint8_t)oct;  // the parent's mask: tree_mask_kernel
         }
         if (d >= pd.end_lo && node[d] < cap) nend[node[d]] = (uint32_t)i + 1u;
     }
@@ -177,6 +177,27 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
     for (int d = 1; d < kDepths; ++d)
         if (d <= pd.leaf) lf = node[d];
     leaf_of[i] = lf;
+}
+
+// Child masks without atomics: a parent's children are consecutive nodes
+// (one level, Morton order), so the first child of each run ORs the octants
+// of its <= 8 siblings and stores the parent's mask (link.y, zero until
+// here for internal nodes).  m = min(*count, cap) nodes, or n_nodes.
+__global__ __launch_bounds__(kBlock) void tree_mask_kernel(const uint32_t* __restrict__ parent,
+                                                           const uint8_t* __restrict__ octant,
+                                                           uint2* __restrict__ link,
+                                                           const unsigned long long* count,
+                                                           size_t n_nodes) {
+    const size_t m = count ? (size_t)(*count < n_nodes ? *count : n_nodes) : n_nodes;
+    uint32_t* link32 = reinterpret_cast<uint32_t*>(link);
+    for (size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x; c < m;
+         c += (size_t)gridDim.x * kBlock) {
+        const uint32_t p = parent[c];
+        if (p == PS_NONE || (c > 0 && parent[c - 1] == p)) continue;
+        uint32_t mask = 1u << octant[c];
+        for (size_t k = c + 1; k < m && k < c + 8 && parent[k] == p; ++k) mask |= 1u << octant[k];
+        link32[2 * (size_t)p + 1] = mask;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, uint32_t end,
@@ -387,8 +408,9 @@ hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
 
 hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, float4 root,
                          const float4* roots, void* tmp, float4* geom, uint2* link,
-                         uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, size_t n_nodes,
-                         const unsigned long long* n_nodes_dev, int leaf_cap, hipStream_t s) {
+                         uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, uint8_t* octant,
+                         size_t n_nodes, const unsigned long long* n_nodes_dev, int leaf_cap,
+                         hipStream_t s) {
     const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
     const uint32_t* off = reinterpret_cast<const uint32_t*>(tmp);
     if (n_nodes_dev)
@@ -397,7 +419,13 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
         ps_fill_u32(reinterpret_cast<uint32_t*>(link), 0u, 2 * n_nodes, s);
     const uint32_t cap = n_nodes < 0xFFFFFFFFull ? (uint32_t)n_nodes : 0xFFFFFFFFu;
     tree_write_kernel<<<nb, kBlock, 0, s>>>(codes, sys, n, off, nb, root, roots, geom, link,
-                                            nend, parent, leaf_of, leaf_cap, cap);
+                                            nend, parent, leaf_of, octant, leaf_cap, cap);
+    if (n_nodes) {
+        size_t mb = (n_nodes + kBlock - 1) / kBlock;
+        if (mb > 4096) mb = 4096;
+        tree_mask_kernel<<<(unsigned)mb, kBlock, 0, s>>>(parent, octant, link, n_nodes_dev,
+                                                         (size_t)cap);
+    }
     return hipGetLastError();
 }
 

@@ -45,7 +45,7 @@ FAMILIES = {
     "trav_step": ("trav_step_kernel",),
     "p2p": ("p2p_kernel", "p2p_big_kernel", "p2p_collect_kernel"),
     "sort": ("sort_upsweep_kernel", "sort_rowscan_kernel", "sort_downsweep_kernel"),
-    "build": ("tree_count_kernel", "tree_levels_kernel", "tree_write_kernel"),
+    "build": ("tree_count_kernel", "tree_levels_kernel", "tree_write_kernel", "tree_mask_kernel"),
     "upward": ("upward_level_kernel", "upward_level_dev_kernel"),
     "l2l": ("leaf_locals_kernel", "l2l_level_kernel", "l2l_level_dev_kernel"),
     "prepare": ("bbox_partial_kernel", "bbox_final_kernel", "morton_kernel", "bbox_seg_kernel",

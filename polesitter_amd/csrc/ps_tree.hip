@@ -200,6 +200,33 @@ __global__ __launch_bounds__(kBlock) void tree_mask_kernel(const uint32_t* __res
     }
 }
 
+// P2M of particles [i0, e) about the cell centre g, added in particle order
+// (H:978-1004); the loads run four particles ahead of the sums
+__device__ __forceinline__ void p2m_leaf(const float4* __restrict__ pos4, uint32_t i0,
+                                         uint32_t e, float4 g, float& m0, float& mx, float& my,
+                                         float& mz) {
+#pragma clang fp contract(off)
+    auto add = [&](float4 p) {
+#pragma clang fp contract(off)
+        const float dx = p.x - g.x;
+        const float dy = p.y - g.y;
+        const float dz = p.z - g.z;
+        m0 += p.w;
+        mx += p.w * dx;
+        my += p.w * dy;
+        mz += p.w * dz;
+    };
+    uint32_t i = i0;
+    for (; i + 4 <= e; i += 4) {
+        const float4 p0 = pos4[i], p1 = pos4[i + 1], p2 = pos4[i + 2], p3 = pos4[i + 3];
+        add(p0);
+        add(p1);
+        add(p2);
+        add(p3);
+    }
+    for (; i < e; ++i) add(pos4[i]);
+}
+
 __global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, uint32_t end,
                                                               const float4* __restrict__ pos4,
                                                               const float4* __restrict__ geom,
@@ -213,17 +240,7 @@ __global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, ui
     const uint2 l = link[nidx];
     float m0 = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
     if (l.y & PS_LEAF) {  // P2M about the cell centre, H:978-1004
-        const uint32_t e = nend[nidx];
-        for (uint32_t i = l.x; i < e; ++i) {
-            float4 p = pos4[i];
-            float dx = p.x - g.x;
-            float dy = p.y - g.y;
-            float dz = p.z - g.z;
-            m0 += p.w;
-            mx += p.w * dx;
-            my += p.w * dy;
-            mz += p.w * dz;
-        }
+        p2m_leaf(pos4, l.x, nend[nidx], g, m0, mx, my, mz);
     } else {  // M2M in slot order, H:1013-1039
         uint32_t ch = l.x;
         for (uint32_t s = 0; s < 8; ++s) {
@@ -322,17 +339,7 @@ __global__ __launch_bounds__(kBlock) void upward_level_dev_kernel(
         const uint2 l = link[nidx];
         float m0 = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
         if (l.y & PS_LEAF) {
-            const uint32_t e = nend[nidx];
-            for (uint32_t i = l.x; i < e; ++i) {
-                float4 p = pos4[i];
-                float dx = p.x - g.x;
-                float dy = p.y - g.y;
-                float dz = p.z - g.z;
-                m0 += p.w;
-                mx += p.w * dx;
-                my += p.w * dy;
-                mz += p.w * dz;
-            }
+            p2m_leaf(pos4, l.x, nend[nidx], g, m0, mx, my, mz);
         } else {
             uint32_t ch = l.x;
             for (uint32_t sl = 0; sl < 8; ++sl) {

@@ -227,6 +227,36 @@ __device__ __forceinline__ void p2m_leaf(const float4* __restrict__ pos4, uint32
     for (; i < e; ++i) add(pos4[i]);
 }
 
+// M2M of an internal node's children about its centre g, added in slot order
+// (H:1013-1039).  The children are the consecutive nodes from l.x, one per
+// set bit of the mask, so all their records are loaded before the sums.
+__device__ __forceinline__ void m2m_children(const float4* __restrict__ geom,
+                                             const float4* __restrict__ mp, uint2 l, float4 g,
+                                             float& m0, float& mx, float& my, float& mz) {
+#pragma clang fp contract(off)
+    const uint32_t cnt = (uint32_t)__popc(l.y & 0xFFu);
+    float4 cg[8], cm[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        if (k < cnt) {
+            cg[k] = geom[l.x + k];
+            cm[k] = mp[l.x + k];
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        if (k < cnt) {
+            const float dx = cg[k].x - g.x;
+            const float dy = cg[k].y - g.y;
+            const float dz = cg[k].z - g.z;
+            m0 += cm[k].x;
+            mx += cm[k].y + cm[k].x * dx;
+            my += cm[k].z + cm[k].x * dy;
+            mz += cm[k].w + cm[k].x * dz;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, uint32_t end,
                                                               const float4* __restrict__ pos4,
                                                               const float4* __restrict__ geom,
@@ -242,20 +272,7 @@ __global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, ui
     if (l.y & PS_LEAF) {  // P2M about the cell centre, H:978-1004
         p2m_leaf(pos4, l.x, nend[nidx], g, m0, mx, my, mz);
     } else {  // M2M in slot order, H:1013-1039
-        uint32_t ch = l.x;
-        for (uint32_t s = 0; s < 8; ++s) {
-            if (!((l.y >> s) & 1u)) continue;
-            float4 cg = geom[ch];
-            float4 cm = mp[ch];
-            ++ch;
-            float dx = cg.x - g.x;
-            float dy = cg.y - g.y;
-            float dz = cg.z - g.z;
-            m0 += cm.x;
-            mx += cm.y + cm.x * dx;
-            my += cm.z + cm.x * dy;
-            mz += cm.w + cm.x * dz;
-        }
+        m2m_children(geom, mp, l, g, m0, mx, my, mz);
     }
     mp[nidx] = make_float4(m0, mx, my, mz);
 }
@@ -341,20 +358,7 @@ __global__ __launch_bounds__(kBlock) void upward_level_dev_kernel(
         if (l.y & PS_LEAF) {
             p2m_leaf(pos4, l.x, nend[nidx], g, m0, mx, my, mz);
         } else {
-            uint32_t ch = l.x;
-            for (uint32_t sl = 0; sl < 8; ++sl) {
-                if (!((l.y >> sl) & 1u)) continue;
-                float4 cg = geom[ch];
-                float4 cm = mp[ch];
-                ++ch;
-                float dx = cg.x - g.x;
-                float dy = cg.y - g.y;
-                float dz = cg.z - g.z;
-                m0 += cm.x;
-                mx += cm.y + cm.x * dx;
-                my += cm.z + cm.x * dy;
-                mz += cm.w + cm.x * dz;
-            }
+            m2m_children(geom, mp, l, g, m0, mx, my, mz);
         }
         mp[nidx] = make_float4(m0, mx, my, mz);
     }

@@ -194,8 +194,22 @@ __global__ __launch_bounds__(kBlock) void tree_mask_kernel(const uint32_t* __res
          c += (size_t)gridDim.x * kBlock) {
         const uint32_t p = parent[c];
         if (p == PS_NONE || (c > 0 && parent[c - 1] == p)) continue;
+        // the next seven nodes' parents and octants are loaded at once; the
+        // run ends at the first node of another parent
+        uint32_t pq[7], oq[7];
+#pragma unroll
+        for (uint32_t k = 0; k < 7; ++k) {
+            const size_t q = c + 1 + k;
+            pq[k] = q < m ? parent[q] : PS_NONE;
+            oq[k] = q < m ? octant[q] : 0u;
+        }
         uint32_t mask = 1u << octant[c];
-        for (size_t k = c + 1; k < m && k < c + 8 && parent[k] == p; ++k) mask |= 1u << octant[k];
+        bool same = true;
+#pragma unroll
+        for (uint32_t k = 0; k < 7; ++k) {
+            same = same && pq[k] == p;
+            mask |= same ? 1u << oq[k] : 0u;
+        }
         link32[2 * (size_t)p + 1] = mask;
     }
 }

@@ -430,7 +430,7 @@ def main(argv=None):
     dom = max(kern, key=lambda k: kern[k]["ms"])
     ach = kern[dom]["flop"] / (kern[dom]["ms"] * 1e-3) / 1e12
     traffic = measured_traffic(a.workload, dom)
-    roof = dict(kernel=dom, bound="mfma", achieved=round(ach, 4), peak=FP32_PEAK_TFLOPS,
+    roof = dict(kernel=dom, bound="valu", achieved=round(ach, 4), peak=FP32_PEAK_TFLOPS,
                 unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 6),
                 traffic=None if traffic is None else round(traffic),
                 algorithmic=("41 FLOP per M2L pair" if dom == "trav_step"

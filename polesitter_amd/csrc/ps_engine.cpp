@@ -85,26 +85,6 @@ hipError_t grow(DevBuf& b, size_t need, hipStream_t s) {
     return hipSuccess;
 }
 
-// grow keeping the first `keep` bytes: for buffers that accumulate across
-// launches (the P2P source CSR fills over several traversal iterations)
-hipError_t grow_keep(DevBuf& b, size_t need, size_t keep, hipStream_t s) {
-    if (need <= b.bytes && b.p) return hipSuccess;
-    DevBuf nb;
-    hipError_t e = grow(nb, need, s);
-    if (e != hipSuccess) return e;
-    if (b.p && keep) {
-        e = hipMemcpyAsync(nb.p, b.p, std::min(keep, b.bytes), hipMemcpyDeviceToDevice, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            (void)hipFree(nb.p);
-            return e;
-        }
-    }
-    if (b.p) (void)hipFree(b.p);
-    b = nb;
-    return hipSuccess;
-}
-
 void release(DevBuf& b) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
@@ -629,10 +609,13 @@ int batch_ensure(ps_hip_batch* b, size_t n, size_t cap) {
     return PS_HIP_OK;
 }
 
-// The batch tick, launches only: per-system prepare, sort by (system,
-// code), forest build, upward, traversal (all PS_MAX_DEPTH_ITERS iterations;
-// the unused ones find no entries), L2L, P2P + L2P.
-int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
+// The batch tick up to the multipoles, launches only: per-system prepare,
+// sort by (system, code), forest build (locals and P2P list lengths
+// cleared), upward.  It sorts the caller's arrays in place, so it runs once
+// per tick: in the AVX2 mode a particle's quantisation depends on its
+// position in its system (the first 8 floor(N_s/8) are rounded), and a
+// second prepare over the sorted arrays would give other codes.
+int batch_run_front(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
     ps_hip_engine* e = b->e;
     hipStream_t s = e->s;
     const size_t n = e->n;
@@ -657,7 +640,20 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
     ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
                        e->totals(), cap, s);
     ps_launch_upward_dev(ls, e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), s);
+    return map_err(hipGetLastError());
+}
 
+// The batch tick from the traversal on (all PS_MAX_DEPTH_ITERS iterations;
+// the unused ones find no entries), L2L, P2P + L2P, the correction.  Needs
+// loc and p2p_len cleared (batch_run_front, or batch_clear_far before a
+// re-run after an arena overflow).
+int batch_run_back(ps_hip_batch* b, uint32_t n_sys) {
+    ps_hip_engine* e = b->e;
+    hipStream_t s = e->s;
+    const size_t n = e->n;
+    const size_t cap = b->node_cap;
+    uint32_t* sys = reinterpret_cast<uint32_t*>(b->sys_b.p);
+    uint32_t* ls = e->small_u32();
     float theta2;
     {
         volatile float tv = e->theta;  // fl(theta * theta), H:1064
@@ -702,6 +698,20 @@ int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
                             e->leaf_of(), e->geom(), e->mp(), e->loc(), e->parent(), e->fx, e->fy,
                             e->fz, s);
     return map_err(hipGetLastError());
+}
+
+// The whole batch tick (launches only).
+int batch_run(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
+    PS_RC(batch_run_front(b, n_sys, morton_mode));
+    return batch_run_back(b, n_sys);
+}
+
+// Clears what the traversal accumulates (node locals, P2P list lengths) for
+// a re-run of batch_run_back over the same sorted state and forest.
+void batch_clear_far(ps_hip_batch* b) {
+    ps_hip_engine* e = b->e;
+    ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
+                       e->totals(), b->node_cap, e->s);
 }
 
 struct PrepWS {
@@ -886,6 +896,9 @@ int ps_hip_mlp_features_at(ps_hip_engine_t* e, const uint32_t* idx, size_t rows,
 int ps_hip_mlp_forward(ps_hip_engine_t* e, const float* params, const float* feat, size_t n,
                        float* out3) {
     if (!e || !params || !feat || !out3) return PS_HIP_EINVAL;
+    // the kernels read and write these directly: host memory would fault
+    if (!is_device_ptr(params) || !is_device_ptr(feat) || !is_device_ptr(out3))
+        return PS_HIP_EINVAL;
     ps_launch_mlp_forward(params, feat, n, out3, e->s);
     PS_CK(hipGetLastError());
     PS_CK(hipStreamSynchronize(e->s));
@@ -895,6 +908,9 @@ int ps_hip_mlp_forward(ps_hip_engine_t* e, const float* params, const float* fea
 int ps_hip_mlp_grad(ps_hip_engine_t* e, const float* params, const float* feat,
                     const float* target, size_t n, float* grad, float* loss_host) {
     if (!e || !params || !feat || !target || !grad || n == 0) return PS_HIP_EINVAL;
+    if (!is_device_ptr(params) || !is_device_ptr(feat) || !is_device_ptr(target) ||
+        !is_device_ptr(grad))
+        return PS_HIP_EINVAL;
     PS_CK(grow(e->mlp_tmp, 4 * (ps_mlp_grad_tmp_floats(n) + 1), e->s));
     float* tmp = reinterpret_cast<float*>(e->mlp_tmp.p);
     float* loss_dev = tmp + ps_mlp_grad_tmp_floats(n);
@@ -910,6 +926,9 @@ int ps_hip_mlp_grad(ps_hip_engine_t* e, const float* params, const float* feat,
 int ps_hip_mlp_adam(ps_hip_engine_t* e, float* params, const float* grad, float* m1, float* m2,
                     float lr, float beta1, float beta2, float eps, int step) {
     if (!e || !params || !grad || !m1 || !m2 || step < 1) return PS_HIP_EINVAL;
+    if (!is_device_ptr(params) || !is_device_ptr(grad) || !is_device_ptr(m1) ||
+        !is_device_ptr(m2))
+        return PS_HIP_EINVAL;
     ps_launch_mlp_adam(params, grad, m1, m2, lr, beta1, beta2, eps, step, e->s);
     PS_CK(hipGetLastError());
     return PS_HIP_OK;
@@ -1023,14 +1042,16 @@ int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z, float* ma
         PS_CK(hipStreamSynchronize(s));
         if (!hst.overflow) break;
         // an arena was too small: grow past the need and run the tick again
-        // directly (the inputs are unchanged: the sort is idempotent)
+        // from the traversal (the sorted arrays and the forest stand; the
+        // prepare and the sort must not run twice, see batch_run_front)
         b->drop_graph();
         if (attempt >= 4 || hst.arena_ctr > 0xFFFFFFFFull || hst.mask_ctr > 0xFFFFFFFFull)
             return PS_HIP_EOOM;
         PS_CK(grow(e->arena_b, std::max<size_t>(2 * e->arena_b.bytes, 5 * hst.arena_ctr), s));
         PS_CK(grow(e->masks_b, std::max<size_t>(2 * e->masks_b.bytes, 10 * hst.mask_ctr), s));
         PS_CK(hipEventRecord(b->ev0, s));
-        PS_RC(batch_run(b, nsys, morton_mode));
+        batch_clear_far(b);
+        PS_RC(batch_run_back(b, nsys));
         PS_CK(hipEventRecord(b->ev1, s));
     }
     float ms = 0.f;

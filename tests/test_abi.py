@@ -109,3 +109,34 @@ def test_invalid_arguments_without_device():
 
 def test_dropin_program_is_built():
     assert os.access(os.path.join(api.LIBDIR, "dropin_tick"), os.X_OK)
+
+
+def integration_c_blocks():
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    return re.findall(r"^```c\n(.*?)^```", text, flags=re.S | re.M)
+
+
+def test_integration_examples_compile_as_c99(tmp_path):
+    # every C example a maintainer would copy from INTEGRATION.md is a whole
+    # translation unit that compiles against the drop-in headers
+    blocks = integration_c_blocks()
+    assert len(blocks) >= 2
+    for k, code in enumerate(blocks):
+        src = tmp_path / f"example{k}.c"
+        src.write_text(code)
+        r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                            f"-I{INCLUDE}", "-c", str(src), "-o", str(tmp_path / f"e{k}.o")],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, (k, r.stderr)
+
+
+def test_integration_config_initialiser_uses_reference_field_order():
+    # ps_config_t is {buff, buff_size, max_particles, theta, thrd_cnt}
+    # (H:304-310); every positional initialiser in the documentation starts
+    # with the buffer
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    inits = re.findall(r"ps_config_t\s+\w+\s*=\s*\{([^}]*)\}", text)
+    assert inits
+    for body in inits:
+        first = body.split(",")[0].strip()
+        assert "buff" in first or first.startswith("b"), body

@@ -203,3 +203,32 @@ def test_stats_modes_time_what_they_say():
     for k in ("ms_sort", "ms_p2p", "ms_total", "ms_trav_steps"):
         assert off[k] == 0.0, k
     assert full["m2l_pairs"] == light["m2l_pairs"] == off["m2l_pairs"] > 0
+
+
+def test_mlp_entry_points_reject_host_pointers():
+    # the training kernels read and write their arguments directly; a host
+    # array would fault the device, so each entry point checks every pointer
+    # first and returns PS_EINVAL (no kernel is launched)
+    L = api.hip_lib()
+    P = 10115
+    with api.Solver(64) as s:
+        e = s.ctx.engine
+        dp, dg, d1, d2 = (api.DeviceArray(P, np.float32) for _ in range(4))
+        feat = api.DeviceArray(4 * 24, np.float32)
+        out = api.DeviceArray(4 * 3, np.float32)
+        hp = np.zeros(P, np.float32)
+        hf = np.zeros(4 * 24, np.float32)
+        ho = np.zeros(4 * 3, np.float32)
+        assert L.ps_hip_mlp_forward(e, hp.ctypes.data, feat.ptr, 4, out.ptr) == api.PS_EINVAL
+        assert L.ps_hip_mlp_forward(e, dp.ptr, hf.ctypes.data, 4, out.ptr) == api.PS_EINVAL
+        assert L.ps_hip_mlp_forward(e, dp.ptr, feat.ptr, 4, ho.ctypes.data) == api.PS_EINVAL
+        assert L.ps_hip_mlp_grad(e, dp.ptr, feat.ptr, ho.ctypes.data, 4, dg.ptr,
+                                 None) == api.PS_EINVAL
+        assert L.ps_hip_mlp_grad(e, dp.ptr, feat.ptr, out.ptr, 4, hp.ctypes.data,
+                                 None) == api.PS_EINVAL
+        assert L.ps_hip_mlp_adam(e, dp.ptr, dg.ptr, hp.ctypes.data, d2.ptr, 1e-3, 0.9, 0.999,
+                                 1e-8, 1) == api.PS_EINVAL
+        assert L.ps_hip_mlp_adam(e, hp.ctypes.data, dg.ptr, d1.ptr, d2.ptr, 1e-3, 0.9, 0.999,
+                                 1e-8, 1) == api.PS_EINVAL
+        # the device arrays themselves are accepted
+        api.check("forward", L.ps_hip_mlp_forward(e, dp.ptr, feat.ptr, 4, out.ptr))

@@ -155,3 +155,35 @@ def test_batch_rejects_bad_layouts():
         rc = L.ps_hip_batch_tick(b.h, b.x.ptr, b.y.ptr, b.z.ptr, b.m.ptr, b.fx.ptr, b.fy.ptr,
                                  b.fz.ptr, b.id.ptr, b.codes.ptr, o.ctypes.data, 2, 0)
         assert rc == api.PS_EINVAL
+
+
+@pytest.mark.parametrize("mode", [api.MORTON_TRUNC, api.MORTON_AVX2])
+def test_batch_arena_overflow_equals_single_ticks(mode):
+    # a list arena of one word per particle overflows in the first tick: the
+    # arenas grow and the tick runs again from the traversal over the sorted
+    # arrays and the forest it already has.  Sizes that are not multiples of
+    # 8 make the AVX2 quantisation depend on each particle's position in its
+    # system, so running the prepare and the sort a second time would change
+    # codes; the result must equal single ticks bit for bit, and so must the
+    # replay of the graph recorded after the overflow.
+    sizes = [333, 5001, 20003, 7, 4099, 64]
+    systems = [W.plummer(k, seed=90 + j, mass=1.0 / k) if j % 2 else
+               W.uniform_cube(k, seed=90 + j, mass=1.0 / k) for j, k in enumerate(sizes)]
+    n = sum(sizes)
+    with api.Batch(n, len(systems), theta=2.0, graph=True) as b:
+        b.set_option(api.OPT_TRAV_LANES, LANES)
+        b.set_option(api.OPT_ARENA_WORDS, 1)
+        b.load(systems)
+        b.tick(mode)
+        res = b.result()
+        b.restore()
+        b.tick(mode)
+        assert b.stats()["used_graph"] == 1
+        rep = b.result()
+    for f in FIELDS:
+        assert np.array_equal(rep[f], res[f]), f
+    for k, sys_ in enumerate(systems):
+        g = b.system(res, k)
+        o = single(sys_, mode, api.TRAV_MT)
+        for f in FIELDS:
+            assert np.array_equal(g[f], o[f]), (k, f)

@@ -120,6 +120,22 @@ int ps_hip_prepare(const float* x, const float* y, const float* z, size_t n,
                    uint32_t* codes, float* out_min, float* out_max,
                    float* out_range, int morton_mode);
 
+/* No reference counterpart: ps_hip_prepare for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  The same results as ps_hip_prepare, bit for bit.
+ * It allocates nothing: its scratch (the bounding-box partials and the four
+ * cube parameters, PS_HIP_PREPARE_NOALLOC_SCRATCH bytes) is a static device
+ * array that each device holds until the process ends.  x, y, z and codes
+ * must be device arrays (host arrays: PS_HIP_EINVAL); n at most
+ * PS_HIP_PREPARE_NOALLOC_MAX (larger: PS_HIP_EINVAL).  Runs on the default
+ * stream and synchronises it.  Every call on one device shares that device's
+ * scratch: calls that overlap on one device race. */
+#define PS_HIP_PREPARE_NOALLOC_SCRATCH 8448
+#define PS_HIP_PREPARE_NOALLOC_MAX 4294967295u
+int ps_hip_prepare_noalloc(const float* x, const float* y, const float* z,
+                           size_t n, uint32_t* codes, float* out_min,
+                           float* out_max, float* out_range, int morton_mode);
+
 /* ps_calc_forces, H:1840-1909: sorts x, y, z, mass, id and codes in place
  * by code and writes fx, fy, fz in the sorted order. */
 int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z,

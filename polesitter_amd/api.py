@@ -103,6 +103,8 @@ def hip_lib():
             "ps_hip_engine_set_option": ([_vp, C.c_int, C.c_double], C.c_int),
             "ps_hip_engine_get_stats": ([_vp, C.POINTER(PsStats)], C.c_int),
             "ps_hip_prepare": ([_vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, C.c_int], C.c_int),
+            "ps_hip_prepare_noalloc": ([_vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, C.c_int],
+                                       C.c_int),
             "ps_hip_calc_forces": ([_vp] * 9 + [_sz, _vp, _f, _f, _f, _f], C.c_int),
             "ps_hip_engine_export_tree": ([_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
             "ps_hip_debug_bind": ([_vp] * 9 + [_sz, _vp, _f, _f, _f, _f], C.c_int),

@@ -849,6 +849,8 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
     }
 }
 
+static_assert(PS_HIP_PREPARE_NOALLOC_SCRATCH == PS_PREP_SCRATCH_BYTES,
+              "the documented scratch size is the array's");
 static_assert(PS_HIP_MLP_PARAMS == PS_MLP_PARAMS && PS_HIP_MLP_IN == PS_MLP_IN,
               "the public and kernel layouts of the correction agree");
 
@@ -1141,6 +1143,24 @@ int ps_hip_prepare(const float* x, const float* y, const float* z, size_t n, uin
     if (out_min) *out_min = w->h_out[0];
     if (out_max) *out_max = w->h_out[1];
     if (out_range) *out_range = w->h_out[2];
+    return PS_HIP_OK;
+}
+
+int ps_hip_prepare_noalloc(const float* x, const float* y, const float* z, size_t n,
+                           uint32_t* codes, float* out_min, float* out_max, float* out_range,
+                           int morton_mode) {
+    if (!x || !y || !z || !codes || n == 0) return PS_HIP_EINVAL;  // H:1914
+    if (n > PS_HIP_PREPARE_NOALLOC_MAX) return PS_HIP_EINVAL;
+    if (morton_mode != PS_HIP_MORTON_TRUNC && morton_mode != PS_HIP_MORTON_AVX2)
+        return PS_HIP_EINVAL;
+    // no staging: host arrays would need device buffers
+    if (!is_device_ptr(x) || !is_device_ptr(y) || !is_device_ptr(z) || !is_device_ptr(codes))
+        return PS_HIP_EINVAL;
+    float out3[3] = {0.f, 0.f, 0.f};
+    PS_CK(ps_prepare_noalloc(x, y, z, n, codes, morton_mode, out3, nullptr));
+    if (out_min) *out_min = out3[0];
+    if (out_max) *out_max = out3[1];
+    if (out_range) *out_range = out3[2];
     return PS_HIP_OK;
 }
 

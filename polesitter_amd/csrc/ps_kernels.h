@@ -45,6 +45,15 @@ void ps_launch_bbox(const float* x, const float* y, const float* z, size_t n, fl
 // mode 1: AVX2 rounding for the first 8*floor(n/8), scalar tail (H:2040-2072)
 void ps_launch_morton(const float* x, const float* y, const float* z, size_t n,
                       const float* prm, uint32_t* codes, int mode, hipStream_t s);
+// The same two passes with no allocation: the scratch is a static device
+// array of PS_PREP_SCRATCH_BYTES in ps_prepare.hip (its per-device address
+// from ps_prepare_scratch).  x, y, z, codes: device arrays; out3 (host) =
+// min, max, range.  Synchronises s.  Calls that overlap on one device share
+// the array and race.
+#define PS_PREP_SCRATCH_BYTES 8448
+hipError_t ps_prepare_scratch(void** out);
+hipError_t ps_prepare_noalloc(const float* x, const float* y, const float* z, size_t n,
+                              uint32_t* codes, int mode, float* out3, hipStream_t s);
 // Many systems in one set of arrays, system k = particles [off[k], off[k+1])
 // (device array, every system non-empty): per system its cube prm[k] = (min,
 // max, range, 1023/range) and root cell root[k] = (min + range/2 (x3),

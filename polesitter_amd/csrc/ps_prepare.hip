@@ -6,6 +6,8 @@
 // of codes written.
 #include <float.h>
 
+#include <mutex>
+
 #include "ps_device.h"
 #include "ps_kernels.h"
 
@@ -225,4 +227,57 @@ void ps_launch_morton(const float* x, const float* y, const float* z, size_t n,
     size_t blocks = (n + kBlock - 1) / kBlock;
     morton_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(x, y, z, n, prm, codes, mode,
                                                       n - (n % 8u));
+}
+
+// ---- the prepare pass without allocation ----------------------------------
+// Its scratch (the bbox partials of at most kMaxParts blocks, then the four
+// parameters min, max, range, 1023/range) lives in this file-scope device
+// array: every device the library runs on holds one copy until the process
+// ends.  Host code never passes the array's name to a kernel (on the host it
+// is only a stand-in for the symbol): each device's address comes from
+// hipGetSymbolAddress, looked up once per device with that device current.
+static_assert(PS_PREP_SCRATCH_BYTES >= kMaxParts * sizeof(float2) + 4 * sizeof(float),
+              "the no-alloc scratch holds the partials and the parameters");
+static __device__ __align__(256) unsigned char scratch[PS_PREP_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {nullptr};
+}  // namespace
+
+hipError_t ps_prepare_scratch(void** out) {
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+hipError_t ps_prepare_noalloc(const float* x, const float* y, const float* z, size_t n,
+                              uint32_t* codes, int mode, float* out3, hipStream_t s) {
+    void* ptr = nullptr;
+    hipError_t e = ps_prepare_scratch(&ptr);
+    if (e != hipSuccess) return e;
+    float2* part = reinterpret_cast<float2*>(ptr);  // bbox partials first
+    float* prm = reinterpret_cast<float*>(reinterpret_cast<char*>(ptr) +
+                                          kMaxParts * sizeof(float2));
+    // every partial the final pass reads is written by this call's first
+    // pass, and the parameters by its final pass: nothing needs clearing
+    ps_launch_bbox(x, y, z, n, part, prm, s);
+    ps_launch_morton(x, y, z, n, prm, codes, mode, s);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(out3, prm, 3 * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
 }

@@ -15,12 +15,17 @@ so every tick re-sorts and rebuilds.
 Prints one JSON line.  `value` = N * K / (wall time of K steps), bracketed
 by device synchronisation.  `roofline` is the dominant kernel's algorithmic
 work per launch over its launch time, both from HIP events recorded on the
-engine's stream inside the timed steps.  `cpu_baseline` times the
-reference's own PS_MULTITHREADING CPU path (oracle/_ref, shipped-style
-flags) on this host's cores on a bounded sample of the same workload.
-`fp64_*` report the force error against an FP64 direct sum on 4096 seeded
-targets matched through id[] (tests/accuracy.c:505-531), for the GPU tick
-and for the reference on the same state.
+engine's stream inside the timed steps; `roofline.step` / `tick_frac` put
+the whole step against the sum of its stages' rooflines.  `cpu_baseline`
+times the reference's own PS_MULTITHREADING CPU path (oracle/_ref,
+shipped-style flags) on this host at 1 thread, at the physical cores the
+process may use and at every usable logical CPU (the host's facts are in
+`cpu_baseline.host`), on a bounded sample of the same workload.  `fp64_*`
+report the force error against an FP64 direct sum on 4096 seeded targets
+matched through id[] (tests/accuracy.c:505-531), for the GPU tick and for
+the reference on the same state.  The default run (plummer1m) adds the
+metric's second size, N = 16,777,216 two Plummers, as `sizes.twoplummer16m`
+with the same fields.
 """
 from __future__ import annotations
 
@@ -99,40 +104,103 @@ def fp64_error(n, pos_sorted_forces, ids, ref_fn, seed=4096):
     return float(np.sqrt(e2.sum() / r2.sum())), float(np.sqrt(e2[ok] / r2[ok]).max())
 
 
-def cpu_baseline(n, x, y, z, m, theta, ticks, gpu_ref_fn):
-    """The reference CPU path (oracle/_ref) on this host, bounded sample."""
+def host_cpus():
+    """What this host gives the process: logical CPUs, the affinity set, a
+    cgroup CPU quota, the physical cores inside the affinity set and the CPU
+    model.  `usable` = the affinity set, capped by the quota."""
+    info = dict(nproc=os.cpu_count())
+    aff = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+        list(range(os.cpu_count() or 1))
+    info["affinity"] = len(aff)
+    quota = None
+    try:
+        q = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q and q[0] != "max":
+            quota = float(q[0]) / float(q[1])
+    except (OSError, ValueError, IndexError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    cores = set()
+    for c in aff:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            cores.add((open(base + "physical_package_id").read().strip(),
+                       open(base + "core_id").read().strip()))
+        except OSError:
+            pass
+    info["physical_cores_in_affinity"] = len(cores) or None
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info["model"] = model
+    usable = len(aff)
+    if quota:
+        usable = max(1, min(usable, int(quota)))
+    info["usable"] = usable
+    return info
+
+
+def cpu_thread_counts(info):
+    """Thread counts the reference is timed at: 1, the physical cores the
+    process may use, and every usable logical CPU."""
+    usable = info["usable"]
+    phys = info["physical_cores_in_affinity"] or usable
+    return sorted({1, max(1, min(phys, usable)), usable})
+
+
+def cpu_baseline(n, x, y, z, m, theta, ticks, gpu_ref_fn, threads, info, err_threads=None):
+    """The reference CPU path (oracle/_ref, shipped-style flags,
+    PS_MULTITHREADING) on this host: one warm-up and `ticks` timed ticks per
+    thread count, each from the workload's initial state; value = the
+    fastest thread count's median.  The FP64 error of the reference on the
+    same state is taken at err_threads (default: the fastest count)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    thr = max(1, min(16, cores or 1))
-    if O.ref_available("shipped_mt"):
-        build, kind = "shipped_mt", "reference"
+    if not O.ref_available("shipped_mt"):
+        # no compiled reference: the single-threaded C restatement
         t0 = time.time()
+        O.tick(x, y, z, m, theta=theta)
+        tick_s = time.time() - t0
+        return dict(value=n / tick_s, unit="particles/s", cores=1, kind="port",
+                    sample="1 tick of the full workload, oracle/ps_oracle.c, 1 thread",
+                    tick_s=tick_s, host=info)
+    per = {}
+    t_all = time.time()
+    for thr in threads:
+        # the reference caps a context at PS_MAX_THRDS + 1 = 33 threads
+        # (H:384-386, H:1709-1714); a wider build lifts the cap
+        build = "shipped_mt" if thr <= 33 else "shipped_mt_wide"
+        if not O.ref_available(build):
+            continue
         secs = []
-        for k in range(ticks + 1):  # one call per tick, each from the initial state
+        for k in range(ticks + 1):
             secs.append(float(O.ref_time_ticks(x, y, z, m, theta=theta, thrd_cnt=thr,
                                                n_ticks=1, build=build)[0]))
-            log(f"bench: reference CPU tick {k + 1}/{ticks + 1}: {secs[-1]:.3f} s")
-        wall = time.time() - t0
-        tick_s = float(np.median(secs[1:]))
-        # the reference's error on the same initial state
-        r = O.ref_tick(x, y, z, m, theta=theta, thrd_cnt=thr, build=build, dump=False)
-        log("bench: reference CPU tick for the FP64 error done")
-        f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
-        err = fp64_error(n, f, r["id"], lambda pos: gpu_ref_fn(r, pos))
-        sample = (f"{ticks} ticks (+1 warm-up) of the full workload from its initial state, "
-                  f"median; "
-                  f"src/polesitter.h built -O3 -ffast-math -march=x86-64-v3 -DPS_MULTITHREADING, "
-                  f"thrd_cnt={thr}; {wall:.1f} s wall on {thr} threads")
-        return dict(value=n / tick_s, unit="particles/s", cores=thr, kind=kind, sample=sample,
-                    tick_s=tick_s, fp64_rms_err=err[0], fp64_max_err=err[1])
-    # fallback: the single-threaded C restatement
-    t0 = time.time()
-    O.tick(x, y, z, m, theta=theta)
-    tick_s = time.time() - t0
-    return dict(value=n / tick_s, unit="particles/s", cores=1, kind="port",
-                sample="1 tick of the full workload, oracle/ps_oracle.c, 1 thread",
-                tick_s=tick_s)
+            log(f"bench: reference CPU tick {k + 1}/{ticks + 1} on {thr} threads: "
+                f"{secs[-1]:.3f} s")
+        per[thr] = dict(tick_s=float(np.median(secs[1:])), build=build)
+    best = min(per, key=lambda t: per[t]["tick_s"])
+    et = err_threads if err_threads in per else best
+    r = O.ref_tick(x, y, z, m, theta=theta, thrd_cnt=et, build=per[et]["build"], dump=False)
+    log(f"bench: reference CPU tick for the FP64 error done ({et} threads)")
+    f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
+    err = fp64_error(n, f, r["id"], lambda pos: gpu_ref_fn(r, pos))
+    wall = time.time() - t_all
+    sample = (f"{ticks} ticks (+1 warm-up) of the full workload from its initial state per "
+              f"thread count, median; src/polesitter.h built -O3 -ffast-math -march=x86-64-v3 "
+              f"-DPS_MULTITHREADING (PS_MAX_THRDS=255 above 33 threads); thread counts "
+              f"{sorted(per)} on {info['model']}; {wall:.1f} s wall")
+    return dict(value=n / per[best]["tick_s"], unit="particles/s", cores=best, kind="reference",
+                sample=sample, tick_s=per[best]["tick_s"],
+                by_threads={str(t): dict(tick_s=round(v["tick_s"], 5),
+                                         particles_per_s=round(n / v["tick_s"], 1))
+                            for t, v in sorted(per.items())},
+                fp64_rms_err=err[0], fp64_max_err=err[1], fp64_err_threads=et, host=info)
 
 
 def _timed_steps(s, n, v, steps, warmup, mmode):
@@ -302,6 +370,206 @@ def bench_ensemble(a):
     return 0
 
 
+def step_roofline(n, nodes, m2l, p2p, stages_ms, ms_step):
+    """The whole step against its roofline: t_roof = sum over the stages of
+    max(algorithmic bytes / 8 TB/s, algorithmic FLOP / 157.3 TFLOP/s), with
+    N particles, M nodes, L M2L pairs and P P2P particle pairs; frac =
+    t_roof / the measured step time.  L2P is fused into the P2P kernel, so
+    its 16 B per particle are priced with P2P; the frame step (gather +
+    Euler + id reset) is part of the timed step and of t_roof."""
+    N, M, L, P = float(n), float(nodes), float(m2l), float(p2p)
+    work = (("prepare", 16 * N, 0.0, None), ("sort", 48 * N, 0.0, "sort"),
+            ("build", 4 * N + 32 * M, 0.0, "build"), ("upward", 16 * N + 32 * M, 0.0, "upward"),
+            ("m2l", 0.0, FLOP_M2L * L, "traverse"), ("l2l", 24 * M, 0.0, "l2l"),
+            ("p2p_l2p", 16 * N, FLOP_P2P * P, "p2p"), ("frame", 68 * N, 0.0, None))
+    stages = {}
+    total = 0.0
+    for name, by, fl, key in work:
+        roof = max(by / (HBM_PEAK_GBS * 1e9), fl / (FP32_PEAK_TFLOPS * 1e12)) * 1e3
+        total += roof
+        d = dict(bytes=by, flop=fl, roof_ms=round(roof, 5),
+                 bound="hbm" if by / (HBM_PEAK_GBS * 1e9) >= fl / (FP32_PEAK_TFLOPS * 1e12)
+                 else "valu")
+        if key and stages_ms.get(key):
+            d["measured_ms"] = round(stages_ms[key], 4)
+            d["frac"] = round(roof / stages_ms[key], 4)
+        stages[name] = d
+    return dict(t_roof_ms=round(total, 5), t_step_ms=round(ms_step, 4),
+                frac=round(total / ms_step, 5), stages=stages,
+                peaks="8 TB/s HBM, 157.3 TFLOP/s FP32 (published MI355X)")
+
+
+def measure_plain(a, name, steps, warmup, cpu_ticks, host_ticks, do_cpu, cpu_threads=None):
+    """One plain workload: the timed steps, the dominant kernel's roofline,
+    the whole step's roofline, the FP64 error with plain and compensated
+    accumulation, the host-pointer rate and the reference CPU baseline."""
+    from polesitter_amd import api
+    n, x, y, z, m, v, desc = workload(name)
+    log(f"bench: {api.device_name()} workload {name}: {desc}")
+    L = api.hip_lib()
+    s = api.Solver(n, theta=a.theta, kahan=a.kahan, stats=True)
+    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_LANES, a.trav_lanes))
+    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_P2P_BIG, a.p2p_big))
+    if a.trav_blocks:
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_BLOCKS, a.trav_blocks))
+    if a.trav_segments:
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_SEGMENTS,
+                                                     a.trav_segments))
+    s.load(x, y, z, m)
+    vel = [api.DeviceArray.from_numpy(vv) for vv in v]
+    tmp = api.DeviceArray(3 * n, np.float32)
+    dt = 1e-3
+    mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
+
+    def step():  # ids are 0..n-1 on entry (load, or the previous frame step)
+        s.tick(mmode)
+        s.frame_step(vel, tmp, dt)
+        return s.stats()
+
+    # error of the first tick (initial state), untimed
+    rec = {}
+    if not a.no_error:
+        s.reset_ids()
+        s.tick(mmode)
+        r = s.result()
+        f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
+        err = fp64_error(n, f, r["id"], s.direct_f64)
+        del r, f
+        log(f"bench: GPU FP64 error on 4096 targets rms={err[0]:.3e} max={err[1]:.3e}")
+        # the same tick with the other accumulation (compensated vs plain FP32)
+        # in the far field and P2P sums, on the same targets
+        s.load(x, y, z, m)
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_KAHAN, 0 if a.kahan else 1))
+        s.reset_ids()
+        s.tick(mmode)
+        r2 = s.result()
+        f2 = np.stack([r2["fx"], r2["fy"], r2["fz"]], 1)
+        err2 = fp64_error(n, f2, r2["id"], s.direct_f64)
+        del r2, f2
+        ms_other = s.stats()["ms_total"]
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_KAHAN, 1 if a.kahan else 0))
+        s.reset_ids()
+        s.tick(mmode)
+        ms_this = s.stats()["ms_total"]
+        this, other = ("kahan", "plain") if a.kahan else ("plain", "kahan")
+        rec["fp64_rms_err"], rec["fp64_max_err"] = err
+        rec["accumulation"] = {this: dict(fp64_rms_err=err[0], fp64_max_err=err[1],
+                                          ms_tick=round(ms_this, 4)),
+                               other: dict(fp64_rms_err=err2[0], fp64_max_err=err2[1],
+                                           ms_tick=round(ms_other, 4))}
+        log(f"bench: {other} accumulation: FP64 rms={err2[0]:.6e} max={err2[1]:.4e}")
+        s.load(x, y, z, m)
+
+    # warm-up steps count the P2P pairs (extra reductions); the timed steps
+    # record four events each (the traversal and the P2P stage, for the
+    # kernel roofline), which takes its pair counts from the last warm-up
+    # step; the stage breakdown comes from a few more steps after the timed
+    # ones, with every stage event recorded
+    counted = None
+    warmup = max(warmup, 1)
+    for _ in range(warmup):
+        counted = step()
+    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 3))
+    api.check("sync", L.ps_hip_synchronize())
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(step())
+    api.check("sync", L.ps_hip_synchronize())
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / steps
+    value = n * steps / wall
+    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 2))
+    staged = [step() for _ in range(min(steps, 5))]
+
+    mean = lambda k: float(np.mean([st[k] for st in stats]))
+    mean_staged = lambda k: float(np.mean([st[k] for st in staged]))
+    kern = {
+        "trav_step": dict(ms=mean("ms_trav_steps"), flop=FLOP_M2L * mean("m2l_pairs")),
+        "p2p": dict(ms=mean("ms_p2p"), flop=FLOP_P2P * counted["p2p_pairs"]),
+    }
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    ach = kern[dom]["flop"] / (kern[dom]["ms"] * 1e-3) / 1e12
+    traffic = measured_traffic(name, dom)
+    roof = dict(kernel=dom, bound="valu", achieved=round(ach, 4), peak=FP32_PEAK_TFLOPS,
+                unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 6),
+                traffic=None if traffic is None else round(traffic),
+                algorithmic=("41 FLOP per M2L pair" if dom == "trav_step"
+                             else "20 FLOP per particle pair"),
+                ms_per_tick=round(kern[dom]["ms"], 4),
+                launch="all of the family's dispatches in one tick (achieved = FLOP per tick / "
+                       "their summed duration from HIP events on the engine's stream)",
+                note="FP32 VALU work (no MFMA); peak is the published FP32 figure, the "
+                     "non-packed VALU ceiling is half of it; traffic = HBM-side bytes per tick "
+                     "(read requests by size + WRITE_SIZE, each scaled by a streaming kernel "
+                     "of known size) from profiles/traffic_" + name + ".md")
+    if traffic is not None:
+        roof["traffic_GBps"] = round(traffic / (kern[dom]["ms"] * 1e-3) / 1e9, 1)
+    stages_ms = {k: round(mean_staged("ms_" + k), 4) for k in
+                 ("sort", "build", "upward", "traverse", "l2l", "p2p", "total", "trav_steps")}
+    counts = dict(nodes=int(stats[-1]["n_nodes"]), m2l_pairs=int(mean("m2l_pairs")),
+                  p2p_pairs=int(counted["p2p_pairs"]), iterations=int(stats[-1]["iterations"]))
+    roof["step"] = step_roofline(n, counts["nodes"], counts["m2l_pairs"], counts["p2p_pairs"],
+                                 stages_ms, ms_step)
+    roof["tick_frac"] = roof["step"]["frac"]
+    out = dict(value=round(value, 1), ms_per_step=round(ms_step, 4), steps=steps, warmup=warmup,
+               config=dict(workload=name, description=desc, n_particles=n, theta=a.theta,
+                           traversal="MT depth-2 jobs (PS_MULTITHREADING)",
+                           step="tick + id-gather + semi-implicit Euler dt=1e-3",
+                           kahan=bool(a.kahan), trav_lanes=a.trav_lanes or "auto",
+                           p2p_big=a.p2p_big, morton=a.morton, pointers="device-resident"),
+               roofline=roof, stages_ms=stages_ms, counts=counts)
+    out.update(rec)
+    if host_ticks > 0:
+        # the same tick through host arrays: ps_calc_forces stages them through
+        # device buffers and copies the sorted state and forces back
+        hx, hy, hz, hm = (np.ascontiguousarray(t, np.float32).copy() for t in (x, y, z, m))
+        hf = [np.zeros(n, np.float32) for _ in range(3)]
+        hid = np.arange(n, dtype=np.uint32)
+        hcodes = np.zeros(n, np.uint32)
+        harrs = api.make_arrs(hx, hy, hz, hm, hf[0], hf[1], hf[2], hid, n)
+        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 0))
+
+        def host_tick():
+            hid[:] = np.arange(n, dtype=np.uint32)
+            rc, lo, _, rg = api.ps_prepare_particles(harrs, hcodes, mmode)
+            api.check("ps_prepare_particles", rc)
+            c = np.float32(np.float32(lo) + np.float32(rg) / np.float32(2.0))
+            api.check("ps_calc_forces", api.ps_calc_forces(s.ctx, harrs, hcodes, c, c, c,
+                                                           np.float32(rg) / np.float32(2.0)))
+
+        host_tick()
+        t0 = time.perf_counter()
+        for _ in range(host_ticks):
+            host_tick()
+        th = (time.perf_counter() - t0) / host_ticks
+        out["host_pointers"] = dict(value=round(n / th, 1), unit="particles/s",
+                                    ms_per_tick=round(1e3 * th, 4), ticks=host_ticks,
+                                    note="prepare + calc_forces on pageable host arrays "
+                                         "(uploads, tick, sorted state and forces copied "
+                                         "back); not the headline value")
+    s.close()
+    if do_cpu:
+        info = host_cpus()
+        threads = cpu_threads or cpu_thread_counts(info)
+
+        def ref_fn(r, pos):
+            # FP64 direct sum on the device over the reference's sorted state
+            with api.Solver(n) as t:
+                t.load(r["x"], r["y"], r["z"], r["m"])
+                return t.direct_f64(pos)
+        try:
+            cb = cpu_baseline(n, x, y, z, m, a.theta, cpu_ticks, ref_fn, threads, info)
+            out["cpu_baseline"] = cb
+            out["gpu_over_cpu"] = round(value / cb["value"], 2)
+            if "by_threads" in cb and "1" in cb["by_threads"]:
+                out["gpu_over_cpu_1thread"] = round(
+                    value / cb["by_threads"]["1"]["particles_per_s"], 1)
+        except Exception as exc:  # report, never hide, a failed baseline
+            out["cpu_baseline"] = dict(value=None, error=str(exc))
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,6 +592,8 @@ def main(argv=None):
     ap.add_argument("--cpu-ticks", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-16m", action="store_true",
+                    help="plummer1m: skip the 16M two-Plummer record (the metric's second size)")
     ap.add_argument("--mlp-cap", type=int, default=16,
                     help="mlp4m: leaf capacity of the corrected coarse setting")
     ap.add_argument("--mlp-theta", type=float, default=1.9,
@@ -339,173 +609,26 @@ def main(argv=None):
               f"(--gpus {a.gpus} requested)", file=sys.stderr)
         return 2
 
-    from polesitter_amd import api
-
     if a.workload == "ensemble512":
         return bench_ensemble(a)
     if a.workload == "mlp4m":
         return bench_mlp(a)
-    n, x, y, z, m, v, desc = workload(a.workload)
-    log(f"bench: {api.device_name()} workload {a.workload}: {desc}")
-    L = api.hip_lib()
-    s = api.Solver(n, theta=a.theta, kahan=a.kahan, stats=True)
-    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_LANES, a.trav_lanes))
-    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_P2P_BIG, a.p2p_big))
-    if a.trav_blocks:
-        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_BLOCKS, a.trav_blocks))
-    if a.trav_segments:
-        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_TRAV_SEGMENTS,
-                                                     a.trav_segments))
-    s.load(x, y, z, m)
-    vel = [api.DeviceArray.from_numpy(vv) for vv in v]
-    tmp = api.DeviceArray(3 * n, np.float32)
-    dt = 1e-3
-
-    mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
-
-    def step():  # ids are 0..n-1 on entry (load, or the previous frame step)
-        s.tick(mmode)
-        s.frame_step(vel, tmp, dt)
-        return s.stats()
-
-    # error of the first tick (initial state), untimed
-    err = None
-    if not a.no_error:
-        s.reset_ids()
-        s.tick(mmode)
-        r = s.result()
-        f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
-        err = fp64_error(n, f, r["id"], s.direct_f64)
-        log(f"bench: GPU FP64 error on 4096 targets rms={err[0]:.3e} max={err[1]:.3e}")
-        # the same tick with the other accumulation (compensated vs plain FP32)
-        # in the far field and P2P sums, on the same targets
-        s.load(x, y, z, m)
-        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_KAHAN, 0 if a.kahan else 1))
-        s.reset_ids()
-        s.tick(mmode)
-        r2 = s.result()
-        f2 = np.stack([r2["fx"], r2["fy"], r2["fz"]], 1)
-        err2 = fp64_error(n, f2, r2["id"], s.direct_f64)
-        ms_other = s.stats()["ms_total"]
-        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_KAHAN, 1 if a.kahan else 0))
-        s.reset_ids()
-        s.tick(mmode)
-        ms_this = s.stats()["ms_total"]
-        this, other = ("kahan", "plain") if a.kahan else ("plain", "kahan")
-        accumulation = {this: dict(fp64_rms_err=err[0], fp64_max_err=err[1],
-                                   ms_tick=round(ms_this, 4)),
-                        other: dict(fp64_rms_err=err2[0], fp64_max_err=err2[1],
-                                    ms_tick=round(ms_other, 4))}
-        log(f"bench: {other} accumulation: FP64 rms={err2[0]:.6e} max={err2[1]:.4e}")
-        s.load(x, y, z, m)
-
-    # warm-up ticks count the P2P pairs (extra reductions); the timed ticks
-    # record four events each (the traversal and the P2P stage, for the
-    # roofline), and the roofline takes its pair counts from the last warm-up
-    # tick; the stage breakdown comes from a few more ticks after the timed
-    # ones, with every stage event recorded
-    counted = None
-    a.warmup = max(a.warmup, 1)
-    for _ in range(a.warmup):
-        counted = step()
-    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 3))
-    api.check("sync", L.ps_hip_synchronize())
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        stats.append(step())
-    api.check("sync", L.ps_hip_synchronize())
-    wall = time.perf_counter() - t0
-    ms_step = 1e3 * wall / a.steps
-    value = n * a.steps / wall
-    api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 2))
-    staged = [step() for _ in range(min(a.steps, 5))]
-
-    mean = lambda k: float(np.mean([st[k] for st in stats]))
-    mean_staged = lambda k: float(np.mean([st[k] for st in staged]))
-    kern = {
-        "trav_step": dict(ms=mean("ms_trav_steps"), flop=FLOP_M2L * mean("m2l_pairs")),
-        "p2p": dict(ms=mean("ms_p2p"), flop=FLOP_P2P * counted["p2p_pairs"]),
-    }
-    dom = max(kern, key=lambda k: kern[k]["ms"])
-    ach = kern[dom]["flop"] / (kern[dom]["ms"] * 1e-3) / 1e12
-    traffic = measured_traffic(a.workload, dom)
-    roof = dict(kernel=dom, bound="valu", achieved=round(ach, 4), peak=FP32_PEAK_TFLOPS,
-                unit="TFLOP/s", frac=round(ach / FP32_PEAK_TFLOPS, 6),
-                traffic=None if traffic is None else round(traffic),
-                algorithmic=("41 FLOP per M2L pair" if dom == "trav_step"
-                             else "20 FLOP per particle pair"),
-                ms_per_tick=round(kern[dom]["ms"], 4),
-                launch="all of the family's dispatches in one tick (achieved = FLOP per tick / "
-                       "their summed duration from HIP events on the engine's stream)",
-                note="FP32 VALU work (no MFMA); peak is the published FP32 figure, the "
-                     "non-packed VALU ceiling is half of it; traffic = calibrated "
-                     "FETCH_SIZE + WRITE_SIZE bytes per tick from profiles/traffic_" +
-                     a.workload + ".md")
-    if traffic is not None:
-        roof["traffic_GBps"] = round(traffic / (kern[dom]["ms"] * 1e-3) / 1e9, 1)
-
-    line = dict(metric=METRIC, value=round(value, 1), unit="particles/s", n_gpus=1,
-                steps=a.steps, warmup=a.warmup, ms_per_step=round(ms_step, 4),
+    r = measure_plain(a, a.workload, a.steps, a.warmup, a.cpu_ticks, a.host_ticks,
+                      not a.no_cpu_baseline)
+    line = dict(metric=METRIC, value=r.pop("value"), unit="particles/s", n_gpus=1,
+                steps=r.pop("steps"), warmup=r.pop("warmup"), ms_per_step=r.pop("ms_per_step"),
                 higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
-                data="synthetic",
-                config=dict(workload=a.workload, description=desc, n_particles=n,
-                            theta=a.theta, traversal="MT depth-2 jobs (PS_MULTITHREADING)",
-                            step="tick + id-gather + semi-implicit Euler dt=1e-3",
-                            kahan=bool(a.kahan), trav_lanes=a.trav_lanes or "auto", p2p_big=a.p2p_big,
-                            morton=a.morton,
-                            pointers="device-resident"),
-                roofline=roof,
-                stages_ms={k: round(mean_staged("ms_" + k), 4) for k in
-                           ("sort", "build", "upward", "traverse", "l2l", "p2p", "total",
-                            "trav_steps")},
-                counts=dict(nodes=int(stats[-1]["n_nodes"]), m2l_pairs=int(mean("m2l_pairs")),
-                            p2p_pairs=int(counted["p2p_pairs"]),
-                            iterations=int(stats[-1]["iterations"])))
-    if err:
-        line["fp64_rms_err"], line["fp64_max_err"] = err
-        line["accumulation"] = accumulation
-    if a.host_ticks > 0:
-        # the same tick through host arrays: ps_calc_forces stages them through
-        # device buffers and copies the sorted state and forces back
-        hx, hy, hz, hm = (np.ascontiguousarray(t, np.float32).copy() for t in (x, y, z, m))
-        hf = [np.zeros(n, np.float32) for _ in range(3)]
-        hid = np.arange(n, dtype=np.uint32)
-        hcodes = np.zeros(n, np.uint32)
-        harrs = api.make_arrs(hx, hy, hz, hm, hf[0], hf[1], hf[2], hid, n)
-        api.check("ps_set_option", api.ps_set_option(s.ctx, api.OPT_STATS, 0))
-
-        def host_tick():
-            hid[:] = np.arange(n, dtype=np.uint32)
-            rc, lo, _, rg = api.ps_prepare_particles(harrs, hcodes, mmode)
-            api.check("ps_prepare_particles", rc)
-            c = np.float32(np.float32(lo) + np.float32(rg) / np.float32(2.0))
-            api.check("ps_calc_forces", api.ps_calc_forces(s.ctx, harrs, hcodes, c, c, c,
-                                                           np.float32(rg) / np.float32(2.0)))
-
-        host_tick()
-        t0 = time.perf_counter()
-        for _ in range(a.host_ticks):
-            host_tick()
-        th = (time.perf_counter() - t0) / a.host_ticks
-        line["host_pointers"] = dict(value=round(n / th, 1), unit="particles/s",
-                                     ms_per_tick=round(1e3 * th, 4), ticks=a.host_ticks,
-                                     note="prepare + calc_forces on pageable host arrays "
-                                          "(uploads, tick, sorted state and forces copied "
-                                          "back); not the headline value")
-    if not a.no_cpu_baseline:
-        def ref_fn(r, pos):
-            # FP64 direct sum on the device over the reference's sorted state
-            with api.Solver(n) as t:
-                t.load(r["x"], r["y"], r["z"], r["m"])
-                return t.direct_f64(pos)
-        try:
-            line["cpu_baseline"] = cpu_baseline(n, x, y, z, m, a.theta, a.cpu_ticks, ref_fn)
-            cb = line["cpu_baseline"]
-            line["gpu_over_cpu"] = round(value / cb["value"], 2)
-        except Exception as exc:  # report, never hide, a failed baseline
-            line["cpu_baseline"] = dict(value=None, error=str(exc))
-    s.close()
+                data="synthetic", config=r.pop("config"))
+    line.update(r)
+    if a.workload == "plummer1m" and not a.no_16m:
+        # the metric's second size: N = 16,777,216 two Plummers, fewer steps
+        # (each is ~10x the 1M step) and a bounded CPU sample at the thread
+        # count that was fastest at 1M
+        cb = line.get("cpu_baseline", {})
+        thr = [cb["cores"]] if cb.get("cores") else None
+        r16 = measure_plain(a, "twoplummer16m", max(3, min(a.steps, 10)), min(a.warmup, 2),
+                            1, min(a.host_ticks, 1), not a.no_cpu_baseline, cpu_threads=thr)
+        line["sizes"] = {"twoplummer16m": r16}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -181,6 +181,15 @@ int ps_hip_op_morton_exhaustive(uint64_t* out2);
  * calibrates the profiler's HBM byte counters */
 int ps_hip_op_copy(const void* src, void* dst, size_t bytes);
 
+/* no reference counterpart: streaming kernels of known traffic for checking
+ * the profiler's HBM byte counters, synchronous.  kind 0: copy n 16-byte
+ * words src -> dst; 1: read n 16-byte words of src (dst receives one 16-byte
+ * word per workgroup, at most 8192 of them); 2: write n 16-byte words of dst
+ * (src unused); 3: gather dst[i] = src[idx[i]] over n 4-byte elements
+ * (idx: device array, every index inside src). */
+int ps_hip_op_bandwidth(int kind, const void* src, void* dst, const uint32_t* idx,
+                        size_t n);
+
 /* no reference counterpart: FP64 direct sum (tests/accuracy.c:42-72) on the
  * device for nt target positions (host array) against all n sources (device
  * arrays); out3 (host) = 3 doubles per target. */

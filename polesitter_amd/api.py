@@ -113,6 +113,7 @@ def hip_lib():
             "ps_hip_op_m2l": ([_vp, _vp, _vp, _f, _vp], C.c_int),
             "ps_hip_op_morton_exhaustive": ([_vp], C.c_int),
             "ps_hip_op_copy": ([_vp, _vp, C.c_size_t], C.c_int),
+            "ps_hip_op_bandwidth": ([C.c_int, _vp, _vp, _vp, C.c_size_t], C.c_int),
             "ps_hip_engine_set_mlp": ([_vp, _vp], C.c_int),
             "ps_hip_mlp_features": ([_vp, _vp, _vp], C.c_int),
             "ps_hip_mlp_features_at": ([_vp, _vp, _sz, _vp, _vp], C.c_int),

@@ -1346,6 +1346,15 @@ int ps_hip_op_copy(const void* src, void* dst, size_t bytes) {
     return PS_HIP_OK;
 }
 
+int ps_hip_op_bandwidth(int kind, const void* src, void* dst, const uint32_t* idx, size_t n) {
+    if (kind < 0 || kind > 3 || !dst || !is_device_ptr(dst)) return PS_HIP_EINVAL;
+    if (kind != 2 && (!src || !is_device_ptr(src))) return PS_HIP_EINVAL;
+    if (kind == 3 && (!idx || !is_device_ptr(idx))) return PS_HIP_EINVAL;
+    PS_CK(ps_op_bandwidth(kind, src, dst, idx, n, nullptr));
+    PS_CK(hipDeviceSynchronize());
+    return PS_HIP_OK;
+}
+
 int ps_hip_direct_f64(const float* x, const float* y, const float* z, const float* mass,
                       size_t n, const uint32_t* targets, size_t nt, double* out3) {
     if (!x || !y || !z || !mass || !targets || !out3 || n == 0) return PS_HIP_EINVAL;

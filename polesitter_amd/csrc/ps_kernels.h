@@ -220,6 +220,12 @@ hipError_t ps_op_m2l(const float* tgeom, const float* sgeom, const float* smp, f
 hipError_t ps_op_morton_exhaustive(unsigned long long* out2, hipStream_t s);
 // plain copy of bytes (a multiple of 16) between device buffers
 hipError_t ps_op_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// streaming kernels of known traffic (the profiler's byte-counter checks):
+// kind 0 copy n 16-byte words src -> dst; 1 read n words of src (dst: one
+// 16-byte word per block, at most 8192); 2 write n words of dst; 3 gather
+// dst[i] = src[idx[i]] over n 4-byte elements
+hipError_t ps_op_bandwidth(int kind, const void* src, void* dst, const uint32_t* idx, size_t n,
+                           hipStream_t s);
 
 // ---- ps_mlp.hip: learned far-field correction ------------------------------
 #define PS_MLP_IN 24

@@ -51,6 +51,56 @@ __global__ __launch_bounds__(256) void op_copy_kernel(const uint4* __restrict__ 
         dst[i] = src[i];
 }
 
+// read-only stream of n 16-byte words; one 16-byte xor per block is written
+// (so the loads are not dropped): the known-size read for the fetch counters
+__global__ __launch_bounds__(256) void op_read_kernel(const uint4* __restrict__ src,
+                                                      uint4* __restrict__ out, size_t n) {
+    uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const uint4 v = src[i];
+        acc.x ^= v.x;
+        acc.y ^= v.y;
+        acc.z ^= v.z;
+        acc.w ^= v.w;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        acc.x ^= __shfl_down(acc.x, d, 64);
+        acc.y ^= __shfl_down(acc.y, d, 64);
+        acc.z ^= __shfl_down(acc.z, d, 64);
+        acc.w ^= __shfl_down(acc.w, d, 64);
+    }
+    __shared__ uint4 part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            acc.x ^= part[w].x;
+            acc.y ^= part[w].y;
+            acc.z ^= part[w].z;
+            acc.w ^= part[w].w;
+        }
+        out[blockIdx.x] = acc;
+    }
+}
+
+// write-only stream of n 16-byte words: the known-size write
+__global__ __launch_bounds__(256) void op_fill_kernel(uint4* __restrict__ dst, size_t n,
+                                                      uint32_t v) {
+    const uint4 w = make_uint4(v, v, v, v);
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = w;
+}
+
+// dst[i] = src[idx[i]] over 4-byte elements: scattered small reads, the
+// access shape of the id-gather and of the traversal's node loads
+__global__ __launch_bounds__(256) void op_gather_kernel(const uint32_t* __restrict__ src,
+                                                        const uint32_t* __restrict__ idx,
+                                                        uint32_t* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[idx[i]];
+}
+
 __global__ void zero2_kernel(unsigned long long* p) {
     p[0] = 0;
     p[1] = 0;
@@ -85,5 +135,32 @@ hipError_t ps_op_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     const unsigned blocks = (unsigned)(want < 8192 ? want : 8192);
     op_copy_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<const uint4*>(src),
                                           reinterpret_cast<uint4*>(dst), n);
+    return hipGetLastError();
+}
+
+hipError_t ps_op_bandwidth(int kind, const void* src, void* dst, const uint32_t* idx, size_t n,
+                           hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t want = (n + 255) / 256;
+    const unsigned blocks = (unsigned)(want < 8192 ? want : 8192);
+    switch (kind) {
+        case 0:
+            op_copy_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<const uint4*>(src),
+                                                  reinterpret_cast<uint4*>(dst), n);
+            break;
+        case 1:
+            op_read_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<const uint4*>(src),
+                                                  reinterpret_cast<uint4*>(dst), n);
+            break;
+        case 2:
+            op_fill_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<uint4*>(dst), n, 0x3f800000u);
+            break;
+        case 3:
+            op_gather_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<const uint32_t*>(src), idx,
+                                                    reinterpret_cast<uint32_t*>(dst), n);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }

@@ -4,20 +4,24 @@ Two uses:
 
 ``python -m polesitter_amd.profiling run --workload plummer1m --steps 3``
     The program to put under ``rocprofv3`` (kernel trace, or one ``--pmc``
-    pass at a time).  It first copies ``--copy-mb`` MiB between two device
-    buffers three times with a plain streaming kernel (``op_copy_kernel``,
-    known byte counts: the calibration of the HBM byte counters), then runs
-    ``--steps`` frame steps of the workload (tick + id-gather + Euler), one
-    ``trav_init_kernel`` dispatch per tick.
+    pass at a time).  It first runs four streaming kernels of known traffic
+    over buffers of ``--cal-mb`` MiB, three times each (larger than the
+    256 MiB last-level MALL, so they stream from HBM): a read-only pass
+    (``op_read_kernel``), a write-only pass (``op_fill_kernel``), a copy
+    (``op_copy_kernel``, half the size each way) and a random 4-byte gather
+    (``op_gather_kernel``).  Then it runs ``--steps`` frame steps of the
+    workload (tick + id-gather + Euler), one ``trav_init_kernel`` dispatch
+    per tick.
 
-``python -m polesitter_amd.profiling summarize --fetch DIR --write DIR``
-    Reads rocprofv3's CSV output (``*_counter_collection.csv`` of a
-    FETCH_SIZE pass and of a WRITE_SIZE pass) and prints a Markdown
-    summary: per kernel family the counter
-    values per tick, the copy kernel's counter reading against its known
-    bytes (the scale factor), and the calibrated HBM bytes per tick.  The
-    last line, ``traffic_bytes_per_tick <family> <bytes>``, is what bench.py
-    reads from the committed summary.
+``python -m polesitter_amd.profiling summarize --dirs DIR... --trace DIR``
+    Reads rocprofv3's CSV output of the four counter passes (FETCH_SIZE;
+    WRITE_SIZE; the read requests by size TCC_EA0_RDREQ_32B/_64B/_128B and
+    TCC_EA0_RDREQ; TCC_BUBBLE, TCC_EA0_RDREQ_DRAM, TCC_EA0_WRREQ and
+    TCC_EA0_WRREQ_64B) and optionally a kernel trace of the same program.
+    It checks each byte formula against the calibration kernels' known
+    bytes, then prints per kernel family the bytes per tick and, with the
+    trace, their rate.  The last lines, ``traffic_bytes_per_tick <family>
+    <bytes>``, are what bench.py reads from the committed summary.
 
 ``python -m polesitter_amd.profiling trace-summary --dir DIR --cmd "..."``
     Markdown table of a ``--kernel-trace --stats`` run, with per-tick sums of
@@ -27,7 +31,9 @@ Two uses:
     Markdown table of ``--pmc`` passes (any counters): each counter summed
     over every kernel family's dispatches of the run's last tick.
 
-FETCH_SIZE and WRITE_SIZE are KiB per dispatch (derived TCC counters).
+FETCH_SIZE and WRITE_SIZE are KiB per dispatch (derived TCC counters);
+the raw TCC counters are request counts per dispatch, summed over the
+channels and XCDs.
 """
 from __future__ import annotations
 
@@ -52,8 +58,12 @@ FAMILIES = {
                 "morton_seg_kernel"),
     "mlp": ("mlp_apply_kernel",),
     "frame": ("frame_kernel", "frame_finish_kernel"),
-    "copy": ("op_copy_kernel",),
+    "cal_copy": ("op_copy_kernel",),
+    "cal_read": ("op_read_kernel",),
+    "cal_fill": ("op_fill_kernel",),
+    "cal_gather": ("op_gather_kernel",),
 }
+CAL = ("cal_read", "cal_fill", "cal_copy", "cal_gather")
 TICK_MARK = "trav_init_kernel"
 
 
@@ -64,19 +74,40 @@ def family_of(name: str):
     return None
 
 
+def cal_bytes(cal_mb: int):
+    """Known traffic of one dispatch of each calibration kernel: (read, write)."""
+    nb = cal_mb << 20
+    blocks = min(8192, (nb // 16 + 255) // 256)
+    g = gather_elems(cal_mb)
+    return {"cal_read": (nb, 16 * blocks), "cal_fill": (0, nb), "cal_copy": (nb // 2, nb // 2),
+            # the gather reads its index array and one 4-byte element per
+            # index (the request size decides the fetched bytes), writes 4 B
+            "cal_gather": (4 * g + 4 * g, 4 * g)}
+
+
+def gather_elems(cal_mb: int) -> int:
+    return (cal_mb << 20) // 64
+
+
 def run(args):
     from . import api
     from . import workloads as W
 
     L = api.hip_lib()
-    # calibration copies
-    nbytes = args.copy_mb << 20
-    src = api.DeviceArray(nbytes // 4, np.float32)
-    dst = api.DeviceArray(nbytes // 4, np.float32)
+    nb = args.cal_mb << 20
+    a = api.DeviceArray(nb // 4, np.float32)
+    b = api.DeviceArray(nb // 4, np.float32)
+    g = gather_elems(args.cal_mb)
+    rng = np.random.default_rng(5)
+    idx = api.DeviceArray.from_numpy(rng.integers(0, nb // 4, g, dtype=np.uint32))
+    out = api.DeviceArray(g, np.uint32)
     for _ in range(3):
-        api.check("ps_hip_op_copy", L.ps_hip_op_copy(src.ptr, dst.ptr, nbytes))
-    print(f"profiling: copied {nbytes} bytes x3", flush=True)
-    del src, dst
+        api.check("fill", L.ps_hip_op_bandwidth(2, None, a.ptr, None, nb // 16))
+        api.check("read", L.ps_hip_op_bandwidth(1, a.ptr, b.ptr, None, nb // 16))
+        api.check("copy", L.ps_hip_op_bandwidth(0, a.ptr, b.ptr, None, nb // 32))
+        api.check("gather", L.ps_hip_op_bandwidth(3, a.ptr, out.ptr, idx.ptr, g))
+    print(f"profiling: calibration kernels over {nb} bytes x3", flush=True)
+    del a, b, idx, out
 
     if args.workload == "plummer1m":
         n = 1 << 20
@@ -105,89 +136,155 @@ def _csvs(d, suffix):
     return sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
 
 
-def read_counters(d, counter):
-    """Per dispatch value of one counter: {dispatch_id: (kernel, value)}."""
+def read_all(dirs):
+    """{(path, dispatch): (kernel, {counter: value})} over the passes in dirs."""
     out = {}
-    for path in _csvs(d, "counter_collection.csv"):
-        with open(path, newline="") as f:
-            for r in csv.DictReader(f):
-                if r.get("Counter_Name") != counter:
-                    continue
-                key = (path, r["Dispatch_Id"])
-                name = r["Kernel_Name"]
-                val = float(r["Counter_Value"])
-                if key in out:
-                    out[key] = (name, out[key][1] + val)
-                else:
-                    out[key] = (name, val)
+    for d in dirs:
+        for path in _csvs(d, "counter_collection.csv"):
+            with open(path, newline="") as f:
+                for r in csv.DictReader(f):
+                    key = (path, int(r["Dispatch_Id"]))
+                    name, vals = out.get(key, (r["Kernel_Name"], {}))
+                    c = r["Counter_Name"]
+                    vals[c] = vals.get(c, 0.0) + float(r["Counter_Value"])
+                    out[key] = (name, vals)
     return out
 
 
-def per_family(disp):
-    fam = defaultdict(float)
+def _formulas(v):
+    """Byte estimates of one dispatch from whichever counters it carries."""
+    out = {}
+    if "FETCH_SIZE" in v:
+        out["fetch_size"] = 1024.0 * v["FETCH_SIZE"]
+    if all(k in v for k in ("TCC_EA0_RDREQ_32B", "TCC_EA0_RDREQ_64B", "TCC_EA0_RDREQ_128B")):
+        out["read_by_size"] = (32.0 * v["TCC_EA0_RDREQ_32B"] + 64.0 * v["TCC_EA0_RDREQ_64B"] +
+                               128.0 * v["TCC_EA0_RDREQ_128B"])
+    if "WRITE_SIZE" in v:
+        out["write_size"] = 1024.0 * v["WRITE_SIZE"]
+    if "TCC_EA0_WRREQ" in v and "TCC_EA0_WRREQ_64B" in v:
+        out["write_by_size"] = (32.0 * (v["TCC_EA0_WRREQ"] - v["TCC_EA0_WRREQ_64B"]) +
+                                64.0 * v["TCC_EA0_WRREQ_64B"])
+    return out
+
+
+def family_ms_per_tick(trace_dir):
+    """{family: ms per tick} from a --kernel-trace run of the same program."""
+    fam_ns = defaultdict(float)
     ticks = 0
-    copies = []
-    for name, val in disp.values():
-        if TICK_MARK in name:
-            ticks += 1
-        f = family_of(name)
-        if f == "copy":
-            copies.append(val)
-        elif f:
-            fam[f] += val
-    return fam, ticks, copies
+    for path in _csvs(trace_dir, "kernel_trace.csv"):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                name = r["Kernel_Name"]
+                if TICK_MARK in name:
+                    ticks += 1
+                fm = family_of(name)
+                if fm:
+                    fam_ns[fm] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    if not ticks:
+        return {}, 0
+    out = {f: ns / 1e6 / (1 if f in CAL else ticks) for f, ns in fam_ns.items()}
+    return out, ticks
 
 
 def summarize(args):
-    copy_bytes = args.copy_mb << 20
-    fetch = read_counters(args.fetch, "FETCH_SIZE")
-    write = read_counters(args.write, "WRITE_SIZE")
-    ff, tf, cf = per_family(fetch)
-    fw, tw, cw = per_family(write)
-    if not cf or not cw or not tf or not tw:
-        print("no copy-kernel or tick dispatches found", file=sys.stderr)
-        return 1
-    # the copy kernel reads and writes copy_bytes each: counter KiB -> bytes
-    kf = copy_bytes / (float(np.median(cf)) * 1024.0)
-    kw = copy_bytes / (float(np.median(cw)) * 1024.0)
-    lines = [
-        f"# HBM traffic per tick ({args.workload})",
-        "",
-        "Commands (each pass its own run, from the repository root):",
-        "",
-        "```",
-        f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir_fetch> -o run -- "
-        f"python3 -u -m polesitter_amd.profiling run --workload {args.workload} "
-        f"--steps {args.steps} --copy-mb {args.copy_mb}",
-        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir_write> -o run -- "
-        f"python3 -u -m polesitter_amd.profiling run --workload {args.workload} "
-        f"--steps {args.steps} --copy-mb {args.copy_mb}",
-        f"python3 -m polesitter_amd.profiling summarize --fetch <dir_fetch> --write <dir_write> "
-        f"--workload {args.workload} --steps {args.steps} --copy-mb {args.copy_mb}",
-        "```",
-        "",
-        f"Calibration: `op_copy_kernel` moves {copy_bytes} B each way per dispatch; "
-        f"FETCH_SIZE reads {np.median(cf):.0f} KiB, WRITE_SIZE {np.median(cw):.0f} KiB "
-        f"(median of {len(cf)}): scale factors fetch {kf:.4f}, write {kw:.4f} "
-        f"(bytes per counter KiB / 1024).",
-        "",
-        f"Ticks profiled: {tf} (fetch pass), {tw} (write pass).",
-        "",
-        "| kernel family | FETCH_SIZE KiB/tick | WRITE_SIZE KiB/tick | calibrated HBM bytes/tick |",
-        "|---|---|---|---|",
-    ]
-    traffic = {}
-    for fam in FAMILIES:
-        if fam == "copy" or (fam not in ff and fam not in fw):
+    disp = read_all(args.dirs)
+    # per family: summed formulas over the run (tick families) or the median
+    # per dispatch (calibration kernels)
+    fam = defaultdict(lambda: defaultdict(float))
+    cal = defaultdict(lambda: defaultdict(list))
+    ticks = defaultdict(int)  # per pass file: trav_init dispatches
+    for (path, _), (name, vals) in disp.items():
+        if TICK_MARK in name:
+            ticks[path] += 1
+        f = family_of(name)
+        if not f:
             continue
-        a = ff.get(fam, 0.0) / tf
-        b = fw.get(fam, 0.0) / tw
-        t = a * 1024.0 * kf + b * 1024.0 * kw
-        traffic[fam] = t
-        lines.append(f"| {fam} | {a:.0f} | {b:.0f} | {t:.4g} |")
+        for k, b in _formulas(vals).items():
+            if f in CAL:
+                cal[f][k].append(b)
+            else:
+                fam[f][k] += b
+        for c in ("TCC_EA0_RDREQ", "TCC_EA0_RDREQ_32B", "TCC_EA0_RDREQ_64B",
+                  "TCC_EA0_RDREQ_128B", "TCC_BUBBLE", "TCC_EA0_RDREQ_DRAM"):
+            if c in vals and f in CAL:
+                cal[f][c].append(vals[c])
+    n_ticks = max(ticks.values()) if ticks else 0
+    if not n_ticks:
+        print("no tick dispatches found", file=sys.stderr)
+        return 1
+    known = cal_bytes(args.cal_mb)
+    med = lambda f, k: float(np.median(cal[f][k])) if cal[f].get(k) else float("nan")
+    # scale factors: known bytes over what each formula reads for the
+    # read-only and the write-only kernel (both far larger than the MALL)
+    kr_fetch = known["cal_read"][0] / med("cal_read", "fetch_size")
+    kr_size = known["cal_read"][0] / med("cal_read", "read_by_size")
+    kw_write = known["cal_fill"][1] / med("cal_fill", "write_size")
+    ms, _ = family_ms_per_tick(args.trace) if args.trace else ({}, 0)
+    cmd = (f"python3 -u -m polesitter_amd.profiling run --workload {args.workload} "
+           f"--steps {args.steps} --cal-mb {args.cal_mb}")
+    lines = [
+        f"# HBM traffic per tick ({args.workload})", "",
+        "Commands (each pass its own run, from the repository root):", "", "```",
+        f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d <d1> -o run -- {cmd}",
+        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d <d2> -o run -- {cmd}",
+        f"rocprofv3 --pmc TCC_EA0_RDREQ_32B TCC_EA0_RDREQ_64B TCC_EA0_RDREQ_128B TCC_EA0_RDREQ "
+        f"--output-format csv -d <d3> -o run -- {cmd}",
+        f"rocprofv3 --pmc TCC_BUBBLE TCC_EA0_RDREQ_DRAM TCC_EA0_WRREQ TCC_EA0_WRREQ_64B "
+        f"--output-format csv -d <d4> -o run -- {cmd}",
+        f"rocprofv3 --kernel-trace --stats --output-format csv -d <d5> -o run -- {cmd}",
+        f"python3 -m polesitter_amd.profiling summarize --dirs <d1> <d2> <d3> <d4> --trace <d5> "
+        f"--workload {args.workload} --steps {args.steps} --cal-mb {args.cal_mb}",
+        "```", "",
+        "## Calibration (median per dispatch; buffers of "
+        f"{args.cal_mb} MiB, the MALL is 256 MiB)", "",
+        "| kernel | known read B | FETCH_SIZE B | read by size B | known write B | "
+        "WRITE_SIZE B | RDREQ | 32B | 64B | 128B | BUBBLE | ms |",
+        "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for f in CAL:
+        if f not in cal:
+            continue
+        kr, kw = known[f]
+        t = ms.get(f)
+        lines.append(f"| {f} | {kr:.4g} | {med(f, 'fetch_size'):.4g} | "
+                     f"{med(f, 'read_by_size'):.4g} | {kw:.4g} | {med(f, 'write_size'):.4g} | "
+                     f"{med(f, 'TCC_EA0_RDREQ'):.4g} | {med(f, 'TCC_EA0_RDREQ_32B'):.4g} | "
+                     f"{med(f, 'TCC_EA0_RDREQ_64B'):.4g} | {med(f, 'TCC_EA0_RDREQ_128B'):.4g} | "
+                     f"{med(f, 'TCC_BUBBLE'):.4g} | "
+                     f"{'' if t is None else f'{t / 3:.3f}'} |")
+    lines += ["", f"Scale factors (known bytes / formula, read-only and write-only kernels): "
+              f"FETCH_SIZE {kr_fetch:.4f}, read by size {kr_size:.4f}, WRITE_SIZE "
+              f"{kw_write:.4f}.", ""]
+    g = cal.get("cal_gather")
+    if g and g.get("read_by_size"):
+        gb = med("cal_gather", "read_by_size")
+        gf = med("cal_gather", "fetch_size") * kr_fetch
+        lines += [f"Gather check (scattered 4-byte reads, 4 B index + 4 B element per "
+                  f"lookup): reads by size x factor = {gb * kr_size:.4g} B, FETCH_SIZE x "
+                  f"factor = {gf:.4g} B, for {known['cal_gather'][0]:.4g} B of useful reads "
+                  f"({(gb * kr_size) / (gather_elems(args.cal_mb)):.1f} B fetched per lookup).",
+                  ""]
+    lines += [f"Ticks profiled per pass: {n_ticks}.", "",
+              "## Per tick and kernel family", "",
+              "Reads = read requests by size x factor; writes = WRITE_SIZE x factor.", "",
+              "| family | reads B/tick | (FETCH_SIZE x factor) | writes B/tick | total B/tick | "
+              "ms/tick | rate GB/s |", "|---|---|---|---|---|---|---|"]
+    traffic = {}
+    for f in FAMILIES:
+        if f in CAL or f not in fam:
+            continue
+        v = fam[f]
+        rd = v.get("read_by_size", 0.0) * kr_size / n_ticks
+        fs = v.get("fetch_size", 0.0) * kr_fetch / n_ticks
+        wr = v.get("write_size", 0.0) * kw_write / n_ticks
+        tot = rd + wr
+        traffic[f] = tot
+        t = ms.get(f)
+        rate = "" if not t else f"{tot / (t * 1e-3) / 1e9:.0f}"
+        lines.append(f"| {f} | {rd:.4g} | ({fs:.4g}) | {wr:.4g} | {tot:.4g} | "
+                     f"{'' if t is None else f'{t:.4f}'} | {rate} |")
     lines.append("")
-    for fam, t in traffic.items():
-        lines.append(f"traffic_bytes_per_tick {fam} {t:.6g}")
+    for f, t in traffic.items():
+        lines.append(f"traffic_bytes_per_tick {f} {t:.6g}")
     print("\n".join(lines))
     return 0
 
@@ -288,13 +385,13 @@ def main(argv=None):
     r.add_argument("--steps", type=int, default=3)
     r.add_argument("--theta", type=float, default=2.0)
     r.add_argument("--kahan", action="store_true")
-    r.add_argument("--copy-mb", type=int, default=512)
+    r.add_argument("--cal-mb", type=int, default=2048)
     m = sub.add_parser("summarize")
-    m.add_argument("--fetch", required=True)
-    m.add_argument("--write", required=True)
+    m.add_argument("--dirs", nargs="+", required=True)
+    m.add_argument("--trace", default="")
     m.add_argument("--workload", default="plummer1m")
     m.add_argument("--steps", type=int, default=3)
-    m.add_argument("--copy-mb", type=int, default=512)
+    m.add_argument("--cal-mb", type=int, default=2048)
     t = sub.add_parser("trace-summary")
     t.add_argument("--dir", required=True)
     t.add_argument("--title", default="tick")

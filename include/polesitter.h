@@ -107,6 +107,32 @@ ps_result_t ps_set_option(ps_context_t* ctx, int key, double value);
 struct ps_hip_engine;
 struct ps_hip_engine* ps_get_engine(ps_context_t* ctx);
 
+/* ps_config_t with caller-owned device memory appended.  ps_config_t itself
+ * stays the reference's five fields: appending to it would make every
+ * positional initialiser of the reference's callers (tests/main.c:82) miss
+ * fields, which -Wextra (CMakeLists.txt:37-45) reports.  With dev_buff set
+ * (device memory of at least ps_device_scratch_bytes(max_particles) bytes)
+ * ps_init_ex makes no allocation at all: the device buffers of every tick
+ * are carved from dev_buff and the solver's host state lives in buff after
+ * the context.  NULL: the same as ps_init. */
+typedef struct {
+    ps_config_t base;
+    void*       dev_buff;
+    size_t      dev_buff_size;
+} ps_config_ex_t;
+
+ps_result_t ps_init_ex(ps_context_t** out_ctx, const ps_config_ex_t* cfg);
+size_t      ps_device_scratch_bytes(size_t max_particles);
+
+/* ps_calc_forces enqueued on a HIP stream (NULL: the default stream) without
+ * waiting; device arrays only.  ps_calc_forces_wait completes it and returns
+ * what ps_calc_forces would have (see ps_hip_calc_forces_async). */
+ps_result_t ps_calc_forces_async(ps_context_t* ctx, const ps_particle_arrs_t* arrs,
+                                 uint32_t* morton_codes, float root_cx,
+                                 float root_cy, float root_cz, float root_hw,
+                                 void* stream);
+ps_result_t ps_calc_forces_wait(ps_context_t* ctx);
+
 #ifdef __cplusplus
 }
 #endif
@@ -143,7 +169,8 @@ static ps_result_t ps_impl_result(int rc) {
     }
 }
 
-ps_result_t ps_init(ps_context_t** out_ctx, const ps_config_t* cfg) {
+static ps_result_t ps_impl_init(ps_context_t** out_ctx, const ps_config_t* cfg,
+                                void* dev_buff, size_t dev_buff_size) {
     /* validation and buffer contract of H:1700-1743 */
     if (!cfg || !cfg->buff || cfg->buff_size < sizeof(ps_context_t) || !out_ctx) {
         return PS_EINVAL;
@@ -166,14 +193,62 @@ ps_result_t ps_init(ps_context_t** out_ctx, const ps_config_t* cfg) {
 #else
     int traversal = PS_HIP_TRAV_ST;
 #endif
-    int rc = ps_hip_engine_create(&ctx->engine, cfg->max_particles,
-                                  ctx->max_nodes, ctx->theta, traversal);
+    /* with caller device memory, the engine's host state goes into the
+     * part of buff the reference uses for its node arena */
+    void* host_mem = dev_buff ? (void*)((char*)cfg->buff + arena_start) : NULL;
+    int   rc = ps_hip_engine_create_in(&ctx->engine, cfg->max_particles, ctx->max_nodes,
+                                       ctx->theta, traversal, host_mem,
+                                       dev_buff ? usable : 0, dev_buff, dev_buff_size);
     if (rc != PS_HIP_OK) {
         ctx->engine = NULL;
+        if (dev_buff && rc == PS_HIP_EOOM) {
+            return PS_EOOM; /* dev_buff smaller than ps_device_scratch_bytes */
+        }
         return rc == PS_HIP_EOOM ? PS_EALLOC : ps_impl_result(rc);
     }
     *out_ctx = ctx;
     return PS_OK;
+}
+
+ps_result_t ps_init(ps_context_t** out_ctx, const ps_config_t* cfg) {
+    return ps_impl_init(out_ctx, cfg, NULL, 0);
+}
+
+ps_result_t ps_init_ex(ps_context_t** out_ctx, const ps_config_ex_t* cfg) {
+    if (!cfg) {
+        return PS_EINVAL;
+    }
+    return ps_impl_init(out_ctx, &cfg->base, cfg->dev_buff, cfg->dev_buff_size);
+}
+
+size_t ps_device_scratch_bytes(size_t max_particles) {
+    return ps_hip_engine_scratch_bytes(max_particles);
+}
+
+ps_result_t ps_calc_forces_async(ps_context_t* ctx, const ps_particle_arrs_t* arrs,
+                                 uint32_t* morton_codes, float root_cx,
+                                 float root_cy, float root_cz, float root_hw,
+                                 void* stream) {
+    if (!ctx || !arrs || !ctx->engine) {
+        return PS_EINVAL;
+    }
+    if (arrs->cnt == 0) {
+        return PS_OK;
+    }
+    if (!morton_codes || arrs->cnt > ctx->max_particles) {
+        return PS_EINVAL;
+    }
+    return ps_impl_result(ps_hip_calc_forces_async(
+        ctx->engine, arrs->x, arrs->y, arrs->z, arrs->mass, arrs->fx, arrs->fy,
+        arrs->fz, arrs->id, arrs->cnt, morton_codes, root_cx, root_cy, root_cz,
+        root_hw, stream));
+}
+
+ps_result_t ps_calc_forces_wait(ps_context_t* ctx) {
+    if (!ctx || !ctx->engine) {
+        return PS_EINVAL;
+    }
+    return ps_impl_result(ps_hip_calc_forces_wait(ctx->engine));
 }
 
 ps_result_t ps_calc_forces(ps_context_t* ctx, const ps_particle_arrs_t* arrs,

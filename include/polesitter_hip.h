@@ -105,6 +105,34 @@ typedef struct {
 int ps_hip_engine_create(ps_hip_engine_t** out, size_t max_particles,
                          size_t max_nodes, float theta, int traversal);
 
+/* ps_init with caller-owned memory (no reference counterpart beyond the
+ * reference's own rule that the caller owns every buffer, H:23-24,
+ * H:1699-1767).  host_mem (optional, at least ps_hip_engine_host_bytes()):
+ * the engine's state and its host mirrors are placed there instead of the
+ * heap.  dev_mem (optional, device memory of at least
+ * ps_hip_engine_scratch_bytes(max_particles) bytes): every device buffer a
+ * tick uses is carved from it, and the engine makes no device allocation,
+ * ever -- a tick that would need more (a traversal list arena overflow, a
+ * learned correction's training buffers) returns PS_HIP_EOOM.  The caller
+ * frees both after ps_hip_engine_destroy. */
+int ps_hip_engine_create_in(ps_hip_engine_t** out, size_t max_particles,
+                            size_t max_nodes, float theta, int traversal,
+                            void* host_mem, size_t host_bytes, void* dev_mem,
+                            size_t dev_bytes);
+/* sizes for ps_hip_engine_create_in: host bytes for the engine's state;
+ * device bytes for every buffer of a tick of up to max_particles particles
+ * (node arrays for the most nodes that many particles can give, the list
+ * arena at 144 words per particle -- the capacity a self-allocating engine
+ * has after its first tick) */
+size_t ps_hip_engine_host_bytes(void);
+size_t ps_hip_engine_scratch_bytes(size_t max_particles);
+/* the most octree nodes n particles can give (one new node per particle and
+ * depth, at most 8^d at depth d, depth <= 10) */
+size_t ps_hip_worst_case_nodes(size_t n);
+/* no reference counterpart: device and pinned host allocations the library
+ * has made in this process (count and bytes; ps_hip_malloc not included) */
+int ps_hip_alloc_stats(uint64_t* calls, uint64_t* bytes);
+
 /* ps_destroy, H:2172-2192 */
 int ps_hip_engine_destroy(ps_hip_engine_t* e);
 
@@ -142,6 +170,28 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z,
                        float* mass, float* fx, float* fy, float* fz,
                        uint32_t* id, size_t n, uint32_t* codes, float cx,
                        float cy, float cz, float hw);
+
+/* ps_calc_forces (H:1840-1909) enqueued on the caller's HIP stream (NULL:
+ * the default stream); it returns without waiting for the device.  Device
+ * arrays only (host arrays: PS_HIP_EINVAL).  Every size of the tick is read
+ * from device memory, as in the batched tick: the node arrays are grown to
+ * ps_hip_worst_case_nodes(n) first and all traversal iterations are
+ * launched.  The results are bit for bit those of ps_hip_calc_forces at the
+ * same lanes per traversal entry (PS_HIP_OPT_TRAV_LANES; with the default 0
+ * the asynchronous tick chooses them from the node capacity, the synchronous
+ * one from the level sizes, which can change the order of an entry's M2L
+ * sums).  ps_hip_calc_forces_wait must follow before the arrays are read,
+ * another tick is started on the engine, or the stream is destroyed: it waits
+ * for the tick, reports what ps_hip_calc_forces would have returned (invalid
+ * codes, a tree over the node capacity) and, after a traversal list arena
+ * overflow, grows the arenas and runs the tick again from the traversal
+ * before it returns.  Stage events and pair counts (PS_HIP_OPT_STATS) are
+ * not recorded by the asynchronous tick. */
+int ps_hip_calc_forces_async(ps_hip_engine_t* e, float* x, float* y, float* z,
+                             float* mass, float* fx, float* fy, float* fz,
+                             uint32_t* id, size_t n, uint32_t* codes, float cx,
+                             float cy, float cz, float hw, void* stream);
+int ps_hip_calc_forces_wait(ps_hip_engine_t* e);
 
 /* no reference counterpart: copies the last tick's octree (canonical level
  * order) to host arrays of capacity cap: geom4 = (x,y,z,hw), mp4, loc4 (total
@@ -275,6 +325,11 @@ void* ps_hip_malloc(size_t bytes);
 int   ps_hip_free(void* p);
 int   ps_hip_memcpy(void* dst, const void* src, size_t bytes, int kind);
 int   ps_hip_synchronize(void);
+/* HIP streams for callers without a HIP runtime of their own (the
+ * asynchronous tick's stream argument); non-blocking, default priority */
+int   ps_hip_stream_create(void** out);
+int   ps_hip_stream_destroy(void* stream);
+int   ps_hip_stream_synchronize(void* stream);
 int   ps_hip_device_name(char* buf, size_t len);
 
 #ifdef __cplusplus

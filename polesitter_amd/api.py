@@ -49,6 +49,11 @@ class PsConfig(C.Structure):
                 ("max_particles", C.c_size_t), ("theta", C.c_float), ("thrd_cnt", C.c_size_t)]
 
 
+class PsConfigEx(C.Structure):
+    """ps_config_ex_t: ps_config_t followed by caller-owned device memory"""
+    _fields_ = [("base", PsConfig), ("dev_buff", C.c_void_p), ("dev_buff_size", C.c_size_t)]
+
+
 class PsParticleArrs(C.Structure):
     """ps_particle_arrs_t, src/polesitter.h:312-322"""
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
@@ -134,6 +139,17 @@ def hip_lib():
             "ps_hip_free": ([_vp], C.c_int),
             "ps_hip_memcpy": ([_vp, _vp, _sz, C.c_int], C.c_int),
             "ps_hip_synchronize": ([], C.c_int),
+            "ps_hip_stream_create": ([C.POINTER(_vp)], C.c_int),
+            "ps_hip_stream_destroy": ([_vp], C.c_int),
+            "ps_hip_stream_synchronize": ([_vp], C.c_int),
+            "ps_hip_engine_create_in": ([C.POINTER(_vp), _sz, _sz, _f, C.c_int, _vp, _sz, _vp,
+                                         _sz], C.c_int),
+            "ps_hip_engine_host_bytes": ([], _sz),
+            "ps_hip_engine_scratch_bytes": ([_sz], _sz),
+            "ps_hip_worst_case_nodes": ([_sz], _sz),
+            "ps_hip_alloc_stats": ([C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+            "ps_hip_calc_forces_async": ([_vp] * 9 + [_sz, _vp, _f, _f, _f, _f, _vp], C.c_int),
+            "ps_hip_calc_forces_wait": ([_vp], C.c_int),
             "ps_hip_device_name": ([C.c_char_p, _sz], C.c_int),
         }
         for name, (args, res) in sig.items():
@@ -168,6 +184,15 @@ def host_lib():
         L.ps_set_option.restype = C.c_int
         L.ps_get_engine.argtypes = [_vp]
         L.ps_get_engine.restype = _vp
+        L.ps_init_ex.argtypes = [C.POINTER(_vp), C.POINTER(PsConfigEx)]
+        L.ps_init_ex.restype = C.c_int
+        L.ps_device_scratch_bytes.argtypes = [_sz]
+        L.ps_device_scratch_bytes.restype = _sz
+        L.ps_calc_forces_async.argtypes = [_vp, C.POINTER(PsParticleArrs), _vp, _f, _f, _f, _f,
+                                           _vp]
+        L.ps_calc_forces_async.restype = C.c_int
+        L.ps_calc_forces_wait.argtypes = [_vp]
+        L.ps_calc_forces_wait.restype = C.c_int
         _host = L
     return _host
 
@@ -274,6 +299,53 @@ def ps_init(buff_size: int, max_particles: int, theta: float = 0.0, thrd_cnt: in
     return rc, (Context(ctx.value, buff) if rc == PS_OK else None)
 
 
+def ps_init_ex(buff_size: int, max_particles: int, theta: float = 0.0, thrd_cnt: int = 4,
+               dev_buff=None, dev_buff_size: int = 0):
+    """ps_init_ex: ps_init with caller-owned device memory (dev_buff: a
+    DeviceArray or device address).  Returns (result, Context or None)."""
+    buff = (C.c_uint8 * int(buff_size))()
+    base = PsConfig(C.cast(buff, _vp), int(buff_size), int(max_particles), float(theta),
+                    int(thrd_cnt))
+    cfg = PsConfigEx(base, _ptr(dev_buff), int(dev_buff_size))
+    ctx = _vp()
+    rc = host_lib().ps_init_ex(C.byref(ctx), C.byref(cfg))
+    return rc, (Context(ctx.value, buff) if rc == PS_OK else None)
+
+
+def device_scratch_bytes(max_particles: int) -> int:
+    return int(host_lib().ps_device_scratch_bytes(int(max_particles)))
+
+
+def alloc_stats():
+    """(calls, bytes) of the library's own device and pinned allocations"""
+    c, b = C.c_uint64(), C.c_uint64()
+    check("ps_hip_alloc_stats", hip_lib().ps_hip_alloc_stats(C.byref(c), C.byref(b)))
+    return c.value, b.value
+
+
+class Stream:
+    """A HIP stream from the library (ps_hip_stream_create)."""
+
+    def __init__(self):
+        h = _vp()
+        check("ps_hip_stream_create", hip_lib().ps_hip_stream_create(C.byref(h)))
+        self.h = h.value
+
+    def synchronize(self):
+        check("ps_hip_stream_synchronize", hip_lib().ps_hip_stream_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            hip_lib().ps_hip_stream_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def ps_prepare_particles(arrs: PsParticleArrs, codes, mode: int | None = None):
     """ps_prepare_particles (H:1911-2170): returns (result, min, max, range)."""
     lo, hi, rg = _f(), _f(), _f()
@@ -291,6 +363,19 @@ def ps_calc_forces(ctx: Context, arrs: PsParticleArrs, codes, cx, cy, cz, hw) ->
     """ps_calc_forces (H:1840-1909)."""
     return host_lib().ps_calc_forces(ctx.ptr if ctx else None, C.byref(arrs), _ptr(codes),
                                      float(cx), float(cy), float(cz), float(hw))
+
+
+def ps_calc_forces_async(ctx: Context, arrs: PsParticleArrs, codes, cx, cy, cz, hw,
+                         stream=None) -> int:
+    """ps_calc_forces enqueued on a stream (a Stream, a handle, or None for
+    the default stream) without waiting; complete with ps_calc_forces_wait."""
+    h = stream.h if isinstance(stream, Stream) else stream
+    return host_lib().ps_calc_forces_async(ctx.ptr if ctx else None, C.byref(arrs), _ptr(codes),
+                                           float(cx), float(cy), float(cz), float(hw), h)
+
+
+def ps_calc_forces_wait(ctx: Context) -> int:
+    return host_lib().ps_calc_forces_wait(ctx.ptr if ctx else None)
 
 
 def ps_destroy(ctx: Context) -> int:
@@ -343,10 +428,18 @@ class Solver:
     (README.md:94-126)."""
 
     def __init__(self, n: int, theta: float = 2.0, buff_size: int | None = None,
-                 traversal: int = TRAV_MT, kahan: bool = False, stats: bool = False):
+                 traversal: int = TRAV_MT, kahan: bool = False, stats: bool = False,
+                 device_scratch: bool = False):
         self.n = int(n)
-        rc, self.ctx = ps_init(buff_size or buff_size_for(n), n, theta)
-        check("ps_init", rc)
+        self.scratch = None
+        if device_scratch:  # caller-owned device memory for every tick buffer
+            self.scratch = DeviceArray(device_scratch_bytes(n), np.uint8)
+            rc, self.ctx = ps_init_ex(buff_size or buff_size_for(n), n, theta,
+                                      dev_buff=self.scratch, dev_buff_size=self.scratch.nbytes)
+            check("ps_init_ex", rc)
+        else:
+            rc, self.ctx = ps_init(buff_size or buff_size_for(n), n, theta)
+            check("ps_init", rc)
         if traversal != TRAV_MT:
             check("ps_set_option", ps_set_option(self.ctx, OPT_TRAVERSAL, traversal))
         if kahan:
@@ -388,6 +481,20 @@ class Solver:
         check("ps_calc_forces", ps_calc_forces(self.ctx, self.arrs, self.codes, rc_, rc_, rc_, hw))
         return lo, hi, rg
 
+    def tick_async(self, stream=None, mode=None):
+        """prepare (synchronous: the root comes from its bounding cube), then
+        calc_forces enqueued on the stream; complete with wait()"""
+        rc, lo, hi, rg = ps_prepare_particles(self.arrs, self.codes, mode)
+        check("ps_prepare_particles", rc)
+        rc_ = np.float32(np.float32(lo) + np.float32(rg) / np.float32(2.0))
+        hw = np.float32(rg) / np.float32(2.0)
+        check("ps_calc_forces_async", ps_calc_forces_async(self.ctx, self.arrs, self.codes, rc_,
+                                                           rc_, rc_, hw, stream))
+        return lo, hi, rg
+
+    def wait(self):
+        check("ps_calc_forces_wait", ps_calc_forces_wait(self.ctx))
+
     def result(self):
         return dict(x=self.x.numpy(), y=self.y.numpy(), z=self.z.numpy(), m=self.m.numpy(),
                     fx=self.fx.numpy(), fy=self.fy.numpy(), fz=self.fz.numpy(),
@@ -418,6 +525,9 @@ class Solver:
             self.ctx = None
         for a in (self.x, self.y, self.z, self.m, self.fx, self.fy, self.fz, self.id, self.codes):
             a.free()
+        if self.scratch is not None:  # after ps_destroy: the engine no longer uses it
+            self.scratch.free()
+            self.scratch = None
 
     def __enter__(self):
         return self

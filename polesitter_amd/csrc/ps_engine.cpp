@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <iterator>
 #include <cstdio>
 #include <cstring>
@@ -62,11 +63,36 @@ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool fixed = false;  // carved from caller-owned memory: never reallocated or freed
 };
 
-// grow-only device buffer; synchronises the stream before replacing memory
+// device and pinned host allocations the library has made (process-wide)
+std::atomic<unsigned long long> g_alloc_calls{0}, g_alloc_bytes{0};
+
+hipError_t dev_malloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) {
+        g_alloc_calls.fetch_add(1, std::memory_order_relaxed);
+        g_alloc_bytes.fetch_add(bytes, std::memory_order_relaxed);
+    }
+    return e;
+}
+
+hipError_t host_malloc(void** p, size_t bytes) {
+    hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) {
+        g_alloc_calls.fetch_add(1, std::memory_order_relaxed);
+        g_alloc_bytes.fetch_add(bytes, std::memory_order_relaxed);
+    }
+    return e;
+}
+
+// grow-only device buffer; synchronises the stream before replacing memory.
+// A buffer carved from caller-owned memory cannot grow: a need beyond its
+// size is out of memory.
 hipError_t grow(DevBuf& b, size_t need, hipStream_t s) {
     if (need <= b.bytes && b.p) return hipSuccess;
+    if (b.fixed) return hipErrorOutOfMemory;
     if (b.p) {
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) return e;
@@ -75,7 +101,7 @@ hipError_t grow(DevBuf& b, size_t need, hipStream_t s) {
         b.bytes = 0;
     }
     size_t nb = align256(std::max<size_t>(need + need / 2, 256));
-    hipError_t e = hipMalloc(&b.p, nb);
+    hipError_t e = dev_malloc(&b.p, nb);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         b.p = nullptr;
@@ -86,9 +112,10 @@ hipError_t grow(DevBuf& b, size_t need, hipStream_t s) {
 }
 
 void release(DevBuf& b) {
-    if (b.p) (void)hipFree(b.p);
+    if (b.p && !b.fixed) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
+    b.fixed = false;
 }
 
 // most nodes n particles can produce: at most one new node per particle per
@@ -156,8 +183,15 @@ struct ps_hip_engine {
     DevBuf far_b, block_pairs_b, tstate_b, big_b;
     // small device / pinned host scalars
     DevBuf small_b;  // level_start[12], flags, pad | u64 totals[4] | u64 pairs | u64 leaf pairs
-    uint32_t* h_small = nullptr;  // pinned mirror
+    uint32_t* h_small = nullptr;  // pinned mirror (in caller memory when placed)
     PsTravState* h_tstate = nullptr;  // pinned copy of the traversal's state
+    bool placed = false;    // the engine and its host mirrors live in caller memory
+    bool borrowed = false;  // every device buffer is carved from caller memory
+    // an asynchronous tick launched and not yet waited for
+    bool pending = false;
+    size_t node_alloc = 0;  // node capacity of the asynchronous tick (worst case of n)
+    hipStream_t async_s = nullptr;
+    hipEvent_t ev_async = nullptr;
     hipEvent_t ev_counts = nullptr;   // the tree's counts have reached h_small
 
     // arrays of the current tick
@@ -237,12 +271,16 @@ struct ps_hip_engine {
         for (DevBuf* b : all) release(*b);
         for (auto& es : ent)
             for (auto& b : es) release(b);
-        if (h_small) (void)hipHostFree(h_small);
+        if (!placed) {
+            if (h_small) (void)hipHostFree(h_small);
+            if (h_tstate) (void)hipHostFree(h_tstate);
+        }
         h_small = nullptr;
-        if (h_tstate) (void)hipHostFree(h_tstate);
         h_tstate = nullptr;
         if (ev_counts) (void)hipEventDestroy(ev_counts);
         ev_counts = nullptr;
+        if (ev_async) (void)hipEventDestroy(ev_async);
+        ev_async = nullptr;
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : ev_it)
@@ -258,6 +296,74 @@ struct ps_hip_engine {
 };
 
 namespace {
+
+constexpr uint32_t kMaxTravBlocks = 8192;  // PS_HIP_OPT_TRAV_BLOCKS upper bound
+
+// Every device buffer a tick can use, with its size for P particles and M
+// nodes: the buffers ps_hip_engine_create sizes at once, and the ones the
+// tick grows (traversal entries and arenas, P2P lists, host-array staging,
+// the correction's parameters).  With caller-owned device memory all of them
+// are carved from it at creation; the sizes are upper bounds of every need
+// below, so nothing grows later (a larger need, e.g. a list arena overflow,
+// is PS_HIP_EOOM).
+template <class F>
+void for_each_tick_buffer(ps_hip_engine* e, size_t P, size_t M, uint32_t arena_words, F&& f) {
+    f(e->sort_t, 2 * 6 * P * 4);
+    f(e->sort_tmp, ps_sort_tmp_bytes(P));
+    f(e->tree_tmp, ps_tree_tmp_bytes(P));
+    f(e->pos4_b, 16 * P);
+    f(e->leaf_of_b, 4 * P);
+    DevBuf* node16[] = {&e->geom_b, &e->mp_b, &e->loc_b, &e->ltot_b};
+    for (DevBuf* d : node16) f(*d, 16 * M);
+    f(e->oct_b, M);
+    f(e->link_b, 8 * M);
+    DevBuf* node4[] = {&e->nend_b, &e->parent_b, &e->p2p_off_b, &e->p2p_len_b, &e->p2p_moff_b};
+    for (DevBuf* d : node4) f(*d, 4 * M);
+    f(e->small_b, 256);
+    for (auto& es : e->ent)
+        for (auto& b : es) f(b, 4 * M);
+    f(e->far_b, 8 * (size_t)kMaxTravBlocks * PS_MAX_DEPTH_ITERS);
+    f(e->tstate_b, sizeof(PsTravState));
+    // the list arena with the slack a self-allocating engine's first
+    // allocation has (grow adds half): 144 words per particle by default
+    f(e->arena_b, 4 * ((size_t)arena_words * P * 3 / 2 + 4096));
+    f(e->masks_b, 8 * (5 * M + 4112));
+    f(e->big_b, 4 * (M + 2));
+    f(e->block_pairs_b, 16 * (size_t)ps_p2p_blocks(P, (uint32_t)std::min<size_t>(M, 0xFFFFFFFFu)));
+    for (auto& b : e->stage) f(b, 4 * P);
+    f(e->mlp_b, 4 * PS_MLP_PARAMS);
+    f(e->mlp_pk_b, PS_MLP_PACKED_BYTES);
+}
+
+size_t tick_scratch_bytes(size_t P, size_t M, uint32_t arena_words) {
+    ps_hip_engine probe;
+    size_t total = 0;
+    for_each_tick_buffer(&probe, P, M, arena_words,
+                         [&](DevBuf&, size_t b) { total += align256(std::max<size_t>(b, 1)); });
+    return total + 256;  // room to align the base
+}
+
+// Carve every tick buffer from [base, base + bytes).  The node arrays hold
+// the most nodes max_p particles can give, so the asynchronous tick (which
+// cannot look at the tree's size before it launches) always fits; the
+// caller's node capacity is still checked after each tick.
+int carve_all(ps_hip_engine* e, void* base, size_t bytes) {
+    const size_t M = worst_case_nodes(e->max_p);
+    const size_t need = tick_scratch_bytes(e->max_p, M, e->arena_words);
+    if (bytes < need) return PS_HIP_EOOM;
+    char* p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(base) + 255) & ~(uintptr_t)255);
+    for_each_tick_buffer(e, e->max_p, M, e->arena_words, [&](DevBuf& b, size_t sz) {
+        b.p = p;
+        b.bytes = align256(std::max<size_t>(sz, 1));
+        b.fixed = true;
+        p += b.bytes;
+    });
+    // the one buffer outside a tick (the correction's training scratch) gets
+    // no memory: with caller-owned memory it cannot grow either
+    e->mlp_tmp.fixed = true;
+    e->borrowed = true;
+    return PS_HIP_OK;
+}
 
 hipError_t alloc_fixed(ps_hip_engine* e) {
     const size_t P = e->max_p, M = e->node_cap;
@@ -282,11 +388,14 @@ hipError_t alloc_fixed(ps_hip_engine* e) {
     PS_AL(p2p_moff_b, 4 * M);
     PS_AL(small_b, 256);
 #undef PS_AL
-    r = hipHostMalloc(reinterpret_cast<void**>(&e->h_small), 256, hipHostMallocDefault);
+    return hipSuccess;
+}
+
+hipError_t alloc_mirrors(ps_hip_engine* e) {
+    if (e->placed) return hipSuccess;  // in the caller's host memory
+    hipError_t r = host_malloc(reinterpret_cast<void**>(&e->h_small), 256);
     if (r == hipSuccess)
-        r = hipHostMalloc(reinterpret_cast<void**>(&e->h_tstate), sizeof(PsTravState),
-                          hipHostMallocDefault);
-    if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_counts, hipEventDisableTiming);
+        r = host_malloc(reinterpret_cast<void**>(&e->h_tstate), sizeof(PsTravState));
     return r;
 }
 
@@ -543,6 +652,103 @@ int run_tick(ps_hip_engine* e) {
     return run_from_traverse(e);
 }
 
+// ---------------------------------------------------------------------------
+// The asynchronous tick: the same stages as run_tick with every size read
+// from device memory (as the batched tick does), so nothing waits on the
+// host between the launches.  The node arrays are sized for the most nodes
+// n particles can give (worst_case_nodes), so the tree's real size, known
+// only on the device, always fits; the traversal launches all
+// PS_MAX_DEPTH_ITERS iterations (the ones past the tree's depth find no
+// entries) and the L2L runs level by level over the node locals (bit for
+// bit the leaves' sums of the one-launch form).
+// ---------------------------------------------------------------------------
+
+int async_ensure(ps_hip_engine* e, size_t n) {
+    hipStream_t s = e->s;
+    const size_t M = worst_case_nodes(n);
+    DevBuf* node16[] = {&e->geom_b, &e->mp_b, &e->loc_b};
+    for (DevBuf* d : node16) PS_CK(grow(*d, 16 * M, s));
+    PS_CK(grow(e->link_b, 8 * M, s));
+    PS_CK(grow(e->oct_b, M, s));
+    DevBuf* node4[] = {&e->nend_b, &e->parent_b, &e->p2p_off_b, &e->p2p_len_b, &e->p2p_moff_b};
+    for (DevBuf* d : node4) PS_CK(grow(*d, 4 * M, s));
+    for (auto& es : e->ent)
+        for (auto& b : es) PS_CK(grow(b, 4 * M, s));
+    PS_CK(grow(e->far_b, 8 * (size_t)e->trav_blocks * PS_MAX_DEPTH_ITERS, s));
+    PS_CK(grow(e->tstate_b, sizeof(PsTravState), s));
+    PS_CK(grow(e->arena_b, 4 * ((size_t)e->arena_words * n + 4096), s));
+    PS_CK(grow(e->masks_b, 8 * (4 * M + 4096), s));
+    PS_CK(grow(e->big_b, 4 * (M + 2), s));
+    e->node_alloc = M;
+    return PS_HIP_OK;
+}
+
+// sort, build (locals and P2P list lengths cleared), upward
+int async_front(ps_hip_engine* e, hipStream_t cs) {
+    const size_t M = e->node_alloc;
+    uint32_t* ls = e->small_u32();
+    PsSortArrays io = {e->code, e->id, e->x, e->y, e->z, e->m};
+    PS_CK(ps_radix_sort(io, e->sort_set(0), e->sort_set(1), e->pos4(), e->n, e->sort_tmp.p, cs));
+    PS_CK(ps_tree_count(e->code, nullptr, e->n, ls, ls + 12, e->totals(), e->tree_tmp.p,
+                        e->leaf_cap, cs));
+    PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
+                        e->link(), e->nend(), e->parent(), e->leaf_of(), e->oct(), M,
+                        e->totals(), e->leaf_cap, cs));
+    ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
+                       e->totals(), M, cs);
+    ps_launch_upward_dev(ls, e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), cs);
+    return map_err(hipGetLastError());
+}
+
+// traversal (M2L + P2P lists), L2L, P2P + L2P, the correction; then the
+// counts and the traversal's state to the host mirrors, and ev_async
+int async_back(ps_hip_engine* e, hipStream_t cs) {
+    const uint32_t ub = (uint32_t)std::min<size_t>(e->node_alloc, 0xFFFFFFFFull);
+    uint32_t* ls = e->small_u32();
+    const float theta2 = theta_squared(e);
+    const bool mt = e->trav == PS_HIP_TRAV_MT;
+    PsTravState* st = reinterpret_cast<PsTravState*>(e->tstate_b.p);
+    const uint32_t arena_cap = (uint32_t)std::min<size_t>(e->arena_b.bytes / 4, 0xFFFFFFFFull);
+    const uint32_t mask_cap = (uint32_t)std::min<size_t>(e->masks_b.bytes / 8, 0xFFFFFFFFull);
+    ps_launch_trav_init(ls, mt ? 1 : 0, 1u, nullptr, e->nend(), e->link(), e->entries(0), st,
+                        e->arena(), arena_cap, mask_cap, ub, cs);
+    PsTravTree t;
+    t.n_nodes = ub;
+    t.geom = e->geom();
+    t.mp = e->mp();
+    t.link = e->link();
+    t.nend = e->nend();
+    t.loc = e->loc();
+    t.p2p_off = e->p2p_off();
+    t.p2p_len = e->p2p_len();
+    t.p2p_moff = e->p2p_moff();
+    unsigned long long* far_part = reinterpret_cast<unsigned long long*>(e->far_b.p);
+    size_t far_n = 0;
+    for (int k = 0; k < PS_MAX_DEPTH_ITERS; ++k) {
+        const int cur = k & 1, nxt = cur ^ 1;
+        const int lanes = trav_lanes_for(e, ub);
+        ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
+                            e->entries(nxt), far_part + far_n, e->kahan, lanes, e->trav_blocks,
+                            e->trav_spb, cs);
+        far_n += ps_trav_blocks(ub, lanes, e->trav_blocks);
+    }
+    ps_reduce_u64(far_part, far_n, &st->m2l_pairs, cs);
+    ps_launch_l2l_dev(ls, e->parent(), e->loc(), cs);
+    ps_launch_p2p(e->n, e->pos4(), e->leaf_of(), e->loc(), e->p2p_off(), e->p2p_len(),
+                  e->p2p_moff(), e->arena(), e->masks(), e->link(), e->nend(), e->fx, e->fy,
+                  e->fz, 0, e->kahan, ls, ub, e->p2p_big, reinterpret_cast<uint32_t*>(e->big_b.p),
+                  nullptr, nullptr, cs, e->s2, e->fork, e->join);
+    if (e->mlp_on)
+        ps_launch_mlp_apply(e->n, e->mlp_pk_b.p, e->pos4(), e->leaf_of(), e->geom(), e->mp(),
+                            e->loc(), e->parent(), e->fx, e->fy, e->fz, cs);
+    PS_CK(hipGetLastError());
+    PS_CK(hipMemcpyAsync(e->h_small, e->small_b.p, 96, hipMemcpyDeviceToHost, cs));
+    PS_CK(hipMemcpyAsync(e->h_tstate, e->tstate_b.p, sizeof(PsTravState), hipMemcpyDeviceToHost,
+                         cs));
+    PS_CK(hipEventRecord(e->ev_async, cs));
+    return PS_HIP_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -736,7 +942,7 @@ PrepWS* prep_ws(int dev, hipError_t* err) {
     if ((*err = hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking)) != hipSuccess ||
         (*err = grow(w->part, 1024 * sizeof(float2), w->s)) != hipSuccess ||
         (*err = grow(w->prm, 16, w->s)) != hipSuccess ||
-        (*err = hipHostMalloc(reinterpret_cast<void**>(&w->h_out), 64, hipHostMallocDefault)) !=
+        (*err = host_malloc(reinterpret_cast<void**>(&w->h_out), 64)) !=
             hipSuccess) {
         delete w;
         return nullptr;
@@ -752,44 +958,102 @@ PrepWS* prep_ws(int dev, hipError_t* err) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int ps_hip_engine_create(ps_hip_engine_t** out, size_t max_particles, size_t max_nodes,
-                         float theta, int traversal) {
+size_t ps_hip_engine_host_bytes(void) {
+    return align256(sizeof(ps_hip_engine)) + 256 + align256(sizeof(PsTravState)) + 256;
+}
+
+size_t ps_hip_engine_scratch_bytes(size_t max_particles) {
+    const size_t P = std::max<size_t>(max_particles, 1);
+    return tick_scratch_bytes(P, worst_case_nodes(P), 96);
+}
+
+size_t ps_hip_worst_case_nodes(size_t n) { return worst_case_nodes(n); }
+
+int ps_hip_engine_create_in(ps_hip_engine_t** out, size_t max_particles, size_t max_nodes,
+                            float theta, int traversal, void* host_mem, size_t host_bytes,
+                            void* dev_mem, size_t dev_bytes) {
     if (!out) return PS_HIP_EINVAL;
     *out = nullptr;
-    ps_hip_engine* e = new (std::nothrow) ps_hip_engine();
-    if (!e) return PS_HIP_EOOM;
+    if (dev_mem && !is_device_ptr(dev_mem)) return PS_HIP_EINVAL;
+    ps_hip_engine* e = nullptr;
+    char* host_tail = nullptr;
+    if (host_mem && host_bytes >= ps_hip_engine_host_bytes()) {
+        // the engine and its two host mirrors in the caller's memory
+        char* h = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(host_mem) + 255) &
+                                          ~(uintptr_t)255);
+        e = new (h) ps_hip_engine();
+        e->placed = true;
+        host_tail = h + align256(sizeof(ps_hip_engine));
+    } else {
+        e = new (std::nothrow) ps_hip_engine();
+        if (!e) return PS_HIP_EOOM;
+    }
     std::memset(&e->st, 0, sizeof(e->st));
     e->max_p = std::max<size_t>(max_particles, 1);
     e->node_cap = std::max<size_t>(std::min(max_nodes, worst_case_nodes(e->max_p)), 1);
     e->theta = theta > 0.0f ? theta : 1.0f;
     e->trav = traversal == PS_HIP_TRAV_ST ? PS_HIP_TRAV_ST : PS_HIP_TRAV_MT;
+    if (host_tail) {
+        e->h_small = reinterpret_cast<uint32_t*>(host_tail);
+        e->h_tstate = reinterpret_cast<PsTravState*>(host_tail + 256);
+    }
     hipError_t r = hipGetDevice(&e->device);
     int cus = 0;
     if (r == hipSuccess)
         r = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device);
-    if (r == hipSuccess && cus >= 8) e->trav_blocks = (uint32_t)cus & ~7u;
+    if (r == hipSuccess && cus >= 8) e->trav_blocks = std::min<uint32_t>((uint32_t)cus & ~7u,
+                                                                         kMaxTravBlocks);
     if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking);
     if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking);
     if (r == hipSuccess) r = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
     if (r == hipSuccess) r = hipEventCreateWithFlags(&e->join, hipEventDisableTiming);
+    if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_counts, hipEventDisableTiming);
+    if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_async, hipEventDisableTiming);
     for (int k = 0; k < 12 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev[k]);
     for (int k = 0; k < 64 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev_it[k]);
-    if (r == hipSuccess) r = alloc_fixed(e);
-    if (r != hipSuccess) {
+    int rc = PS_HIP_OK;
+    if (r == hipSuccess) r = alloc_mirrors(e);
+    if (r == hipSuccess) {
+        if (dev_mem)
+            rc = carve_all(e, dev_mem, dev_bytes);
+        else
+            r = alloc_fixed(e);
+    }
+    if (r != hipSuccess || rc != PS_HIP_OK) {
         (void)hipGetLastError();
         e->free_all();
-        delete e;
+        if (e->placed)
+            e->~ps_hip_engine();
+        else
+            delete e;
+        if (rc != PS_HIP_OK) return rc;
         return map_err(r) == PS_HIP_EINVAL ? PS_HIP_EDEVICE : map_err(r);
     }
     *out = e;
     return PS_HIP_OK;
 }
 
+int ps_hip_engine_create(ps_hip_engine_t** out, size_t max_particles, size_t max_nodes,
+                         float theta, int traversal) {
+    return ps_hip_engine_create_in(out, max_particles, max_nodes, theta, traversal, nullptr, 0,
+                                   nullptr, 0);
+}
+
 int ps_hip_engine_destroy(ps_hip_engine_t* e) {
     if (!e) return PS_HIP_EINVAL;
     if (e->s) (void)hipStreamSynchronize(e->s);
+    if (e->pending && e->async_s) (void)hipStreamSynchronize(e->async_s);
     e->free_all();
-    delete e;
+    if (e->placed)
+        e->~ps_hip_engine();
+    else
+        delete e;
+    return PS_HIP_OK;
+}
+
+int ps_hip_alloc_stats(uint64_t* calls, uint64_t* bytes) {
+    if (calls) *calls = g_alloc_calls.load(std::memory_order_relaxed);
+    if (bytes) *bytes = g_alloc_bytes.load(std::memory_order_relaxed);
     return PS_HIP_OK;
 }
 
@@ -1181,7 +1445,7 @@ static int bind_all(ps_hip_engine_t* e, float* x, float* y, float* z, float* mas
 int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* mass, float* fx,
                        float* fy, float* fz, uint32_t* id, size_t n, uint32_t* codes, float cx,
                        float cy, float cz, float hw) {
-    if (!e) return PS_HIP_EINVAL;
+    if (!e || e->pending) return PS_HIP_EINVAL;
     if (n == 0) return PS_HIP_OK;  // nothing to sort or build (H:1592)
     if (n > e->max_p) return PS_HIP_EINVAL;
     std::memset(&e->st, 0, sizeof(e->st));
@@ -1231,6 +1495,84 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
         e->st.ms_upload = e->elapsed(0, 1);
         e->st.ms_total = e->elapsed(0, 9);
     }
+    return PS_HIP_OK;
+}
+
+int ps_hip_calc_forces_async(ps_hip_engine_t* e, float* x, float* y, float* z, float* mass,
+                             float* fx, float* fy, float* fz, uint32_t* id, size_t n,
+                             uint32_t* codes, float cx, float cy, float cz, float hw,
+                             void* stream) {
+    if (!e || e->pending) return PS_HIP_EINVAL;
+    if (n == 0) return PS_HIP_OK;  // nothing to sort or build (H:1592)
+    if (n > e->max_p) return PS_HIP_EINVAL;
+    void* arrs[9] = {x, y, z, mass, fx, fy, fz, id, codes};
+    for (void* p : arrs)
+        if (!is_device_ptr(p)) return PS_HIP_EINVAL;  // host arrays would need a wait
+    PS_RC(async_ensure(e, n));
+    std::memset(&e->st, 0, sizeof(e->st));
+    e->n = n;
+    e->root = make_float4(cx, cy, cz, hw);
+    e->x = x;
+    e->y = y;
+    e->z = z;
+    e->m = mass;
+    e->fx = fx;
+    e->fy = fy;
+    e->fz = fz;
+    e->id = id;
+    e->code = codes;
+    e->bound = true;
+    e->built = false;
+    e->l2l_pending = false;
+    hipStream_t cs = reinterpret_cast<hipStream_t>(stream);
+    int rc = async_front(e, cs);
+    if (rc == PS_HIP_OK) rc = async_back(e, cs);
+    if (rc != PS_HIP_OK) {
+        (void)hipStreamSynchronize(cs);
+        return rc;
+    }
+    e->pending = true;
+    e->async_s = cs;
+    return PS_HIP_OK;
+}
+
+int ps_hip_calc_forces_wait(ps_hip_engine_t* e) {
+    if (!e) return PS_HIP_EINVAL;
+    if (!e->pending) return PS_HIP_OK;
+    e->pending = false;
+    PS_CK(hipEventSynchronize(e->ev_async));
+    const uint32_t flags = e->h_small[12];
+    const unsigned long long total =
+        *reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(e->h_small) + 64);
+    if (flags & 1u) return PS_HIP_EINVAL;  // codes must be 30-bit Morton codes
+    if (total > e->node_cap) return PS_HIP_EOOM;  // the caller's node capacity (H:1742-1743)
+    for (int attempt = 0;; ++attempt) {
+        const PsTravState hst = *e->h_tstate;
+        if (!hst.overflow) break;
+        // an arena was too small: grow past the need and run the tick again
+        // from the traversal (the sorted arrays and the tree stand)
+        if (attempt >= 4 || hst.arena_ctr > 0xFFFFFFFFull || hst.mask_ctr > 0xFFFFFFFFull)
+            return PS_HIP_EOOM;
+        PS_CK(grow(e->arena_b, std::max<size_t>(2 * e->arena_b.bytes, 5 * hst.arena_ctr), e->s));
+        PS_CK(grow(e->masks_b, std::max<size_t>(2 * e->masks_b.bytes, 10 * hst.mask_ctr), e->s));
+        ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
+                           e->totals(), e->node_alloc, e->async_s);
+        PS_RC(async_back(e, e->async_s));
+        PS_CK(hipEventSynchronize(e->ev_async));
+    }
+    std::memcpy(e->ls, e->h_small, sizeof(e->ls));
+    e->n_nodes = (uint32_t)total;
+    e->max_depth = 0;
+    for (int d = 0; d <= 10; ++d)
+        if (e->ls[d + 1] > e->ls[d]) e->max_depth = (uint32_t)d;
+    e->built = true;
+    e->st.n_particles = e->n;
+    e->st.n_nodes = e->n_nodes;
+    e->st.max_depth = e->max_depth;
+    std::memcpy(e->st.level_start, e->ls, sizeof(e->ls));
+    e->st.m2l_pairs = e->h_tstate->m2l_pairs;
+    e->st.list_entries = e->h_tstate->arena_ctr - 1;
+    e->st.iterations = e->max_depth + 1;
     return PS_HIP_OK;
 }
 
@@ -1321,7 +1663,7 @@ int ps_hip_op_m2l(const float* tgeom4, const float* sgeom4, const float* smp4, f
                   float* out4) {
     if (!tgeom4 || !sgeom4 || !smp4 || !out4) return PS_HIP_EINVAL;
     float* d = nullptr;
-    PS_CK(hipMalloc(reinterpret_cast<void**>(&d), 16));
+    PS_CK(dev_malloc(reinterpret_cast<void**>(&d), 16));
     hipError_t r = ps_op_m2l(tgeom4, sgeom4, smp4, theta, d, nullptr);
     if (r == hipSuccess) r = hipMemcpy(out4, d, 16, hipMemcpyDeviceToHost);
     (void)hipFree(d);
@@ -1331,7 +1673,7 @@ int ps_hip_op_m2l(const float* tgeom4, const float* sgeom4, const float* smp4, f
 int ps_hip_op_morton_exhaustive(uint64_t* out2) {
     if (!out2) return PS_HIP_EINVAL;
     unsigned long long* d = nullptr;
-    PS_CK(hipMalloc(reinterpret_cast<void**>(&d), 16));
+    PS_CK(dev_malloc(reinterpret_cast<void**>(&d), 16));
     hipError_t r = ps_op_morton_exhaustive(d, nullptr);
     if (r == hipSuccess) r = hipMemcpy(out2, d, 16, hipMemcpyDeviceToHost);
     (void)hipFree(d);
@@ -1366,9 +1708,9 @@ int ps_hip_direct_f64(const float* x, const float* y, const float* z, const floa
     float4* pos4 = nullptr;
     uint32_t* dt = nullptr;
     double* dout = nullptr;
-    hipError_t r = hipMalloc(reinterpret_cast<void**>(&pos4), 16 * n);
-    if (r == hipSuccess) r = hipMalloc(reinterpret_cast<void**>(&dt), 4 * nt);
-    if (r == hipSuccess) r = hipMalloc(reinterpret_cast<void**>(&dout), 24 * nt);
+    hipError_t r = dev_malloc(reinterpret_cast<void**>(&pos4), 16 * n);
+    if (r == hipSuccess) r = dev_malloc(reinterpret_cast<void**>(&dt), 4 * nt);
+    if (r == hipSuccess) r = dev_malloc(reinterpret_cast<void**>(&dout), 24 * nt);
     if (r == hipSuccess) r = hipMemcpy(dt, targets, 4 * nt, hipMemcpyHostToDevice);
     if (r == hipSuccess) {
         ps_launch_pack_pos4(x, y, z, mass, n, pos4, nullptr);
@@ -1422,6 +1764,7 @@ int ps_hip_step_frame(ps_hip_engine_t* e, uint32_t* id, float* vx, float* vy, fl
 }
 
 void* ps_hip_malloc(size_t bytes) {
+    // the caller's own memory: not counted among the library's allocations
     void* p = nullptr;
     if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
         (void)hipGetLastError();
@@ -1445,6 +1788,23 @@ int ps_hip_memcpy(void* dst, const void* src, size_t bytes, int kind) {
 }
 
 int ps_hip_synchronize(void) { return map_err(hipDeviceSynchronize()); }
+
+int ps_hip_stream_create(void** out) {
+    if (!out) return PS_HIP_EINVAL;
+    hipStream_t s = nullptr;
+    PS_CK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = s;
+    return PS_HIP_OK;
+}
+
+int ps_hip_stream_destroy(void* stream) {
+    if (!stream) return PS_HIP_EINVAL;
+    return map_err(hipStreamDestroy(reinterpret_cast<hipStream_t>(stream)));
+}
+
+int ps_hip_stream_synchronize(void* stream) {
+    return map_err(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+}
 
 int ps_hip_device_name(char* buf, size_t len) {
     if (!buf || len == 0) return PS_HIP_EINVAL;

@@ -18,8 +18,8 @@ def declared(header):
     text = open(os.path.join(INCLUDE, header)).read()
     text = text.split("#ifdef POLESITTER_IMPLEMENTATION")[0]
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^\s*(?:int|void\s*\*|ps_result_t|struct\s+ps_hip_engine\s*\*)\s*"
-                       r"(\w+)\s*\(", text, flags=re.M)
+    names = re.findall(r"^\s*(?:int|size_t|void\s*\*|ps_result_t|struct\s+ps_hip_engine\s*\*)"
+                       r"\s*(\w+)\s*\(", text, flags=re.M)
     return sorted(set(names))
 
 
@@ -76,6 +76,25 @@ def test_struct_layout_matches_reference():
     assert [f[0] for f in api.PsParticleArrs._fields_] == ["x", "y", "z", "mass", "fx", "fy",
                                                            "fz", "id", "cnt"]
     assert C.sizeof(api.PsParticleArrs) == 72
+    # the extension keeps ps_config_t whole as its first member
+    assert api.PsConfigEx.base.offset == 0 and api.PsConfigEx.dev_buff.offset == 40
+    assert C.sizeof(api.PsConfigEx) == 56
+
+
+def test_extended_config_leaves_reference_initialisers_warning_free(tmp_path):
+    # appending fields to ps_config_t itself would make the reference's
+    # 5-value initialisers miss fields, which -Wextra reports; the extension
+    # is a separate struct, so both forms compile with -Werror
+    src = tmp_path / "x.c"
+    src.write_text('#define POLESITTER_IMPLEMENTATION\n#include "polesitter.h"\n'
+                   'int main(void){ static char b[1<<20]; ps_context_t* c = 0;\n'
+                   ' ps_config_t cfg = {b, sizeof b, 4, 2.0F, 4};\n'
+                   ' ps_config_ex_t ex = {{b, sizeof b, 4, 2.0F, 4}, 0, 0};\n'
+                   ' (void)cfg; return ps_init_ex(&c, 0) == PS_EINVAL && ex.dev_buff == 0 ? 0 : 1; }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                        f"-I{INCLUDE}", "-c", str(src), "-o", str(tmp_path / "x.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
 
 
 def test_header_compiles_as_c99_with_positional_initialisers(tmp_path):

@@ -10,7 +10,7 @@ semi-implicit Euler dt = 1e-3, ids reset for the next frame; one fused call),
 so every tick re-sorts and rebuilds.
 
     python bench.py [--steps K] [--warmup W]
-                    [--workload plummer1m|twoplummer16m|plummer4m|ensemble512|mlp4m]
+                    [--workload plummer1m|twoplummer16m|plummer4m|ensemble512|mlp4m|sweep]
 
 Prints one JSON line.  `value` = N * K / (wall time of K steps), bracketed
 by device synchronisation.  `roofline` is the dominant kernel's algorithmic
@@ -370,6 +370,123 @@ def bench_ensemble(a):
     return 0
 
 
+# tests/benchmark.c:136-137, and the reference's published times for them
+# (seconds; docs/performance_graph_log.svg, hardware not stated): direct
+# FP32 O(N^2), FMM thrd_cnt=1, FMM thrd_cnt=8, all at theta = 1.0
+SWEEP_SIZES = (1000, 2500, 5000, 10000, 15000, 20000, 25000, 50000, 75000, 100000, 150000,
+               200000)
+PUBLISHED_SWEEP = {
+    1000: (0.000315, 0.000459, 0.000341), 2500: (0.002507, 0.003155, 0.000596),
+    5000: (0.007735, 0.003078, 0.003785), 10000: (0.02789, 0.00425, 0.004522),
+    15000: (0.06154, 0.006603, 0.003075), 20000: (0.1097, 0.008098, 0.00389),
+    25000: (0.1719, 0.01012, 0.005192), 50000: (0.6908, 0.01958, 0.01788),
+    75000: (1.567, 0.02886, 0.01484), 100000: (2.866, 0.03768, 0.01975),
+    150000: (6.924, 0.05461, 0.03162), 200000: (11.95, 0.07593, 0.04226)}
+
+
+def bench_sweep(a):
+    """tests/benchmark.c's sweep through the drop-in: N from 1,000 to
+    200,000 in a uniform cube [0,100)^3 (glibc rand(), seed 7331), mass 1,
+    the threaded build's traversal, at theta = 1.0 (the benchmark's) and 2.0.
+    Per point: prepare + calc_forces on device arrays (median of --steps) and
+    on host arrays as the benchmark passes them (median of 3), the
+    reference's CPU path at thrd_cnt = 1, 8 and every usable CPU (median of
+    3, after one warm-up), the published times, and the FP64 error of the
+    GPU and of the reference.  At theta = 1.0 the reference's depth-2
+    traversal roots make a target two levels below a containing source
+    'far' (dist^2 = 1.6875 hw0^2 > 1.5625 hw0^2 for the corner depth-2 cells
+    against the root), so those cells take the root's M2L and skip their
+    near field: the GPU reproduces the same interaction lists, and the FP64
+    error shows it.  value = GPU particles/s at N = 200,000, theta = 1.0."""
+    from polesitter_amd import api
+    from polesitter_amd import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    L = api.hip_lib()
+    info = host_cpus()
+    cpu_thr = sorted({1, 8, info["usable"]})
+    mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
+    rows = []
+    log(f"bench: {api.device_name()} workload sweep: benchmark.c sizes, theta 1.0 and 2.0")
+    for theta in (1.0, 2.0):
+        for n in SWEEP_SIZES:
+            x, y, z, m = W.benchmark_cube(n)
+            row = dict(n=n, theta=theta)
+            with api.Solver(n, theta=theta, stats=True) as s:
+                s.load(x, y, z, m)
+                s.tick(mmode)
+                st = s.stats()
+                r = s.result()
+                f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
+                row["gpu_fp64_rms_err"] = fp64_error(n, f, r["id"], s.direct_f64)[0]
+                row["m2l_pairs"], row["p2p_pairs"] = int(st["m2l_pairs"]), int(st["p2p_pairs"])
+                api.check("opt", api.ps_set_option(s.ctx, api.OPT_STATS, 0))
+                ts = []
+                for _ in range(max(a.steps, 3)):
+                    s.load(x, y, z, m)
+                    api.check("sync", L.ps_hip_synchronize())
+                    t0 = time.perf_counter()
+                    s.tick(mmode)  # prepare + calc_forces, synchronous
+                    ts.append(time.perf_counter() - t0)
+                row["gpu_s"] = float(np.median(ts))
+                # host arrays, as the benchmark passes them
+                hx, hy, hz, hm = (np.array(t, np.float32) for t in (x, y, z, m))
+                hf = [np.zeros(n, np.float32) for _ in range(3)]
+                hid, hcodes = np.arange(n, dtype=np.uint32), np.zeros(n, np.uint32)
+                harrs = api.make_arrs(hx, hy, hz, hm, hf[0], hf[1], hf[2], hid, n)
+                th = []
+                for _ in range(4):
+                    hx[:], hy[:], hz[:], hid[:] = x, y, z, np.arange(n, dtype=np.uint32)
+                    t0 = time.perf_counter()
+                    rc, lo, _, rg = api.ps_prepare_particles(harrs, hcodes, mmode)
+                    api.check("ps_prepare_particles", rc)
+                    c = np.float32(np.float32(lo) + np.float32(rg) / np.float32(2.0))
+                    api.check("ps_calc_forces", api.ps_calc_forces(
+                        s.ctx, harrs, hcodes, c, c, c, np.float32(rg) / np.float32(2.0)))
+                    th.append(time.perf_counter() - t0)
+                row["gpu_host_arrays_s"] = float(np.median(th[1:]))
+
+                def ref_fn(rr, pos):
+                    with api.Solver(n) as t:
+                        t.load(rr["x"], rr["y"], rr["z"], rr["m"])
+                        return t.direct_f64(pos)
+                if O.ref_available("shipped_mt"):
+                    for thr in cpu_thr:
+                        build = "shipped_mt" if thr <= 33 else "shipped_mt_wide"
+                        secs = O.ref_time_ticks(x, y, z, m, theta=theta, thrd_cnt=thr, n_ticks=4,
+                                                build=build)
+                        row[f"ref_{thr}thr_s"] = float(np.median(secs[1:]))
+                    rr = O.ref_tick(x, y, z, m, theta=theta, thrd_cnt=8, build="shipped_mt",
+                                    dump=False)
+                    fr = np.stack([rr["fx"], rr["fy"], rr["fz"]], 1)
+                    row["ref_fp64_rms_err"] = fp64_error(n, fr, rr["id"],
+                                                         lambda pos: ref_fn(rr, pos))[0]
+            if theta == 1.0:
+                d, p1, p8 = PUBLISHED_SWEEP[n]
+                row.update(published_direct_s=d, published_1core_s=p1, published_8core_s=p8)
+            row["gpu_particles_per_s"] = round(n / row["gpu_s"], 1)
+            rows.append(row)
+            log(f"bench: sweep theta={theta} n={n}: gpu {1e3 * row['gpu_s']:.3f} ms, host arrays "
+                f"{1e3 * row['gpu_host_arrays_s']:.3f} ms, ref 1 thr "
+                f"{1e3 * row.get('ref_1thr_s', float('nan')):.2f} ms, FP64 rms gpu "
+                f"{row['gpu_fp64_rms_err']:.3e} ref {row.get('ref_fp64_rms_err', float('nan')):.3e}")
+    head = next(r for r in rows if r["n"] == 200000 and r["theta"] == 1.0)
+    line = dict(metric=METRIC, value=head["gpu_particles_per_s"], unit="particles/s", n_gpus=1,
+                steps=max(a.steps, 3), warmup=1, ms_per_step=round(1e3 * head["gpu_s"], 4),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+                data="synthetic",
+                config=dict(workload="sweep", sizes=list(SWEEP_SIZES), thetas=[1.0, 2.0],
+                            description="tests/benchmark.c: uniform [0,100)^3, glibc rand() "
+                                        "seed 7331, mass 1, PS_MULTITHREADING traversal",
+                            step="prepare + calc_forces (device-resident arrays; host arrays "
+                                 "in gpu_host_arrays_s)", morton=a.morton),
+                cpu_threads=cpu_thr, host=info, sweep=rows,
+                note="published_* are the reference's plot values at theta=1.0 on unstated "
+                     "hardware; ref_*thr_s are the reference built here, timed on this host")
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def step_roofline(n, nodes, m2l, p2p, stages_ms, ms_step):
     """The whole step against its roofline: t_roof = sum over the stages of
     max(algorithmic bytes / 8 TB/s, algorithmic FLOP / 157.3 TFLOP/s), with
@@ -613,6 +730,8 @@ def main(argv=None):
         return bench_ensemble(a)
     if a.workload == "mlp4m":
         return bench_mlp(a)
+    if a.workload == "sweep":
+        return bench_sweep(a)
     r = measure_plain(a, a.workload, a.steps, a.warmup, a.cpu_ticks, a.host_ticks,
                       not a.no_cpu_baseline)
     line = dict(metric=METRIC, value=r.pop("value"), unit="particles/s", n_gpus=1,

@@ -149,11 +149,26 @@ ps_result_t ps_calc_forces_wait(ps_context_t* ctx);
 #define PS_MORTON_MODE PS_HIP_MORTON_TRUNC
 #endif
 
+/* Compatibility with callers of the reference's internals: its demos clear
+ * the node arena before every frame, ps_impl_arena_clear(&ctx->arena)
+ * (demo/unary.c:120, demo/binary.c:158; H:878-880).  The device rebuilds
+ * its tree in every tick, so clearing only resets this offset. */
+typedef struct {
+    size_t offset;
+} ps_impl_arena_t;
+
+static inline void ps_impl_arena_clear(ps_impl_arena_t* arena) {
+    if (arena) {
+        arena->offset = 0;
+    }
+}
+
 struct ps_context {
     struct ps_hip_engine* engine;
     float                 theta;
     size_t                max_particles;
     size_t                max_nodes;
+    ps_impl_arena_t       arena;
 };
 
 static size_t ps_impl_align_forward(size_t v, size_t align) {
@@ -177,6 +192,7 @@ static ps_result_t ps_impl_init(ps_context_t** out_ctx, const ps_config_t* cfg,
     }
     ps_context_t* ctx = (ps_context_t*)cfg->buff;
     ctx->engine        = NULL;
+    ctx->arena.offset  = 0;
     ctx->theta         = (cfg->theta > 0.0F) ? cfg->theta : 1.0F;
     ctx->max_particles = cfg->max_particles;
 

@@ -8,6 +8,7 @@ Outputs (git-ignored, shipped with the source tree):
     polesitter_amd/lib/libpolesitter_hip.so   HIP kernels + C-ABI (ps_hip_*)
     polesitter_amd/lib/libpolesitter.so       C host layer of include/polesitter.h
     polesitter_amd/lib/dropin_tick            the README loop as a C program
+    polesitter_amd/lib/demo_unary, demo_binary the demos' frame loops, headless
 """
 from __future__ import annotations
 
@@ -88,6 +89,14 @@ def build_host(verbose=False):
           os.path.join(CSRC, "dropin_tick.c"), f"-L{LIBDIR}", "-lpolesitter_hip", "-lm",
           "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"])
     os.replace(exe + ".tmp", exe)
+    # the reference demos' frame loops without their window (unary: the
+    # threaded traversal; binary: the single-threaded one)
+    for name, flag in (("demo_unary", "-DPS_DEMO_UNARY"), ("demo_binary", "-DPS_DEMO_BINARY")):
+        out = os.path.join(LIBDIR, name)
+        _run(["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-pedantic", flag, f"-I{INCLUDE}",
+              os.path.join(CSRC, "demo_headless.c"), f"-L{LIBDIR}", "-lpolesitter_hip", "-lm",
+              "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
     return host_so
 
 

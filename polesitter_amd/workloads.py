@@ -18,6 +18,44 @@ F32 = np.float32
 
 
 # ---------------------------------------------------------------------------
+# glibc rand(): the input of the reference's benchmark (tests/benchmark.c)
+# ---------------------------------------------------------------------------
+
+def glibc_rand(seed: int, count: int) -> np.ndarray:
+    """The first ``count`` values of glibc's rand() after srand(seed): its
+    default additive feedback generator (TYPE_3, degree 31, separation 3).
+    The state r[0..30] is seeded by r[0] = seed, r[i] = 16807 r[i-1] mod
+    (2^31 - 1); r[31..33] = r[0..2]; then r[i] = r[i-3] + r[i-31] mod 2^32,
+    the first 310 of those are discarded, and each output is r[i] >> 1."""
+    seed = int(seed) & 0xFFFFFFFF
+    r = [seed if seed != 0 else 1]
+    for i in range(1, 31):
+        # Schrage's method, signed 32-bit as in glibc
+        prev = r[i - 1] if r[i - 1] < 2 ** 31 else r[i - 1] - 2 ** 32
+        hi, lo = int(prev / 127773), prev - int(prev / 127773) * 127773
+        word = 16807 * lo - 2836 * hi
+        if word < 0:
+            word += 2147483647
+        r.append(word & 0xFFFFFFFF)
+    for i in range(31, 34):
+        r.append(r[i - 31])
+    total = 344 + int(count)
+    for i in range(34, total):
+        r.append((r[i - 31] + r[i - 3]) & 0xFFFFFFFF)
+    return (np.array(r[344:total], dtype=np.uint64) >> np.uint64(1)).astype(np.uint32)
+
+
+def benchmark_cube(n: int, seed: int = 7331):
+    """tests/benchmark.c:158-165: srand(7331); for each particle
+    x, y, z = ((float)rand() / RAND_MAX) * 100.0F in turn; mass 1."""
+    v = glibc_rand(seed, 3 * int(n)).astype(F32)  # (float)rand(), round to nearest
+    u = (v / F32(2147483647.0)) * F32(100.0)        # RAND_MAX converted to float
+    u = u.astype(F32).reshape(int(n), 3)
+    return (np.ascontiguousarray(u[:, 0]), np.ascontiguousarray(u[:, 1]),
+            np.ascontiguousarray(u[:, 2]), np.ones(int(n), F32))
+
+
+# ---------------------------------------------------------------------------
 # the reference's LCG generators (tests/accuracy.c)
 # ---------------------------------------------------------------------------
 

@@ -311,7 +311,8 @@ def bench_ensemble(a):
     """512 independent systems (N_s log-uniform in [2048, 262144], even ones
     uniform cubes, odd ones Plummer with a ~ U[0.5, 2], masses 1/N_s) in one
     batched tick, recorded as a HIP graph and replayed; settings from the
-    autotuner at the reference setting's FP64 error.  One step = one batched
+    autotuner (leaf capacity per system-size class, P2P tile and lanes per
+    batch) at each class's reference-setting FP64 error.  One step = one batched
     tick (prepare + calc_forces of every system); value = total particles /
     tick."""
     from polesitter_amd import api, autotune
@@ -321,14 +322,15 @@ def bench_ensemble(a):
     mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
     log(f"bench: {api.device_name()} workload ensemble512: 512 systems, {n} particles")
     if a.no_tune:  # the reference setting, for profiling runs
-        cfg = dict(leaf_cap=1, p2p_big=24, trav_lanes=8)
-        tuned = dict(config=cfg, reference=None, ms_tick=None, fp64_rms_err=None, table=[])
+        cfg = dict(caps=[1, 1, 1, 1], p2p_big=24, trav_lanes=8)
+        tuned = dict(config=cfg, system_caps=[1] * len(systems), reference=None, ms_tick=None,
+                     fp64_rms_err=None, class_errors=None, table=[], classes=None)
     else:
         tuned = autotune.tune_batch(systems, theta=a.theta, mode=mmode, log=log)
         cfg = tuned["config"]
     L = api.hip_lib()
     with api.Batch(n, len(systems), theta=a.theta, graph=True, kahan=a.kahan) as b:
-        b.set_option(api.OPT_LEAF_CAP, cfg["leaf_cap"])
+        b.set_leaf_caps(tuned["system_caps"])
         b.set_option(api.OPT_P2P_BIG, cfg["p2p_big"])
         b.set_option(api.OPT_TRAV_LANES, cfg["trav_lanes"])
         if a.trav_blocks:
@@ -363,8 +365,15 @@ def bench_ensemble(a):
                 counts=dict(nodes=int(st["n_nodes"]), m2l_pairs=int(st["m2l_pairs"])),
                 autotune=dict(reference=tuned["reference"], chosen_ms=tuned["ms_tick"],
                               chosen_fp64_rms_err=tuned["fp64_rms_err"],
-                              candidates=[dict(c, ms=round(ms_, 3), err=round(e_, 6))
-                                          for c, ms_, e_ in tuned["table"]],
+                              chosen_class_errors=tuned["class_errors"],
+                              classes=tuned["classes"],
+                              speedup_vs_reference_setting=(
+                                  None if a.no_tune else
+                                  round(tuned["reference"]["ms_tick"] / tuned["ms_tick"], 4)),
+                              candidates=[dict(c, ms=round(ms_, 3), err=round(e_, 6),
+                                               class_errors={str(k): round(v, 6)
+                                                             for k, v in per.items()})
+                                          for c, ms_, per, e_ in tuned["table"]],
                               skipped=bool(a.no_tune)))
     print(json.dumps(line), flush=True)
     return 0

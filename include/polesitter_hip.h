@@ -287,6 +287,13 @@ int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z,
                       uint32_t* id, uint32_t* codes, const uint32_t* offsets,
                       size_t n_systems, int morton_mode);
 int ps_hip_batch_get_stats(const ps_hip_batch_t* b, ps_hip_batch_stats_t* out);
+/* no reference counterpart: a leaf capacity per system (host array of
+ * n_systems values in 1..64, the reference's rule is 1) in place of
+ * PS_HIP_OPT_LEAF_CAP for the batch's later ticks, which must have at most
+ * n_systems systems; NULL restores the option.  System k's tree is the one a
+ * single tick with PS_HIP_OPT_LEAF_CAP = caps[k] builds. */
+int ps_hip_batch_set_leaf_caps(ps_hip_batch_t* b, const uint8_t* caps,
+                               size_t n_systems);
 
 /* no reference counterpart: the learned far-field correction.  A small MLP
  * (24 -> 64 -> 64 -> 64 -> 3, GELU, FP32) maps per-particle leaf-local

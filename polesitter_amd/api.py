@@ -130,6 +130,7 @@ def hip_lib():
             "ps_hip_batch_set_option": ([_vp, C.c_int, C.c_double], C.c_int),
             "ps_hip_batch_tick": ([_vp] * 11 + [_sz, C.c_int], C.c_int),
             "ps_hip_batch_get_stats": ([_vp, C.POINTER(PsBatchStats)], C.c_int),
+            "ps_hip_batch_set_leaf_caps": ([_vp, _vp, _sz], C.c_int),
             "ps_hip_direct_f64": ([_vp, _vp, _vp, _vp, _sz, _vp, _sz, _vp], C.c_int),
             "ps_hip_step_reset_ids": ([_vp, _vp, _sz], C.c_int),
             "ps_hip_step_gather": ([_vp, _vp, _vp, _vp, _vp, _vp, _sz], C.c_int),
@@ -563,6 +564,16 @@ class Batch:
 
     def set_option(self, key, value):
         check("ps_hip_batch_set_option", hip_lib().ps_hip_batch_set_option(self.h, key, value))
+
+    def set_leaf_caps(self, caps):
+        """a leaf capacity per system (None: the OPT_LEAF_CAP option)"""
+        if caps is None:
+            check("ps_hip_batch_set_leaf_caps", hip_lib().ps_hip_batch_set_leaf_caps(self.h, None,
+                                                                                     0))
+            return
+        c = np.ascontiguousarray(caps, np.uint8)
+        check("ps_hip_batch_set_leaf_caps", hip_lib().ps_hip_batch_set_leaf_caps(
+            self.h, c.ctypes.data, c.size))
 
     def load(self, systems):
         """systems: list of (x, y, z, m) float32 arrays; ids = index inside the system."""

@@ -761,6 +761,8 @@ struct ps_hip_batch {
     DevBuf sys_b;                // u32 per particle: its system (the sort's second key)
     DevBuf aux_b;                // two u32 arrays: the second key's sort scratch
     DevBuf prm_b, roots_b;       // float4 per system
+    DevBuf caps_b;               // u8 per system: leaf capacities (when caps_n > 0)
+    size_t caps_n = 0;           // systems covered by caps_b (0: the engine option)
     size_t node_cap = 0;         // guaranteed bound for the current layout
     int use_graph = 1;
     hipGraph_t graph = nullptr;
@@ -838,11 +840,12 @@ int batch_run_front(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
     t0.aux = reinterpret_cast<uint32_t*>(b->aux_b.p);
     t1.aux = t0.aux + e->max_p;
     PS_CK(ps_radix_sort(io, t0, t1, e->pos4(), n, e->sort_tmp.p, s));
+    const uint8_t* caps = b->caps_n ? reinterpret_cast<const uint8_t*>(b->caps_b.p) : nullptr;
     PS_CK(ps_tree_count(e->code, sys, n, ls, ls + 12, e->totals(), e->tree_tmp.p, e->leaf_cap,
-                        s));
+                        s, caps));
     PS_CK(ps_tree_write(e->code, sys, n, make_float4(0.f, 0.f, 0.f, 0.f), roots, e->tree_tmp.p,
                         e->geom(), e->link(), e->nend(), e->parent(), e->leaf_of(), e->oct(),
-                        cap, e->totals(), e->leaf_cap, s));
+                        cap, e->totals(), e->leaf_cap, s, caps));
     ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
                        e->totals(), cap, s);
     ps_launch_upward_dev(ls, e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), s);
@@ -1232,7 +1235,7 @@ int ps_hip_batch_destroy(ps_hip_batch_t* b) {
     if (!b) return PS_HIP_EINVAL;
     if (b->e && b->e->s) (void)hipStreamSynchronize(b->e->s);
     b->drop_graph();
-    DevBuf* bufs[] = {&b->off_b, &b->sys_b, &b->aux_b, &b->prm_b, &b->roots_b};
+    DevBuf* bufs[] = {&b->off_b, &b->sys_b, &b->aux_b, &b->prm_b, &b->roots_b, &b->caps_b};
     for (DevBuf* d : bufs) release(*d);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1254,6 +1257,22 @@ int ps_hip_batch_set_option(ps_hip_batch_t* b, int key, double value) {
     return ps_hip_engine_set_option(b->e, key, value);
 }
 
+int ps_hip_batch_set_leaf_caps(ps_hip_batch_t* b, const uint8_t* caps, size_t n_systems) {
+    if (!b || n_systems > b->max_sys) return PS_HIP_EINVAL;
+    b->drop_graph();  // the recorded build carries the old capacities
+    if (!caps || n_systems == 0) {
+        b->caps_n = 0;
+        return PS_HIP_OK;
+    }
+    for (size_t k = 0; k < n_systems; ++k)
+        if (caps[k] < 1 || caps[k] > 64) return PS_HIP_EINVAL;
+    PS_CK(grow(b->caps_b, b->max_sys, b->e->s));
+    PS_CK(hipMemcpyAsync(b->caps_b.p, caps, n_systems, hipMemcpyHostToDevice, b->e->s));
+    PS_CK(hipStreamSynchronize(b->e->s));
+    b->caps_n = n_systems;
+    return PS_HIP_OK;
+}
+
 int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z, float* mass, float* fx,
                       float* fy, float* fz, uint32_t* id, uint32_t* codes,
                       const uint32_t* offsets, size_t n_systems, int morton_mode) {
@@ -1266,6 +1285,7 @@ int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z, float* ma
     const size_t n = offsets[n_systems];
     ps_hip_engine* e = b->e;
     if (n > e->max_p) return PS_HIP_EINVAL;
+    if (b->caps_n && n_systems > b->caps_n) return PS_HIP_EINVAL;  // a system without a cap
     void* arrs[9] = {x, y, z, mass, fx, fy, fz, id, codes};
     for (void* p : arrs)
         if (!is_device_ptr(p)) return PS_HIP_EINVAL;  // a batch works on device arrays

@@ -87,10 +87,12 @@ size_t ps_tree_tmp_bytes(size_t n);
 // sys (optional): the system of each particle (sorted by (system, code));
 // each system then gets its own root, in system order at depth 0.
 // leaf_cap: a node above depth 10 is a leaf iff it holds <= leaf_cap
-// particles (the reference's rule is leaf_cap = 1).
+// particles (the reference's rule is leaf_cap = 1); with sys and sys_cap
+// set, system k uses sys_cap[k] instead (device array, 1..64 per system).
 hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
                          uint32_t* level_start, uint32_t* flags, unsigned long long* total64,
-                         void* tmp, int leaf_cap, hipStream_t s);
+                         void* tmp, int leaf_cap, hipStream_t s,
+                         const uint8_t* sys_cap = nullptr);
 // Phase 2: writes every node (needs phase 1's tmp untouched).  The root cell
 // is `root`, or roots[sys[i]] per system when roots is set.
 // n_nodes_dev (optional): the node count in device memory (then n_nodes is
@@ -101,7 +103,7 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
                          const float4* roots, void* tmp, float4* geom, uint2* link,
                          uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, uint8_t* octant,
                          size_t n_nodes, const unsigned long long* n_nodes_dev, int leaf_cap,
-                         hipStream_t s);
+                         hipStream_t s, const uint8_t* sys_cap = nullptr);
 // Upward (depths 10..0) and L2L (depths 1..10) with the level starts read
 // from device memory ls[0..11]; fixed grids, no host-side sizes.
 void ps_launch_upward_dev(const uint32_t* ls, const float4* pos4, const float4* geom,

@@ -44,15 +44,18 @@ struct PartDepths {
 
 // sys (optional): system of each particle; a system boundary shares no digit,
 // so every system opens its own root (the forest of a batched tick).
-// leaf_cap: most particles of a leaf above depth 10 (the reference: 1).  The
+// leaf_cap: most particles of a leaf above depth 10 (the reference: 1), or
+// sys_cap[system] per system when sys_cap is set (a forest).  The
 // node of depth d around i holds more than leaf_cap particles iff some
 // window of leaf_cap + 1 consecutive particles containing i shares d digits,
 // so i's leaf depth is 1 + the largest lcp(j, j + leaf_cap), j in
 // [i - leaf_cap, i]; with leaf_cap = 1 that is the rule above.
 __device__ __forceinline__ PartDepths part_depths(const uint32_t* codes,
                                                   const uint32_t* __restrict__ sys, size_t n,
-                                                  size_t i, uint32_t c, int leaf_cap) {
+                                                  size_t i, uint32_t c, int leaf_cap,
+                                                  const uint8_t* __restrict__ sys_cap) {
     const uint32_t si = sys ? sys[i] : 0u;
+    if (sys && sys_cap) leaf_cap = sys_cap[si];
     int kp = (i > 0 && (!sys || sys[i - 1] == si)) ? octal_lcp(codes[i - 1], c) : -1;
     int kn = (i + 1 < n && (!sys || sys[i + 1] == si)) ? octal_lcp(c, codes[i + 1]) : -1;
     int K = kp > kn ? kp : kn;
@@ -76,7 +79,8 @@ __global__ __launch_bounds__(kBlock) void tree_count_kernel(const uint32_t* __re
                                                             const uint32_t* __restrict__ sys,
                                                             size_t n, uint32_t* __restrict__ cnt,
                                                             uint32_t nb, uint32_t* flags,
-                                                            int leaf_cap) {
+                                                            int leaf_cap,
+                                                            const uint8_t* __restrict__ sys_cap) {
     __shared__ uint32_t ws[kWaves][kDepths];
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < n;
@@ -84,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void tree_count_kernel(const uint32_t* __re
     if (valid) {
         uint32_t c = codes[i];
         if (c >> 30) atomicOr(flags, 1u);
-        pd = part_depths(codes, sys, n, i, c, leaf_cap);
+        pd = part_depths(codes, sys, n, i, c, leaf_cap, sys_cap);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -112,7 +116,8 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
     const uint32_t* __restrict__ off, uint32_t nb, float4 root,
     const float4* __restrict__ roots, float4* __restrict__ geom, uint2* __restrict__ link,
     uint32_t* __restrict__ nend, uint32_t* __restrict__ parent, uint32_t* __restrict__ leaf_of,
-    uint8_t* __restrict__ octant, int leaf_cap, uint32_t cap) {
+    uint8_t* __restrict__ octant, int leaf_cap, uint32_t cap,
+    const uint8_t* __restrict__ sys_cap) {
 #pragma clang fp contract(off)
     __shared__ uint32_t ws[kWaves][kDepths];
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -121,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
     PartDepths pd = {-1, kDepths, kDepths};
     if (valid) {
         c = codes[i];
-        pd = part_depths(codes, sys, n, i, c, leaf_cap);
+        pd = part_depths(codes, sys, n, i, c, leaf_cap, sys_cap);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t le = ps_lanemask_lt() | (1ull << lane);
@@ -418,13 +423,13 @@ size_t ps_tree_tmp_bytes(size_t n) {
 
 hipError_t ps_tree_count(const uint32_t* codes, const uint32_t* sys, size_t n,
                          uint32_t* level_start, uint32_t* flags, unsigned long long* total64,
-                         void* tmp, int leaf_cap, hipStream_t s) {
+                         void* tmp, int leaf_cap, hipStream_t s, const uint8_t* sys_cap) {
     const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
     uint32_t* cnt = reinterpret_cast<uint32_t*>(tmp);
     const size_t t = (size_t)kDepths * nb;
     void* scan_tmp = reinterpret_cast<char*>(tmp) + align256(t * sizeof(uint32_t));
     ps_fill_u32(flags, 0u, 1, s);
-    tree_count_kernel<<<nb, kBlock, 0, s>>>(codes, sys, n, cnt, nb, flags, leaf_cap);
+    tree_count_kernel<<<nb, kBlock, 0, s>>>(codes, sys, n, cnt, nb, flags, leaf_cap, sys_cap);
     hipError_t e = ps_scan_u32(cnt, cnt, t, total64, scan_tmp, s);
     if (e != hipSuccess) return e;
     tree_levels_kernel<<<1, 64, 0, s>>>(cnt, nb, total64, level_start);
@@ -435,7 +440,7 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
                          const float4* roots, void* tmp, float4* geom, uint2* link,
                          uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, uint8_t* octant,
                          size_t n_nodes, const unsigned long long* n_nodes_dev, int leaf_cap,
-                         hipStream_t s) {
+                         hipStream_t s, const uint8_t* sys_cap) {
     const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
     const uint32_t* off = reinterpret_cast<const uint32_t*>(tmp);
     if (n_nodes_dev)
@@ -444,7 +449,8 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
         ps_fill_u32(reinterpret_cast<uint32_t*>(link), 0u, 2 * n_nodes, s);
     const uint32_t cap = n_nodes < 0xFFFFFFFFull ? (uint32_t)n_nodes : 0xFFFFFFFFu;
     tree_write_kernel<<<nb, kBlock, 0, s>>>(codes, sys, n, off, nb, root, roots, geom, link,
-                                            nend, parent, leaf_of, octant, leaf_cap, cap);
+                                            nend, parent, leaf_of, octant, leaf_cap, cap,
+                                            sys_cap);
     if (n_nodes) {
         size_t mb = (n_nodes + kBlock - 1) / kBlock;
         if (mb > 4096) mb = 4096;

@@ -187,3 +187,34 @@ def test_batch_arena_overflow_equals_single_ticks(mode):
         o = single(sys_, mode, api.TRAV_MT)
         for f in FIELDS:
             assert np.array_equal(g[f], o[f]), (k, f)
+
+
+@pytest.mark.parametrize("mode", [api.MORTON_TRUNC, api.MORTON_AVX2])
+def test_batch_per_system_leaf_caps_equal_single_ticks(mode):
+    # ps_hip_batch_set_leaf_caps: system k builds the tree of a single tick
+    # with PS_HIP_OPT_LEAF_CAP = caps[k], and everything after it follows
+    systems = ragged_systems(seed=67)
+    caps = [1, 2, 4, 8, 16, 3, 64, 1, 5, 12]
+    n = sum(len(s[0]) for s in systems)
+    with api.Batch(n, len(systems), theta=2.0, graph=True) as b:
+        b.set_option(api.OPT_TRAV_LANES, LANES)
+        b.set_leaf_caps(caps)
+        b.load(systems)
+        b.tick(mode)
+        res = b.result()
+        b.restore()
+        b.tick(mode)  # the graph replay keeps the capacities
+        assert b.stats()["used_graph"] == 1
+        rep = b.result()
+        # fewer capacities than systems is refused
+        b.set_leaf_caps(caps[:3])
+        with pytest.raises(api.PsError):
+            b.tick(mode)
+    for f in FIELDS:
+        assert np.array_equal(rep[f], res[f]), f
+    for k, (x, y, z, m) in enumerate(systems):
+        o = gpu_tick(x, y, z, m, theta=2.0, mode=mode, tree=False, leaf_cap=caps[k],
+                     lanes=LANES)
+        g = b.system(res, k)
+        for f in FIELDS:
+            assert np.array_equal(g[f], o[f]), (k, caps[k], f)

@@ -224,85 +224,129 @@ def _timed_steps(s, n, v, steps, warmup, mmode):
 
 
 def bench_mlp(a):
-    """The learned far-field correction at N = 4,194,304 Plummer (a = 1):
-    trained on --mlp-ticks ticks of residuals at the coarse setting (theta
-    --mlp-theta, leaf capacity --mlp-cap; a smaller theta is coarser here:
-    far <=> d^2 > theta^2 (hw_t + hw_s)^2), validated on another realisation
-    (the plain FMM judges: the correction is used only if it lowers the
-    error), then compared on a third realisation with the plain FMM at the
-    reference setting (theta --theta, leaf capacity 1): FP64 RMS error on
-    4096 targets and throughput per step (tick + id-gather + Euler).
-    value = corrected throughput."""
+    """The learned far-field correction at N = 4,194,304 Plummer (a = 1),
+    judged against the best the plain FMM can do at the same accuracy.
+
+    1. Plain settings on the evaluation realisation (seed 3): theta in
+       --mlp-thetas, leaf capacity in --mlp-caps; FP64 RMS error on 4096
+       targets and throughput per step (tick + id-gather + Euler).  The
+       reference setting is theta --theta, leaf capacity 1; the baseline to
+       beat is the fastest plain setting whose error does not exceed the
+       reference setting's by more than --mlp-err-tol (1 %).
+    2. The --mlp-candidates fastest plain settings that are faster than that
+       baseline but less accurate get a correction each: trained on
+       --mlp-ticks ticks of residuals of a training realisation (seed 1),
+       validated on another (seed 2; the plain FMM judges, the correction is
+       used only if it lowers the error), then measured on seed 3 like the
+       plain settings.
+    value = throughput of the fastest setting, plain or corrected, whose
+    error does not exceed the reference setting's;
+    speedup_vs_best_plain_at_equal_error says what the correction itself
+    buys (1.0 when no corrected setting wins)."""
     from polesitter_amd import api, mlp
     from polesitter_amd import workloads as W
     n = 1 << 22
     mmode = api.MORTON_AVX2 if a.morton == "avx2" else api.MORTON_TRUNC
-    th_c = a.mlp_theta if a.mlp_theta else a.theta
-    log(f"bench: {api.device_name()} workload mlp4m: Plummer N={n}, coarse theta {th_c} "
-        f"leaf capacity {a.mlp_cap}")
-    zero = [np.zeros(n, np.float32)] * 3
-    x, y, z, m = W.plummer(n, seed=1)
-    val = W.plummer(n, seed=2) + (zero,)
-    t0 = time.time()
-    res = mlp.build_correction(x, y, z, m, zero, train_ticks=a.mlp_ticks, val_ticks=10,
-                               samples=4096, theta=th_c, leaf_cap=a.mlp_cap, steps=3000,
-                               batch=8192, lr=2e-3, seed=1, val_system=val, log=log)
-    train_s = time.time() - t0
     L = api.hip_lib()
+    zero = [np.zeros(n, np.float32)] * 3
     ex, ey, ez, em = W.plummer(n, seed=3)
-    out = {}
-    for name, th, cap, use in (("plain_reference_setting", a.theta, 1, False),
-                               ("corrected_coarse", th_c, a.mlp_cap, res["enabled"]),
-                               ("plain_coarse", th_c, a.mlp_cap, False),
-                               ("plain_reference_theta_coarse_leaf", a.theta, a.mlp_cap, False)):
+    log(f"bench: {api.device_name()} workload mlp4m: Plummer N={n}")
+
+    def measure(th, cap, params=None):
         with api.Solver(n, theta=th) as s:
             api.check("opt", api.ps_set_option(s.ctx, api.OPT_LEAF_CAP, cap))
-            if use:
-                api.check("set_mlp", L.ps_hip_engine_set_mlp(s.ctx.engine,
-                                                             res["params"].ctypes.data))
+            if params is not None:
+                api.check("set_mlp", L.ps_hip_engine_set_mlp(s.ctx.engine, params.ctypes.data))
             s.load(ex, ey, ez, em)
             s.reset_ids()
             s.tick(mmode)
             r = s.result()
             f = np.stack([r["fx"], r["fy"], r["fz"]], 1)
             err = fp64_error(n, f, r["id"], s.direct_f64)
+            del r, f
             s.load(ex, ey, ez, em)
             dt = _timed_steps(s, n, zero, a.steps, a.warmup, mmode)
-            # stage times of one more tick (events only)
             api.check("opt", api.ps_set_option(s.ctx, api.OPT_STATS, 2))
             s.reset_ids()
             s.tick(mmode)
             st = s.stats()
-        out[name] = dict(theta=th, leaf_cap=cap, correction=bool(use), fp64_rms_err=err[0],
-                         fp64_max_err=err[1], ms_per_step=round(1e3 * dt, 4),
-                         particles_per_s=round(n / dt, 1),
-                         stages_ms={k: round(st["ms_" + k], 4) for k in
-                                    ("sort", "build", "upward", "traverse", "l2l", "p2p", "mlp",
-                                     "total")})
-        log(f"bench: {name}: {out[name]}")
-    c, p = out["corrected_coarse"], out["plain_reference_setting"]
-    q = out["plain_reference_theta_coarse_leaf"]
-    line = dict(metric=METRIC, value=c["particles_per_s"], unit="particles/s", n_gpus=1,
-                steps=a.steps, warmup=a.warmup, ms_per_step=c["ms_per_step"],
+        return dict(theta=th, leaf_cap=cap, correction=params is not None, fp64_rms_err=err[0],
+                    fp64_max_err=err[1], ms_per_step=round(1e3 * dt, 4),
+                    particles_per_s=round(n / dt, 1),
+                    stages_ms={k: round(st["ms_" + k], 4) for k in
+                               ("sort", "build", "upward", "traverse", "l2l", "p2p", "mlp",
+                                "total")})
+
+    thetas = [float(t) for t in a.mlp_thetas.split(",")]
+    caps = [int(c) for c in a.mlp_caps.split(",")]
+    plain = []
+    for th in thetas:
+        for cap in caps:
+            r = measure(th, cap)
+            plain.append(r)
+            log(f"bench: plain theta={th} cap={cap}: rms {r['fp64_rms_err']:.5f} "
+                f"{r['ms_per_step']:.3f} ms")
+    ref = next((r for r in plain if r["theta"] == a.theta and r["leaf_cap"] == 1), None)
+    if ref is None:
+        ref = measure(a.theta, 1)
+        plain.append(ref)
+    # "equal error": within --mlp-err-tol of the reference setting's (leaf
+    # capacity alone moves it by 0.01 %; a strict comparison would crown
+    # the reference setting itself as the plain baseline)
+    bar = ref["fp64_rms_err"] * (1.0 + a.mlp_err_tol)
+    ok_plain = [r for r in plain if r["fp64_rms_err"] <= bar]
+    best_plain = max(ok_plain, key=lambda r: r["particles_per_s"])
+    # candidates for a correction: faster than the best accurate plain
+    # setting, too inaccurate on their own
+    cands = sorted((r for r in plain if r["fp64_rms_err"] > bar and
+                    r["particles_per_s"] > best_plain["particles_per_s"]),
+                   key=lambda r: -r["particles_per_s"])[:a.mlp_candidates]
+    corrected = []
+    x, y, z, m = W.plummer(n, seed=1)
+    val = W.plummer(n, seed=2) + (zero,)
+    for c in cands:
+        t0 = time.time()
+        res = mlp.build_correction(x, y, z, m, zero, train_ticks=a.mlp_ticks, val_ticks=10,
+                                   samples=4096, theta=c["theta"], leaf_cap=c["leaf_cap"],
+                                   steps=3000, batch=8192, lr=2e-3, seed=1, val_system=val,
+                                   log=log)
+        train_s = time.time() - t0
+        if not res["enabled"]:
+            corrected.append(dict(c, correction=False, validation=dict(
+                plain=res["val_plain"], corrected=res["val_corrected"], enabled=False)))
+            continue
+        r = measure(c["theta"], c["leaf_cap"], res["params"])
+        r["validation"] = dict(plain=res["val_plain"], corrected=res["val_corrected"],
+                               enabled=True, train_samples=res["train_samples"],
+                               train_seconds=round(train_s, 1))
+        r["plain_fp64_rms_err"] = c["fp64_rms_err"]
+        corrected.append(r)
+        log(f"bench: corrected theta={c['theta']} cap={c['leaf_cap']}: rms "
+            f"{r['fp64_rms_err']:.5f} (plain {c['fp64_rms_err']:.5f}) {r['ms_per_step']:.3f} ms")
+    ok_corr = [r for r in corrected if r.get("correction") and r["fp64_rms_err"] <= bar]
+    best_corr = max(ok_corr, key=lambda r: r["particles_per_s"]) if ok_corr else None
+    head = best_corr if (best_corr and best_corr["particles_per_s"] >
+                         best_plain["particles_per_s"]) else best_plain
+    line = dict(metric=METRIC, value=head["particles_per_s"], unit="particles/s", n_gpus=1,
+                steps=a.steps, warmup=a.warmup, ms_per_step=head["ms_per_step"],
                 higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
                 data="synthetic",
                 config=dict(workload="mlp4m", n_particles=n, theta=a.theta, morton=a.morton,
-                            coarse_theta=th_c, coarse_leaf_cap=a.mlp_cap,
-                            train_ticks=a.mlp_ticks,
+                            thetas=thetas, caps=caps, train_ticks=a.mlp_ticks,
+                            equal_error_tolerance=a.mlp_err_tol,
                             step="tick + id-gather + semi-implicit Euler dt=1e-3",
                             net="24-64-64-64-3 GELU (tanh form), FP32 weights; in-tick "
                                 "on MFMA with bf16 hi/lo operand pairs, FP32 accumulation"),
-                results=out, speedup_vs_plain_reference_setting=round(
-                    c["particles_per_s"] / p["particles_per_s"], 3),
-                error_ratio_vs_plain_reference_setting=round(
-                    c["fp64_rms_err"] / p["fp64_rms_err"], 4),
-                speedup_vs_plain_reference_theta_coarse_leaf=round(
-                    c["particles_per_s"] / q["particles_per_s"], 3),
-                error_ratio_vs_plain_reference_theta_coarse_leaf=round(
-                    c["fp64_rms_err"] / q["fp64_rms_err"], 4),
-                validation=dict(plain=res["val_plain"], corrected=res["val_corrected"],
-                                enabled=res["enabled"], train_samples=res["train_samples"],
-                                train_seconds=round(train_s, 1)))
+                headline=dict(theta=head["theta"], leaf_cap=head["leaf_cap"],
+                              correction=head["correction"]),
+                reference_setting=ref, best_plain_at_equal_error=best_plain,
+                best_corrected_at_equal_error=best_corr,
+                speedup_vs_best_plain_at_equal_error=round(
+                    (best_corr["particles_per_s"] if best_corr else 0.0) /
+                    best_plain["particles_per_s"], 4) if best_corr else 1.0,
+                speedup_vs_reference_setting=round(head["particles_per_s"] /
+                                                   ref["particles_per_s"], 4),
+                plain=plain, corrected=corrected)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -720,11 +764,15 @@ def main(argv=None):
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-16m", action="store_true",
                     help="plummer1m: skip the 16M two-Plummer record (the metric's second size)")
-    ap.add_argument("--mlp-cap", type=int, default=16,
-                    help="mlp4m: leaf capacity of the corrected coarse setting")
-    ap.add_argument("--mlp-theta", type=float, default=1.9,
-                    help="mlp4m: theta of the corrected coarse setting (0: --theta)")
+    ap.add_argument("--mlp-caps", default="1,8,16,32,64",
+                    help="mlp4m: leaf capacities of the plain settings measured")
+    ap.add_argument("--mlp-thetas", default="2.0,1.9,1.8",
+                    help="mlp4m: thetas of the plain settings measured (smaller is coarser)")
+    ap.add_argument("--mlp-candidates", type=int, default=3,
+                    help="mlp4m: coarse settings that get a trained correction")
     ap.add_argument("--mlp-ticks", type=int, default=200, help="mlp4m: training ticks")
+    ap.add_argument("--mlp-err-tol", type=float, default=0.01,
+                    help="mlp4m: relative FP64-error tolerance of 'equal error'")
     ap.add_argument("--no-tune", action="store_true",
                     help="ensemble512: skip the autotuner (reference setting)")
     ap.add_argument("--host-ticks", type=int, default=3,

@@ -60,6 +60,10 @@ extern "C" {
 #define PS_HIP_OPT_TRAV_SEGMENTS 10 /* traversal work pieces per workgroup and iteration,
                                        1 to 64 (2): more balance the load, each costs a few
                                        block-wide synchronisations */
+#define PS_HIP_OPT_SORT_INCREMENTAL 11 /* 1 (default): when at most 1/8 of the keys differ
+                                          from the last tick's sorted keys in place, sort
+                                          those and merge (the same permutation); 0: always
+                                          the full radix sort */
 
 typedef struct ps_hip_engine ps_hip_engine_t;
 
@@ -81,6 +85,9 @@ typedef struct {
         ms_download, ms_total;
     float ms_trav_steps;
     float ms_mlp;             /* the learned correction, when one is set */
+    uint64_t sort_changed;    /* keys that differed from the last tick's (n: not compared) */
+    uint32_t sort_incremental; /* 1: the last tick sorted incrementally */
+    uint32_t reserved_;
 } ps_hip_stats_t;
 
 typedef struct ps_hip_batch ps_hip_batch_t;

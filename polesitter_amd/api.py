@@ -34,6 +34,7 @@ OPT_TRAVERSAL, OPT_KAHAN, OPT_THETA, OPT_STATS, OPT_TRAV_LANES, OPT_P2P_BIG, OPT
 OPT_ARENA_WORDS = 8
 OPT_TRAV_BLOCKS = 9
 OPT_TRAV_SEGMENTS = 10
+OPT_SORT_INCREMENTAL = 11
 BATCH_OPT_GRAPH = 100
 
 
@@ -77,7 +78,9 @@ class PsStats(C.Structure):
                 ("ms_upload", C.c_float), ("ms_sort", C.c_float), ("ms_build", C.c_float),
                 ("ms_upward", C.c_float), ("ms_traverse", C.c_float), ("ms_l2l", C.c_float),
                 ("ms_p2p", C.c_float), ("ms_download", C.c_float), ("ms_total", C.c_float),
-                ("ms_trav_steps", C.c_float), ("ms_mlp", C.c_float)]
+                ("ms_trav_steps", C.c_float), ("ms_mlp", C.c_float),
+                ("sort_changed", C.c_uint64), ("sort_incremental", C.c_uint32),
+                ("reserved_", C.c_uint32)]
 
     def as_dict(self):
         d = {}

@@ -172,6 +172,17 @@ struct ps_hip_engine {
     DevBuf sort_t;
     DevBuf sort_tmp, tree_tmp;
     DevBuf pos4_b, leaf_of_b;
+    // the keys the last sort produced, in place (two, alternating), for the
+    // incremental sort of the next tick
+    DevBuf prev_b[2];
+    int prev_cur = 0;        // prev_b[prev_cur] holds them
+    size_t prev_n = 0;       // their count (0: none)
+    int incremental = 1;     // PS_HIP_OPT_SORT_INCREMENTAL
+    // the incremental sort's copy back into the caller's arrays runs on s2;
+    // skey is where the sorted keys are meanwhile (the tree reads them)
+    bool copy_pending = false;
+    hipEvent_t ev_sorted = nullptr, ev_copied = nullptr;
+    uint32_t* skey = nullptr;
     // nodes
     DevBuf geom_b, mp_b, loc_b, link_b, nend_b, parent_b, p2p_off_b, p2p_len_b, p2p_moff_b;
     DevBuf ltot_b;  // summed locals of the leaves (the tick's L2L)
@@ -209,6 +220,15 @@ struct ps_hip_engine {
     hipEvent_t ev_it[4 * 16] = {nullptr};  // per traversal iteration: count and write
 
     uint32_t* small_u32() { return reinterpret_cast<uint32_t*>(small_b.p); }
+    // changed keys of the incremental sort (device), and its host mirror
+    unsigned long long* delta_count() {
+        return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(small_b.p) + 160);
+    }
+    unsigned long long host_delta_count() const {
+        return *reinterpret_cast<const unsigned long long*>(
+            reinterpret_cast<const char*>(h_small) + 160);
+    }
+    uint32_t* prev_keys(int k) { return reinterpret_cast<uint32_t*>(prev_b[k].p); }
     unsigned long long* totals() {
         return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(small_b.p) + 64);
     }
@@ -264,6 +284,7 @@ struct ps_hip_engine {
 
     void free_all() {
         for (auto& b : stage) release(b);
+        for (auto& b : prev_b) release(b);
         DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
                          &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,  &ltot_b, &oct_b,
@@ -281,6 +302,9 @@ struct ps_hip_engine {
         ev_counts = nullptr;
         if (ev_async) (void)hipEventDestroy(ev_async);
         ev_async = nullptr;
+        if (ev_sorted) (void)hipEventDestroy(ev_sorted);
+        if (ev_copied) (void)hipEventDestroy(ev_copied);
+        ev_sorted = ev_copied = nullptr;
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : ev_it)
@@ -313,6 +337,8 @@ void for_each_tick_buffer(ps_hip_engine* e, size_t P, size_t M, uint32_t arena_w
     f(e->tree_tmp, ps_tree_tmp_bytes(P));
     f(e->pos4_b, 16 * P);
     f(e->leaf_of_b, 4 * P);
+    f(e->prev_b[0], 4 * P);
+    f(e->prev_b[1], 4 * P);
     DevBuf* node16[] = {&e->geom_b, &e->mp_b, &e->loc_b, &e->ltot_b};
     for (DevBuf* d : node16) f(*d, 16 * M);
     f(e->oct_b, M);
@@ -375,6 +401,8 @@ hipError_t alloc_fixed(ps_hip_engine* e) {
     PS_AL(tree_tmp, ps_tree_tmp_bytes(P));
     PS_AL(pos4_b, 16 * P);
     PS_AL(leaf_of_b, 4 * P);
+    PS_AL(prev_b[0], 4 * P);
+    PS_AL(prev_b[1], 4 * P);
     PS_AL(geom_b, 16 * M);
     PS_AL(mp_b, 16 * M);
     PS_AL(loc_b, 16 * M);
@@ -414,10 +442,58 @@ int bind_array(ps_hip_engine* e, int k, T* p, size_t n, bool upload, T** out) {
     return PS_HIP_OK;
 }
 
+constexpr size_t kIncrementalMin = 4096;  // smaller ticks sort in full (cheaper than a readback)
+
+// The tick's sort.  After a tick of the same size the last sort's keys are
+// kept in place: when few keys changed (the caller's frame loop moves the
+// particles a little between ticks), only those are sorted and merged with
+// the rest (ps_sort_delta_*: the same permutation as the full radix sort);
+// otherwise, or for the first tick, the full radix sort.  One small
+// readback (the changed count) decides.  Either way the sorted keys are
+// kept for the next tick.
 int stage_sort(ps_hip_engine* e) {
     PsSortArrays io = {e->code, e->id, e->x, e->y, e->z, e->m};
+    const int cur = e->prev_cur, nxt = cur ^ 1;
+    e->st.sort_changed = e->n;
+    e->st.sort_incremental = 0;
+    if (e->incremental && e->prev_n == e->n && e->n >= kIncrementalMin) {
+        PsSortArrays t1 = e->sort_set(1);
+        uint32_t* ckey = reinterpret_cast<uint32_t*>(t1.x);
+        uint32_t* cpos = reinterpret_cast<uint32_t*>(t1.y);
+        PS_CK(ps_sort_delta_count(e->code, e->prev_keys(cur), e->n, e->sort_tmp.p, ckey, cpos,
+                                  e->delta_count(), e->s));
+        PS_CK(hipMemcpyAsync(reinterpret_cast<char*>(e->h_small) + 160, e->delta_count(), 8,
+                             hipMemcpyDeviceToHost, e->s));
+        PS_CK(hipStreamSynchronize(e->s));
+        const unsigned long long nc = e->host_delta_count();
+        e->st.sort_changed = nc;
+        if (nc * 8 <= e->n) {
+            PsSortArrays t0 = e->sort_set(0);
+            PS_CK(ps_sort_delta_finish(io, t0, t1, e->pos4(), e->n, e->sort_tmp.p,
+                                       e->prev_keys(cur), ckey, cpos, (uint32_t)nc,
+                                       e->prev_keys(nxt), e->s, e->s2, e->ev_sorted,
+                                       e->ev_copied));
+            e->copy_pending = true;
+            e->skey = t0.key;
+            e->prev_cur = nxt;
+            e->st.sort_incremental = 1;
+            return PS_HIP_OK;
+        }
+    }
     PS_CK(ps_radix_sort(io, e->sort_set(0), e->sort_set(1), e->pos4(), e->n, e->sort_tmp.p,
-                        e->s));
+                        e->s, e->prev_keys(nxt)));
+    e->skey = e->code;
+    e->prev_cur = nxt;
+    e->prev_n = e->n;
+    return PS_HIP_OK;
+}
+
+// Work after this on the engine stream sees the caller's arrays sorted.
+int finish_copy(ps_hip_engine* e) {
+    if (!e->copy_pending) return PS_HIP_OK;
+    e->copy_pending = false;
+    e->skey = e->code;
+    PS_CK(hipStreamWaitEvent(e->s, e->ev_copied, 0));
     return PS_HIP_OK;
 }
 
@@ -428,11 +504,12 @@ int stage_sort(ps_hip_engine* e) {
 // after the stream has drained in the caller).
 int stage_build(ps_hip_engine* e) {
     uint32_t* small = e->small_u32();
-    PS_CK(ps_tree_count(e->code, nullptr, e->n, small, small + 12, e->totals(), e->tree_tmp.p,
+    const uint32_t* keys = e->skey ? e->skey : e->code;  // the sorted keys
+    PS_CK(ps_tree_count(keys, nullptr, e->n, small, small + 12, e->totals(), e->tree_tmp.p,
                         e->leaf_cap, e->s));
     PS_CK(hipMemcpyAsync(e->h_small, e->small_b.p, 96, hipMemcpyDeviceToHost, e->s));
     PS_CK(hipEventRecord(e->ev_counts, e->s));
-    PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
+    PS_CK(ps_tree_write(keys, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
                         e->link(), e->nend(), e->parent(), e->leaf_of(), e->oct(),
                         e->node_cap, e->totals(), e->leaf_cap, e->s));
     ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
@@ -636,6 +713,8 @@ int run_from_traverse(ps_hip_engine* e) {
                             e->fz, e->s);
         PS_CK(hipGetLastError());
     }
+    // the caller's arrays are sorted before the tick's last event
+    PS_RC(finish_copy(e));
     e->record(8);
     return PS_HIP_OK;
 }
@@ -646,10 +725,12 @@ int run_tick(ps_hip_engine* e) {
     e->record(1);
     PS_RC(stage_sort(e));
     e->record(2);
-    PS_RC(stage_build(e));
+    int rc = stage_build(e);
     e->record(3);
-    PS_RC(stage_upward(e));
-    return run_from_traverse(e);
+    if (rc == PS_HIP_OK) rc = stage_upward(e);
+    if (rc == PS_HIP_OK) rc = run_from_traverse(e);
+    const int rj = finish_copy(e);  // joined already unless a stage failed
+    return rc != PS_HIP_OK ? rc : rj;
 }
 
 // ---------------------------------------------------------------------------
@@ -688,7 +769,13 @@ int async_front(ps_hip_engine* e, hipStream_t cs) {
     const size_t M = e->node_alloc;
     uint32_t* ls = e->small_u32();
     PsSortArrays io = {e->code, e->id, e->x, e->y, e->z, e->m};
-    PS_CK(ps_radix_sort(io, e->sort_set(0), e->sort_set(1), e->pos4(), e->n, e->sort_tmp.p, cs));
+    // the full radix sort (the incremental one would need a readback); its
+    // keys are kept for a later synchronous tick
+    const int nxt = e->prev_cur ^ 1;
+    PS_CK(ps_radix_sort(io, e->sort_set(0), e->sort_set(1), e->pos4(), e->n, e->sort_tmp.p, cs,
+                        e->prev_keys(nxt)));
+    e->prev_cur = nxt;
+    e->prev_n = e->n;
     PS_CK(ps_tree_count(e->code, nullptr, e->n, ls, ls + 12, e->totals(), e->tree_tmp.p,
                         e->leaf_cap, cs));
     PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
@@ -1012,6 +1099,8 @@ int ps_hip_engine_create_in(ps_hip_engine_t** out, size_t max_particles, size_t 
     if (r == hipSuccess) r = hipEventCreateWithFlags(&e->join, hipEventDisableTiming);
     if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_counts, hipEventDisableTiming);
     if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_async, hipEventDisableTiming);
+    if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_sorted, hipEventDisableTiming);
+    if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_copied, hipEventDisableTiming);
     for (int k = 0; k < 12 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev[k]);
     for (int k = 0; k < 64 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev_it[k]);
     int rc = PS_HIP_OK;
@@ -1105,6 +1194,10 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
             if (!(value >= 1.0 && value <= 64.0) || value != (double)(int)value)
                 return PS_HIP_EINVAL;
             e->trav_spb = (uint32_t)value;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_SORT_INCREMENTAL:
+            if (value != 0.0 && value != 1.0) return PS_HIP_EINVAL;
+            e->incremental = value != 0.0;
             return PS_HIP_OK;
         case PS_HIP_OPT_ARENA_WORDS:
             if (!(value >= 1.0 && value <= 65536.0) || value != (double)(int)value)
@@ -1666,7 +1759,9 @@ int ps_hip_debug_stage(ps_hip_engine_t* e, int stage) {
             break;
         default: return PS_HIP_EINVAL;
     }
+    const int rj = finish_copy(e);
     if (rc != PS_HIP_OK) return rc;
+    if (rj != PS_HIP_OK) return rj;
     PS_CK(hipStreamSynchronize(e->s));
     return PS_HIP_OK;
 }

@@ -77,8 +77,32 @@ size_t ps_sort_tmp_bytes(size_t n);
 // payload; t0/t1 are two scratch sets of n elements.  pos4 (optional) gets the
 // sorted (x, y, z, m).  With aux set (in all three sets) a fourth pass sorts
 // stably by aux: the order is (aux, key).
+// kcopy (optional): a second copy of the sorted keys.  Payload arrays x
+// (and with it y, z, m) may be null in all three sets: keys and id only.
 hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
-                         size_t n, void* tmp, hipStream_t s);
+                         size_t n, void* tmp, hipStream_t s, uint32_t* kcopy = nullptr);
+// The incremental sort.  prev: the keys the last sort produced, in place
+// (sorted).  Step 1 (count) flags the positions whose key (low 30 bits)
+// differs from prev, writes their keys and positions in order to ckey and
+// cpos and their number to *count (device).  Step 2 (finish, nc = that
+// number) sorts them by key, merges them with the unchanged ones by (key,
+// position) -- the stable order, the same permutation as ps_radix_sort --
+// into t0 and copies the result back to io; pos4 and kcopy as in
+// ps_radix_sort.  With s_copy and the two events set, the copy back runs on
+// s_copy after ev_sorted (recorded on s) and records ev_copied: work after
+// it on s may read t0 (the merged keys, pos4) but not io until it waits for
+// ev_copied.  tmp: ps_sort_tmp_bytes(n), untouched between the steps;
+// ckey and cpos must not be any of the t0, t1 key or id arrays, nor t1.z
+// and t1.m (the packed changed keys, 8 bytes each, from t1.z on).
+size_t ps_sort_delta_tmp_bytes(size_t n);  // the flag words' share of ps_sort_tmp_bytes
+hipError_t ps_sort_delta_count(const uint32_t* key, const uint32_t* prev, size_t n, void* tmp,
+                               uint32_t* ckey, uint32_t* cpos, unsigned long long* count,
+                               hipStream_t s);
+hipError_t ps_sort_delta_finish(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
+                                size_t n, void* tmp, const uint32_t* prev, uint32_t* ckey,
+                                uint32_t* cpos, uint32_t nc, uint32_t* kcopy, hipStream_t s,
+                                hipStream_t s_copy = nullptr, hipEvent_t ev_sorted = nullptr,
+                                hipEvent_t ev_copied = nullptr);
 
 // ---- ps_tree.hip: octree build, upward and L2L passes ------------------------
 size_t ps_tree_tmp_bytes(size_t n);

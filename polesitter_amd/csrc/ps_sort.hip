@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kDBlock) void sort_downsweep_kernel(
     float* __restrict__ xout, float* __restrict__ yout, float* __restrict__ zout,
     float* __restrict__ mout, float4* __restrict__ pos4, const uint32_t* __restrict__ ain,
     uint32_t* __restrict__ aout, size_t n, int shift, const uint32_t* __restrict__ offs,
-    const uint32_t* __restrict__ dtot, uint32_t n_tiles) {
+    const uint32_t* __restrict__ dtot, uint32_t n_tiles, uint32_t* __restrict__ kcopy) {
     __shared__ uint32_t keys[kTile];
     __shared__ uint16_t sidx[kTile];
     __shared__ uint32_t tstart[kRadix];
@@ -237,30 +237,329 @@ __global__ __launch_bounds__(kDBlock) void sort_downsweep_kernel(
         uint32_t d = (k >> shift) & (kRadix - 1);
         size_t g = (size_t)goff[d] + (p - tstart[d]);
         size_t src = base + sidx_p;
-        float vx = xin[src], vy = yin[src], vz = zin[src], vm = min_[src];
         kout[g] = k;
         iout[g] = iin[src];
-        xout[g] = vx;
-        yout[g] = vy;
-        zout[g] = vz;
-        mout[g] = vm;
-        if (pos4) pos4[g] = make_float4(vx, vy, vz, vm);
+        if (kcopy) kcopy[g] = k;
+        if (xin) {  // null: keys and index only
+            float vx = xin[src], vy = yin[src], vz = zin[src], vm = min_[src];
+            xout[g] = vx;
+            yout[g] = vy;
+            zout[g] = vz;
+            mout[g] = vm;
+            if (pos4) pos4[g] = make_float4(vx, vy, vz, vm);
+        }
         if (aout) aout[g] = ain[src];
     }
 }
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// ---- incremental sort: the previous tick's sorted order, a few keys changed ----
+//
+// prev = the keys the last sort produced, position by position (sorted).  A
+// particle whose key (low 30 bits, the bits the radix passes sort) still
+// equals prev at its position is unchanged; the unchanged ones, in position
+// order, are sorted by (key, position) already.  The changed ones are
+// compacted and sorted stably by key, i.e. by (key, position).  Merging the
+// two by (key, position) gives the stable order of all -- the permutation
+// the full radix sort gives, for any input, as long as prev is sorted.
+// Output position of an unchanged particle i: its rank among the unchanged
+// + the number of changed ones before (key_i, i); of the changed one of
+// rank r: r + the number of unchanged ones before (key, position), found by
+// two binary searches in prev (the unchanged ones are prev's entries minus
+// the changed positions).
+constexpr uint32_t kKeyMask = 0x3FFFFFFFu;
+constexpr int kMBlock = 1024;   // merge: positions per block
+constexpr int kSlice = 4096;    // changed keys of a block's key range held in LDS
+
+__device__ __forceinline__ unsigned long long packed_key(uint32_t key, uint32_t pos) {
+    return ((unsigned long long)(key & kKeyMask) << 32) | pos;
+}
+
+// first index in a[0, n) whose value is >= v (a sorted)
+__device__ __forceinline__ uint32_t lower_bound_u64(const unsigned long long* a, uint32_t n,
+                                                    unsigned long long v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// first index in prev[0, n) whose key is >= k (or > k when upper)
+__device__ __forceinline__ size_t bound_keys(const uint32_t* prev, size_t n, uint32_t k,
+                                             bool upper) {
+    size_t lo = 0, hi = n;
+    while (lo < hi) {
+        const size_t mid = lo + ((hi - lo) >> 1);
+        const uint32_t v = prev[mid] & kKeyMask;
+        if (v < k || (upper && v == k))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Changed flags: one 64-bit word per 64 positions (mask[w]); the positions
+// come in blocks of kMBlock (16 words), and bcnt[b] is block b's count.
+constexpr int kWordsPerBlock = kMBlock / 64;
+constexpr uint32_t kSmallSort = 4096;  // changed keys sorted in one block's LDS
+
+__global__ __launch_bounds__(kMBlock) void delta_mask_kernel(const uint32_t* __restrict__ key,
+                                                             const uint32_t* __restrict__ prev,
+                                                             size_t n,
+                                                             unsigned long long* __restrict__ mask,
+                                                             uint32_t* __restrict__ bcnt) {
+    __shared__ uint32_t wc[kWordsPerBlock];
+    const size_t i = (size_t)blockIdx.x * kMBlock + threadIdx.x;
+    const bool ch = i < n && ((key[i] ^ prev[i]) & kKeyMask) != 0u;
+    const unsigned long long b = __ballot(ch);
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wc[wave] = (uint32_t)__popcll(b);
+        mask[i >> 6] = b;  // words past n read as zero
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kWordsPerBlock; ++w) t += wc[w];
+        bcnt[blockIdx.x] = t;
+    }
+}
+
+// boff[b] = changed positions before block b (b = 0..nb), *count = all of
+// them; one block
+__global__ __launch_bounds__(1024) void delta_scan_kernel(const uint32_t* __restrict__ bcnt,
+                                                          uint32_t nb, uint32_t* __restrict__ boff,
+                                                          unsigned long long* __restrict__ count) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t lo = tid * per < nb ? tid * per : nb;
+    const uint32_t hi = lo + per < nb ? lo + per : nb;
+    uint32_t sum = 0;
+    for (uint32_t t = lo; t < hi; ++t) sum += bcnt[t];
+    uint32_t wt;
+    uint32_t ex = ps_wave_excl_scan(sum, &wt);
+    if (lane == 0) wsum[wave] = wt;
+    __syncthreads();
+    uint32_t all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 16; ++w) {
+        if (w < wave) ex += wsum[w];
+        all += wsum[w];
+    }
+    for (uint32_t t = lo; t < hi; ++t) {
+        boff[t] = ex;
+        ex += bcnt[t];
+    }
+    if (tid == 0) {
+        boff[nb] = all;
+        *count = all;
+    }
+}
+
+// number of changed positions before p (p <= n)
+__device__ __forceinline__ uint32_t changed_before(const unsigned long long* mask,
+                                                   const uint32_t* boff, size_t p) {
+    const size_t b = p / kMBlock;
+    const size_t w0 = b * kWordsPerBlock, w = p >> 6;
+    uint32_t c = boff[b];
+    for (size_t q = w0; q < w; ++q) c += (uint32_t)__popcll(mask[q]);
+    const uint32_t bits = (uint32_t)(p & 63);
+    if (bits) c += (uint32_t)__popcll(mask[w] & ((1ull << bits) - 1ull));
+    return c;
+}
+
+// the block's word prefixes in LDS (thread w < 16 reads word w): wp[w] =
+// changed positions of the block before word w
+__device__ __forceinline__ void block_word_prefix(const unsigned long long* mask, size_t nwords,
+                                                  uint32_t* wp, unsigned long long* wm) {
+    if (threadIdx.x < kWordsPerBlock) {
+        const size_t w = (size_t)blockIdx.x * kWordsPerBlock + threadIdx.x;
+        wm[threadIdx.x] = w < nwords ? mask[w] : 0ull;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0;
+        for (int q = 0; q < kWordsPerBlock; ++q) {
+            wp[q] = c;
+            c += (uint32_t)__popcll(wm[q]);
+        }
+    }
+    __syncthreads();
+}
+
+// the changed keys and their positions, compacted in position order
+__global__ __launch_bounds__(kMBlock) void delta_compact_kernel(
+    const uint32_t* __restrict__ key, size_t n, const unsigned long long* __restrict__ mask,
+    const uint32_t* __restrict__ boff, uint32_t* __restrict__ ckey, uint32_t* __restrict__ cpos) {
+    __shared__ uint32_t wp[kWordsPerBlock];
+    __shared__ unsigned long long wm[kWordsPerBlock];
+    block_word_prefix(mask, (n + 63) / 64, wp, wm);
+    const size_t i = (size_t)blockIdx.x * kMBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned long long m = wm[wl];
+    if (!((m >> lane) & 1ull)) return;
+    const uint32_t c = boff[blockIdx.x] + wp[wl] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    ckey[c] = key[i];
+    cpos[c] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void delta_pack_kernel(const uint32_t* __restrict__ ckey,
+                                                         const uint32_t* __restrict__ cpos,
+                                                         uint32_t nc,
+                                                         unsigned long long* __restrict__ cpk) {
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r < nc) cpk[r] = packed_key(ckey[r], cpos[r]);
+}
+
+// at most kSmallSort changed keys: packed (key, position) values are
+// distinct, so one bitonic sort in LDS gives their stable order
+__global__ __launch_bounds__(1024) void delta_small_sort_kernel(const uint32_t* __restrict__ ckey,
+                                                                const uint32_t* __restrict__ cpos,
+                                                                uint32_t nc,
+                                                                unsigned long long* __restrict__ cpk) {
+    __shared__ unsigned long long a[kSmallSort];
+    uint32_t P = 2;
+    while (P < nc) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += 1024)
+        a[i] = i < nc ? packed_key(ckey[i], cpos[i]) : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += 1024) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long u = a[i], v = a[ixj];
+                    const bool asc = (i & k) == 0;
+                    if ((u > v) == asc) {
+                        a[i] = v;
+                        a[ixj] = u;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < nc; i += 1024) cpk[i] = a[i];
+}
+
+__device__ __forceinline__ void move_particle(const PsSortArrays& in, const PsSortArrays& out,
+                                              float4* pos4, uint32_t* kcopy, size_t src,
+                                              size_t dst) {
+    const uint32_t k = in.key[src];
+    const float vx = in.x[src], vy = in.y[src], vz = in.z[src], vm = in.m[src];
+    out.key[dst] = k;
+    out.id[dst] = in.id[src];
+    out.x[dst] = vx;
+    out.y[dst] = vy;
+    out.z[dst] = vz;
+    out.m[dst] = vm;
+    if (pos4) pos4[dst] = make_float4(vx, vy, vz, vm);
+    if (kcopy) kcopy[dst] = k;
+}
+
+// the unchanged particles: block b takes positions [b kMBlock, (b+1) kMBlock);
+// the changed keys that can fall among them are one slice of the sorted
+// changed keys, [lo, hi), held in LDS when it fits
+__global__ __launch_bounds__(kMBlock) void delta_merge_kernel(
+    PsSortArrays in, PsSortArrays out, float4* __restrict__ pos4, uint32_t* __restrict__ kcopy,
+    const uint32_t* __restrict__ prev, size_t n, const unsigned long long* __restrict__ mask,
+    const uint32_t* __restrict__ boff, const unsigned long long* __restrict__ cpk, uint32_t nc) {
+    __shared__ unsigned long long slice[kSlice];
+    __shared__ uint32_t range[2];
+    __shared__ uint32_t wp[kWordsPerBlock];
+    __shared__ unsigned long long wm[kWordsPerBlock];
+    block_word_prefix(mask, (n + 63) / 64, wp, wm);
+    const size_t b0 = (size_t)blockIdx.x * kMBlock;
+    const size_t b1 = b0 + kMBlock < n ? b0 + kMBlock : n;
+    const uint32_t tid = threadIdx.x;
+    if (tid < 2) {
+        const size_t q = tid == 0 ? b0 : b1 - 1;
+        range[tid] = lower_bound_u64(cpk, nc, packed_key(prev[q], (uint32_t)q));
+    }
+    __syncthreads();
+    const uint32_t lo = range[0], hi = range[1];
+    const bool in_lds = hi - lo <= (uint32_t)kSlice;
+    if (in_lds)
+        for (uint32_t j = tid; j < hi - lo; j += kMBlock) slice[j] = cpk[lo + j];
+    __syncthreads();
+    const size_t i = b0 + tid;
+    if (i >= b1) return;
+    const uint32_t wl = tid >> 6, lane = tid & 63;
+    const unsigned long long m = wm[wl];
+    if ((m >> lane) & 1ull) return;  // changed: delta_scatter_kernel
+    const uint32_t rank_u = (uint32_t)i - (boff[blockIdx.x] + wp[wl] +
+                                           (uint32_t)__popcll(m & ((1ull << lane) - 1ull)));
+    const unsigned long long probe = packed_key(prev[i], (uint32_t)i);
+    const uint32_t before = in_lds ? lo + lower_bound_u64(slice, hi - lo, probe)
+                                   : lo + lower_bound_u64(cpk + lo, hi - lo, probe);
+    move_particle(in, out, pos4, kcopy, i, (size_t)rank_u + before);
+}
+
+// the changed particles, by their rank r in the sorted changed keys
+__global__ __launch_bounds__(256) void delta_scatter_kernel(
+    PsSortArrays in, PsSortArrays out, float4* __restrict__ pos4, uint32_t* __restrict__ kcopy,
+    const uint32_t* __restrict__ prev, size_t n, const unsigned long long* __restrict__ mask,
+    const uint32_t* __restrict__ boff, const unsigned long long* __restrict__ cpk, uint32_t nc) {
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nc) return;
+    const unsigned long long v = cpk[r];
+    const uint32_t k = (uint32_t)(v >> 32), src = (uint32_t)v;
+    // the unchanged entries before (k, src): prev's positions p with
+    // (prev[p], p) < (k, src), minus the changed positions among them
+    const size_t a = bound_keys(prev, n, k, false), b = bound_keys(prev, n, k, true);
+    const size_t p = src < a ? a : (src > b ? b : src);
+    const size_t dst = (size_t)r + (p - changed_before(mask, boff, p));
+    move_particle(in, out, pos4, kcopy, src, dst);
+}
+
+// out = in for the six arrays of n particles (16 bytes per load when every
+// array is 16-byte aligned, vec)
+__global__ __launch_bounds__(256) void copy6_kernel(PsSortArrays in, PsSortArrays out, size_t n,
+                                                    int vec) {
+    const size_t nv = vec ? n >> 2 : 0;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < nv; v += stride) {
+        reinterpret_cast<uint4*>(out.key)[v] = reinterpret_cast<const uint4*>(in.key)[v];
+        reinterpret_cast<uint4*>(out.id)[v] = reinterpret_cast<const uint4*>(in.id)[v];
+        reinterpret_cast<float4*>(out.x)[v] = reinterpret_cast<const float4*>(in.x)[v];
+        reinterpret_cast<float4*>(out.y)[v] = reinterpret_cast<const float4*>(in.y)[v];
+        reinterpret_cast<float4*>(out.z)[v] = reinterpret_cast<const float4*>(in.z)[v];
+        reinterpret_cast<float4*>(out.m)[v] = reinterpret_cast<const float4*>(in.m)[v];
+    }
+    for (size_t i = (nv << 2) + (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        out.key[i] = in.key[i];
+        out.id[i] = in.id[i];
+        out.x[i] = in.x[i];
+        out.y[i] = in.y[i];
+        out.z[i] = in.z[i];
+        out.m[i] = in.m[i];
+    }
+}
+
 }  // namespace
 
 size_t ps_sort_tmp_bytes(size_t n) {
     size_t tiles = (n + kTile - 1) / kTile;
     size_t table = (size_t)kRadix * (tiles ? tiles : 1);
-    return align256(table * sizeof(uint32_t)) + align256(kRadix * sizeof(uint32_t));
+    const size_t delta = ps_sort_delta_tmp_bytes(n);
+    const size_t radix = align256(table * sizeof(uint32_t)) + align256(kRadix * sizeof(uint32_t));
+    // the incremental sort keeps its flag words while a radix sort of the
+    // changed keys uses the tables after them
+    return delta + radix;
 }
 
 hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
-                         size_t n, void* tmp, hipStream_t s) {
+                         size_t n, void* tmp, hipStream_t s, uint32_t* kcopy) {
     if (n == 0) return hipSuccess;
     const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
     const size_t table_n = (size_t)kRadix * tiles;
@@ -286,7 +585,93 @@ hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, floa
         sort_rowscan_kernel<<<kRadix, kRowBlock, 0, s>>>(table, tiles, dtot);
         sort_downsweep_kernel<<<tiles, kDBlock, 0, s>>>(
             kin, a.id, a.x, a.y, a.z, a.m, kout, b.id, b.x, b.y, b.z, b.m,
-            pass == passes - 1 ? pos4 : nullptr, ain, aout, n, shift, table, dtot, tiles);
+            pass == passes - 1 ? pos4 : nullptr, ain, aout, n, shift, table, dtot, tiles,
+            pass == passes - 1 ? kcopy : nullptr);
     }
+    return hipGetLastError();
+}
+
+namespace {
+struct DeltaTmp {
+    unsigned long long* mask;  // one word per 64 positions, padded to whole blocks
+    uint32_t* bcnt;            // per block of kMBlock positions
+    uint32_t* boff;            // exclusive prefix of bcnt, nb + 1 entries
+    void* rest;                // the changed keys' radix tables
+};
+size_t delta_blocks(size_t n) { return (n + kMBlock - 1) / kMBlock; }
+DeltaTmp delta_tmp(void* tmp, size_t n) {
+    const size_t nb = delta_blocks(n);
+    char* p = reinterpret_cast<char*>(tmp);
+    DeltaTmp d;
+    d.mask = reinterpret_cast<unsigned long long*>(p);
+    p += align256(nb * kWordsPerBlock * 8 + 8);
+    d.bcnt = reinterpret_cast<uint32_t*>(p);
+    p += align256(nb * 4 + 4);
+    d.boff = reinterpret_cast<uint32_t*>(p);
+    p += align256(nb * 4 + 8);
+    d.rest = p;
+    return d;
+}
+}  // namespace
+
+size_t ps_sort_delta_tmp_bytes(size_t n) {
+    const size_t nb = delta_blocks(n);
+    return align256(nb * kWordsPerBlock * 8 + 8) + align256(nb * 4 + 4) + align256(nb * 4 + 8);
+}
+
+hipError_t ps_sort_delta_count(const uint32_t* key, const uint32_t* prev, size_t n, void* tmp,
+                               uint32_t* ckey, uint32_t* cpos, unsigned long long* count,
+                               hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    DeltaTmp d = delta_tmp(tmp, n);
+    const uint32_t nb = (uint32_t)delta_blocks(n);
+    delta_mask_kernel<<<nb, kMBlock, 0, s>>>(key, prev, n, d.mask, d.bcnt);
+    delta_scan_kernel<<<1, 1024, 0, s>>>(d.bcnt, nb, d.boff, count);
+    delta_compact_kernel<<<nb, kMBlock, 0, s>>>(key, n, d.mask, d.boff, ckey, cpos);
+    return hipGetLastError();
+}
+
+hipError_t ps_sort_delta_finish(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
+                                size_t n, void* tmp, const uint32_t* prev, uint32_t* ckey,
+                                uint32_t* cpos, uint32_t nc, uint32_t* kcopy, hipStream_t s,
+                                hipStream_t s_copy, hipEvent_t ev_sorted, hipEvent_t ev_copied) {
+    if (n == 0) return hipSuccess;
+    DeltaTmp d = delta_tmp(tmp, n);
+    // the changed (key, position) pairs sorted stably by key: t0 and t1 key
+    // and id arrays are the scratch, the result ends in (ckey, cpos)
+    unsigned long long* cpk = reinterpret_cast<unsigned long long*>(t1.z);  // t1.z and t1.m
+    if (nc > kSmallSort) {
+        PsSortArrays c = {ckey, cpos, nullptr, nullptr, nullptr, nullptr, nullptr};
+        PsSortArrays c0 = {t0.key, t0.id, nullptr, nullptr, nullptr, nullptr, nullptr};
+        PsSortArrays c1 = {t1.key, t1.id, nullptr, nullptr, nullptr, nullptr, nullptr};
+        // the radix tables after the flag words, which the merge still reads
+        hipError_t e = ps_radix_sort(c, c0, c1, nullptr, nc, d.rest, s, nullptr);
+        if (e != hipSuccess) return e;
+        delta_pack_kernel<<<(nc + 255) / 256, 256, 0, s>>>(ckey, cpos, nc, cpk);
+    } else if (nc) {
+        delta_small_sort_kernel<<<1, 1024, 0, s>>>(ckey, cpos, nc, cpk);
+    }
+    // the merged order into t0, then back into the caller's arrays
+    delta_merge_kernel<<<(unsigned)delta_blocks(n), kMBlock, 0, s>>>(io, t0, pos4, kcopy, prev, n,
+                                                                   d.mask, d.boff, cpk, nc);
+    if (nc)
+        delta_scatter_kernel<<<(nc + 255) / 256, 256, 0, s>>>(io, t0, pos4, kcopy, prev, n,
+                                                              d.mask, d.boff, cpk, nc);
+    size_t cb = (n / 4 + 255) / 256 + 1;
+    if (cb > 8192) cb = 8192;
+    if (cb == 0) cb = 1;
+    const void* ptrs[12] = {t0.key, t0.id, t0.x, t0.y, t0.z, t0.m,
+                            io.key, io.id, io.x, io.y, io.z, io.m};
+    int vec = 1;
+    for (const void* q : ptrs)
+        if (reinterpret_cast<uintptr_t>(q) & 15u) vec = 0;
+    if (s_copy && ev_sorted && ev_copied) {  // beside the work that follows on s
+        hipError_t e = hipEventRecord(ev_sorted, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s_copy, ev_sorted, 0);
+        if (e != hipSuccess) return e;
+        copy6_kernel<<<(unsigned)cb, 256, 0, s_copy>>>(t0, io, n, vec);
+        return hipEventRecord(ev_copied, s_copy);
+    }
+    copy6_kernel<<<(unsigned)cb, 256, 0, s>>>(t0, io, n, vec);
     return hipGetLastError();
 }

@@ -140,6 +140,19 @@ size_t ps_hip_worst_case_nodes(size_t n);
  * has made in this process (count and bytes; ps_hip_malloc not included) */
 int ps_hip_alloc_stats(uint64_t* calls, uint64_t* bytes);
 
+/* Test hook, no counterpart in the reference: reach the library's error
+ * paths without a real fault.  PS_HIP_FAIL_ALLOC: of the device and pinned
+ * host allocations from now on, the one after `after` others reports out of
+ * memory (PS_HIP_EOOM from the call that made it).  PS_HIP_FAIL_WAIT: of the
+ * library's host waits on its own device work, the one after `after` others
+ * reports a device failure once it has waited (PS_HIP_EDEVICE).  Each arming
+ * fails one operation; after < 0 disarms one kind, PS_HIP_FAIL_OFF both.
+ * Process-wide; PS_HIP_EINVAL for an unknown kind. */
+#define PS_HIP_FAIL_OFF   0
+#define PS_HIP_FAIL_ALLOC 1
+#define PS_HIP_FAIL_WAIT  2
+int ps_hip_debug_fail(int what, int after);
+
 /* ps_destroy, H:2172-2192 */
 int ps_hip_engine_destroy(ps_hip_engine_t* e);
 

@@ -152,6 +152,7 @@ def hip_lib():
             "ps_hip_engine_scratch_bytes": ([_sz], _sz),
             "ps_hip_worst_case_nodes": ([_sz], _sz),
             "ps_hip_alloc_stats": ([C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+            "ps_hip_debug_fail": ([C.c_int, C.c_int], C.c_int),
             "ps_hip_calc_forces_async": ([_vp] * 9 + [_sz, _vp, _f, _f, _f, _f, _vp], C.c_int),
             "ps_hip_calc_forces_wait": ([_vp], C.c_int),
             "ps_hip_device_name": ([C.c_char_p, _sz], C.c_int),
@@ -325,6 +326,15 @@ def alloc_stats():
     c, b = C.c_uint64(), C.c_uint64()
     check("ps_hip_alloc_stats", hip_lib().ps_hip_alloc_stats(C.byref(c), C.byref(b)))
     return c.value, b.value
+
+
+FAIL_OFF, FAIL_ALLOC, FAIL_WAIT = 0, 1, 2
+
+
+def debug_fail(what: int, after: int = 0):
+    """Arm the library's failure hook: the allocation (FAIL_ALLOC) or host
+    wait (FAIL_WAIT) after `after` others fails once; FAIL_OFF disarms."""
+    check("ps_hip_debug_fail", hip_lib().ps_hip_debug_fail(int(what), int(after)))
 
 
 class Stream:

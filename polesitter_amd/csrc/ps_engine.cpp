@@ -69,7 +69,19 @@ struct DevBuf {
 // device and pinned host allocations the library has made (process-wide)
 std::atomic<unsigned long long> g_alloc_calls{0}, g_alloc_bytes{0};
 
+// ps_hip_debug_fail's countdowns (-1: off): the allocation, or the host
+// wait, that the count reaches reports a failure, once
+std::atomic<int> g_fail_alloc{-1}, g_fail_wait{-1};
+
+bool fail_now(std::atomic<int>& c) {
+    int v = c.load(std::memory_order_relaxed);
+    while (v >= 0)
+        if (c.compare_exchange_weak(v, v - 1, std::memory_order_relaxed)) return v == 0;
+    return false;
+}
+
 hipError_t dev_malloc(void** p, size_t bytes) {
+    if (fail_now(g_fail_alloc)) return hipErrorOutOfMemory;
     hipError_t e = hipMalloc(p, bytes);
     if (e == hipSuccess) {
         g_alloc_calls.fetch_add(1, std::memory_order_relaxed);
@@ -79,12 +91,25 @@ hipError_t dev_malloc(void** p, size_t bytes) {
 }
 
 hipError_t host_malloc(void** p, size_t bytes) {
+    if (fail_now(g_fail_alloc)) return hipErrorOutOfMemory;
     hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
     if (e == hipSuccess) {
         g_alloc_calls.fetch_add(1, std::memory_order_relaxed);
         g_alloc_bytes.fetch_add(bytes, std::memory_order_relaxed);
     }
     return e;
+}
+
+// the library's host waits on its own work; a failure the device reports
+// surfaces here
+hipError_t wait_stream(hipStream_t s) {
+    const hipError_t e = hipStreamSynchronize(s);
+    return e == hipSuccess && fail_now(g_fail_wait) ? hipErrorLaunchFailure : e;
+}
+
+hipError_t wait_event(hipEvent_t ev) {
+    const hipError_t e = hipEventSynchronize(ev);
+    return e == hipSuccess && fail_now(g_fail_wait) ? hipErrorLaunchFailure : e;
 }
 
 // grow-only device buffer; synchronises the stream before replacing memory.
@@ -464,7 +489,7 @@ int stage_sort(ps_hip_engine* e) {
                                   e->delta_count(), e->s));
         PS_CK(hipMemcpyAsync(reinterpret_cast<char*>(e->h_small) + 160, e->delta_count(), 8,
                              hipMemcpyDeviceToHost, e->s));
-        PS_CK(hipStreamSynchronize(e->s));
+        PS_CK(wait_stream(e->s));
         const unsigned long long nc = e->host_delta_count();
         e->st.sort_changed = nc;
         if (nc * 8 <= e->n) {
@@ -515,7 +540,7 @@ int stage_build(ps_hip_engine* e) {
     ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
                        e->totals(), e->node_cap, e->s);
     e->l2l_pending = false;
-    PS_CK(hipEventSynchronize(e->ev_counts));
+    PS_CK(wait_event(e->ev_counts));
     const uint32_t flags = e->h_small[12];
     const unsigned long long total =
         *reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(e->h_small) + 64);
@@ -1143,6 +1168,19 @@ int ps_hip_engine_destroy(ps_hip_engine_t* e) {
     return PS_HIP_OK;
 }
 
+int ps_hip_debug_fail(int what, int after) {
+    const int v = after < 0 ? -1 : after;
+    switch (what) {
+        case PS_HIP_FAIL_OFF:
+            g_fail_alloc.store(-1);
+            g_fail_wait.store(-1);
+            return PS_HIP_OK;
+        case PS_HIP_FAIL_ALLOC: g_fail_alloc.store(v); return PS_HIP_OK;
+        case PS_HIP_FAIL_WAIT: g_fail_wait.store(v); return PS_HIP_OK;
+        default: return PS_HIP_EINVAL;
+    }
+}
+
 int ps_hip_alloc_stats(uint64_t* calls, uint64_t* bytes) {
     if (calls) *calls = g_alloc_calls.load(std::memory_order_relaxed);
     if (bytes) *bytes = g_alloc_bytes.load(std::memory_order_relaxed);
@@ -1225,7 +1263,7 @@ int ps_hip_engine_set_mlp(ps_hip_engine_t* e, const float* params) {
     PS_CK(hipMemcpyAsync(e->mlp_b.p, params, 4 * PS_MLP_PARAMS, hipMemcpyDefault, e->s));
     ps_launch_mlp_pack(reinterpret_cast<const float*>(e->mlp_b.p), e->mlp_pk_b.p, e->s);
     PS_CK(hipGetLastError());
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     e->mlp_on = true;
     return PS_HIP_OK;
 }
@@ -1237,7 +1275,7 @@ int ps_hip_mlp_features(ps_hip_engine_t* e, float* feat, float* scale) {
     ps_launch_mlp_features(e->n, nullptr, e->n, e->pos4(), e->leaf_of(), e->geom(), e->mp(),
                            e->loc(), e->parent(), feat, scale, e->s);
     PS_CK(hipGetLastError());
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     return PS_HIP_OK;
 }
 
@@ -1251,7 +1289,7 @@ int ps_hip_mlp_features_at(ps_hip_engine_t* e, const uint32_t* idx, size_t rows,
     ps_launch_mlp_features(e->n, idx, rows, e->pos4(), e->leaf_of(), e->geom(), e->mp(),
                            e->loc(), e->parent(), feat, scale, e->s);
     PS_CK(hipGetLastError());
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     return PS_HIP_OK;
 }
 
@@ -1263,7 +1301,7 @@ int ps_hip_mlp_forward(ps_hip_engine_t* e, const float* params, const float* fea
         return PS_HIP_EINVAL;
     ps_launch_mlp_forward(params, feat, n, out3, e->s);
     PS_CK(hipGetLastError());
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     return PS_HIP_OK;
 }
 
@@ -1280,7 +1318,7 @@ int ps_hip_mlp_grad(ps_hip_engine_t* e, const float* params, const float* feat,
     PS_CK(hipGetLastError());
     float sse = 0.f;
     PS_CK(hipMemcpyAsync(&sse, loss_dev, 4, hipMemcpyDeviceToHost, e->s));
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     if (loss_host) *loss_host = sse / (float)n;
     return PS_HIP_OK;
 }
@@ -1361,7 +1399,7 @@ int ps_hip_batch_set_leaf_caps(ps_hip_batch_t* b, const uint8_t* caps, size_t n_
         if (caps[k] < 1 || caps[k] > 64) return PS_HIP_EINVAL;
     PS_CK(grow(b->caps_b, b->max_sys, b->e->s));
     PS_CK(hipMemcpyAsync(b->caps_b.p, caps, n_systems, hipMemcpyHostToDevice, b->e->s));
-    PS_CK(hipStreamSynchronize(b->e->s));
+    PS_CK(wait_stream(b->e->s));
     b->caps_n = n_systems;
     return PS_HIP_OK;
 }
@@ -1418,7 +1456,7 @@ int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z, float* ma
     for (int attempt = 0;; ++attempt) {
         PS_CK(hipMemcpyAsync(&hst, e->tstate_b.p, sizeof(hst), hipMemcpyDeviceToHost, s));
         PS_CK(hipMemcpyAsync(&nodes, e->totals(), 8, hipMemcpyDeviceToHost, s));
-        PS_CK(hipStreamSynchronize(s));
+        PS_CK(wait_stream(s));
         if (!hst.overflow) break;
         // an arena was too small: grow past the need and run the tick again
         // from the traversal (the sorted arrays and the forest stand; the
@@ -1516,7 +1554,7 @@ int ps_hip_prepare(const float* x, const float* y, const float* z, size_t n, uin
     ps_launch_morton(d[0], d[1], d[2], n, prm, dcode, morton_mode, w->s);
     PS_CK(hipMemcpyAsync(w->h_out, prm, 16, hipMemcpyDeviceToHost, w->s));
     if (host_codes) PS_CK(hipMemcpyAsync(codes, dcode, n * 4, hipMemcpyDeviceToHost, w->s));
-    PS_CK(hipStreamSynchronize(w->s));
+    PS_CK(wait_stream(w->s));
     if (out_min) *out_min = w->h_out[0];
     if (out_max) *out_max = w->h_out[1];
     if (out_range) *out_range = w->h_out[2];
@@ -1582,7 +1620,7 @@ int ps_hip_calc_forces(ps_hip_engine_t* e, float* x, float* y, float* z, float* 
         if (e->count_on)
             PS_CK(hipMemcpyAsync(e->h_small + 32, e->pairs(), 16, hipMemcpyDeviceToHost, e->s));
         e->record(9);
-        PS_CK(hipStreamSynchronize(e->s));
+        PS_CK(wait_stream(e->s));
         bool redo = false;
         PS_RC(trav_finish(e, attempt, &redo));
         if (!redo) break;
@@ -1653,7 +1691,7 @@ int ps_hip_calc_forces_wait(ps_hip_engine_t* e) {
     if (!e) return PS_HIP_EINVAL;
     if (!e->pending) return PS_HIP_OK;
     e->pending = false;
-    PS_CK(hipEventSynchronize(e->ev_async));
+    PS_CK(wait_event(e->ev_async));
     const uint32_t flags = e->h_small[12];
     const unsigned long long total =
         *reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(e->h_small) + 64);
@@ -1671,7 +1709,7 @@ int ps_hip_calc_forces_wait(ps_hip_engine_t* e) {
         ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
                            e->totals(), e->node_alloc, e->async_s);
         PS_RC(async_back(e, e->async_s));
-        PS_CK(hipEventSynchronize(e->ev_async));
+        PS_CK(wait_event(e->ev_async));
     }
     std::memcpy(e->ls, e->h_small, sizeof(e->ls));
     e->n_nodes = (uint32_t)total;
@@ -1697,7 +1735,7 @@ int ps_hip_engine_export_tree(ps_hip_engine_t* e, size_t cap, float* geom4, floa
     const size_t M = e->n_nodes;
     if (cap < M) return PS_HIP_EOOM;
     if (loc4) PS_RC(materialize_locals(e));
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     if (geom4) PS_CK(hipMemcpy(geom4, e->geom(), 16 * M, hipMemcpyDeviceToHost));
     if (mp4) PS_CK(hipMemcpy(mp4, e->mp(), 16 * M, hipMemcpyDeviceToHost));
     if (loc4) PS_CK(hipMemcpy(loc4, e->loc(), 16 * M, hipMemcpyDeviceToHost));
@@ -1736,7 +1774,7 @@ int ps_hip_debug_stage(ps_hip_engine_t* e, int stage) {
             for (int attempt = 0;; ++attempt) {
                 rc = stage_traverse(e);
                 if (rc != PS_HIP_OK) break;
-                PS_CK(hipStreamSynchronize(e->s));
+                PS_CK(wait_stream(e->s));
                 bool redo = false;
                 rc = trav_finish(e, attempt, &redo);
                 if (rc != PS_HIP_OK || !redo) break;
@@ -1751,7 +1789,7 @@ int ps_hip_debug_stage(ps_hip_engine_t* e, int stage) {
             for (int attempt = 0;; ++attempt) {
                 rc = run_from_traverse(e);
                 if (rc != PS_HIP_OK) break;
-                PS_CK(hipStreamSynchronize(e->s));
+                PS_CK(wait_stream(e->s));
                 bool redo = false;
                 rc = trav_finish(e, attempt, &redo);
                 if (rc != PS_HIP_OK || !redo) break;
@@ -1762,7 +1800,7 @@ int ps_hip_debug_stage(ps_hip_engine_t* e, int stage) {
     const int rj = finish_copy(e);
     if (rc != PS_HIP_OK) return rc;
     if (rj != PS_HIP_OK) return rj;
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     return PS_HIP_OK;
 }
 
@@ -1770,7 +1808,7 @@ int ps_hip_debug_set_local(ps_hip_engine_t* e, uint32_t node, const float* field
     if (!e || !e->built || !field3 || node >= e->n_nodes) return PS_HIP_EINVAL;
     float v[4] = {0.f, field3[0], field3[1], field3[2]};
     PS_CK(hipMemcpyAsync(e->loc() + node, v, 16, hipMemcpyHostToDevice, e->s));
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     return PS_HIP_OK;
 }
 
@@ -1842,7 +1880,7 @@ int ps_hip_direct_f64(const float* x, const float* y, const float* z, const floa
 int ps_hip_step_reset_ids(ps_hip_engine_t* e, uint32_t* id, size_t n) {
     if (!e || !id || !is_device_ptr(id)) return PS_HIP_EINVAL;
     ps_launch_iota(id, n, e->s);
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     return PS_HIP_OK;
 }
 
@@ -1853,7 +1891,7 @@ int ps_hip_step_gather(ps_hip_engine_t* e, const uint32_t* id, float* vx, float*
     PS_CK(hipMemcpyAsync(vx, tmp3, 4 * n, hipMemcpyDeviceToDevice, e->s));
     PS_CK(hipMemcpyAsync(vy, tmp3 + n, 4 * n, hipMemcpyDeviceToDevice, e->s));
     PS_CK(hipMemcpyAsync(vz, tmp3 + 2 * n, 4 * n, hipMemcpyDeviceToDevice, e->s));
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     return PS_HIP_OK;
 }
 
@@ -1863,7 +1901,7 @@ int ps_hip_step_euler(ps_hip_engine_t* e, float* x, float* y, float* z, float* v
     if (!e || !x || !y || !z || !vx || !vy || !vz || !fx || !fy || !fz || !mass)
         return PS_HIP_EINVAL;
     ps_launch_euler(x, y, z, vx, vy, vz, fx, fy, fz, mass, dt, n, e->s);
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     return PS_HIP_OK;
 }
 
@@ -1874,7 +1912,7 @@ int ps_hip_step_frame(ps_hip_engine_t* e, uint32_t* id, float* vx, float* vy, fl
         return PS_HIP_EINVAL;
     ps_launch_frame(id, vx, vy, vz, tmp3, x, y, z, fx, fy, fz, mass, dt, n, e->s);
     PS_CK(hipGetLastError());
-    PS_CK(hipStreamSynchronize(e->s));
+    PS_CK(wait_stream(e->s));
     return PS_HIP_OK;
 }
 

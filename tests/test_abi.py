@@ -126,6 +126,32 @@ def test_invalid_arguments_without_device():
     assert L.ps_calc_forces(None, None, None, 0.0, 0.0, 0.0, 1.0) == api.PS_EINVAL
 
 
+def test_failure_hook_arms_and_disarms_without_device():
+    L = api.hip_lib()
+    assert L.ps_hip_debug_fail(7, 0) == api.PS_EINVAL  # unknown kind
+    for what in (api.FAIL_ALLOC, api.FAIL_WAIT):
+        api.debug_fail(what, 3)
+        api.debug_fail(what, -1)
+    api.debug_fail(api.FAIL_OFF)
+
+
+def _has_device():
+    try:
+        api.device_name()
+        return True
+    except api.PsError:
+        return False
+
+
+def test_init_without_device_is_a_device_error():
+    # ps_init reaches the runtime, which has no device here: PS_EDEVICE, and
+    # no context comes back
+    if _has_device():
+        pytest.skip("a device is present")
+    rc, ctx = api.ps_init(api.buff_size_for(1000), 1000, 2.0)
+    assert rc == api.PS_EDEVICE and ctx is None
+
+
 def test_dropin_program_is_built():
     assert os.access(os.path.join(api.LIBDIR, "dropin_tick"), os.X_OK)
 

@@ -76,6 +76,24 @@ __device__ __forceinline__ void ps_m2l_d(float dx, float dy, float dz, float d2,
     ez += a * dz + smp.w * ir3;
 }
 
+// ps_m2l_d for every lane, adding only where `on`: elsewhere the inverse
+// distance is 0, so each term added is +-0.  An accumulator that starts at +0
+// never holds -0 (+0 + -0 = +0 in round-to-nearest), so adding +-0 leaves it
+// unchanged bit for bit: the same sums as calling ps_m2l_d where `on` only.
+__device__ __forceinline__ void ps_m2l_dm(float dx, float dy, float dz, float d2, float4 smp,
+                                          bool on, float& ex, float& ey, float& ez) {
+    const float r = __builtin_amdgcn_rsqf(d2 + PS_SOFT2);
+    const float ir = on ? r : 0.0f;
+    const float ir2 = ir * ir;
+    const float ir3 = ir2 * ir;
+    const float ir5 = ir3 * ir2;
+    const float mdr = smp.y * dx + smp.z * dy + smp.w * dz;
+    const float a = smp.x * ir3 - 3.0f * mdr * ir5;
+    ex += a * dx + smp.y * ir3;
+    ey += a * dy + smp.z * ir3;
+    ez += a * dz + smp.w * ir3;
+}
+
 __device__ __forceinline__ void ps_m2l(float4 t, float4 s, float4 smp, float& ex, float& ey,
                                        float& ez) {
     float dx, dy, dz, d2;

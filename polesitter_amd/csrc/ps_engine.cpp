@@ -136,6 +136,12 @@ hipError_t grow(DevBuf& b, size_t need, hipStream_t s) {
     return hipSuccess;
 }
 
+// capacity of the list arena in words, PS_LIST_PAD words short of its size
+uint32_t list_cap_words(const DevBuf& b) {
+    const size_t w = b.bytes / 4;
+    return (uint32_t)std::min<size_t>(w > PS_LIST_PAD ? w - PS_LIST_PAD : 0, 0xFFFFFFFFull);
+}
+
 void release(DevBuf& b) {
     if (b.p && !b.fixed) (void)hipFree(b.p);
     b.p = nullptr;
@@ -589,7 +595,7 @@ int trav_lanes_for(const ps_hip_engine* e, uint32_t ub) {
 int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     const uint32_t n_nodes = e->n_nodes;
     PsTravState* st = reinterpret_cast<PsTravState*>(e->tstate_b.p);
-    const uint32_t arena_cap = (uint32_t)std::min<size_t>(e->arena_b.bytes / 4, 0xFFFFFFFFull);
+    const uint32_t arena_cap = list_cap_words(e->arena_b);
     const uint32_t mask_cap = (uint32_t)std::min<size_t>(e->masks_b.bytes / 8, 0xFFFFFFFFull);
     ps_launch_trav_init(e->small_u32(), mt ? 1 : 0, 1u, nullptr, e->nend(), e->link(),
                         e->entries(0), st, e->arena(), arena_cap, mask_cap,
@@ -820,7 +826,7 @@ int async_back(ps_hip_engine* e, hipStream_t cs) {
     const float theta2 = theta_squared(e);
     const bool mt = e->trav == PS_HIP_TRAV_MT;
     PsTravState* st = reinterpret_cast<PsTravState*>(e->tstate_b.p);
-    const uint32_t arena_cap = (uint32_t)std::min<size_t>(e->arena_b.bytes / 4, 0xFFFFFFFFull);
+    const uint32_t arena_cap = list_cap_words(e->arena_b);
     const uint32_t mask_cap = (uint32_t)std::min<size_t>(e->masks_b.bytes / 8, 0xFFFFFFFFull);
     ps_launch_trav_init(ls, mt ? 1 : 0, 1u, nullptr, e->nend(), e->link(), e->entries(0), st,
                         e->arena(), arena_cap, mask_cap, ub, cs);
@@ -982,7 +988,7 @@ int batch_run_back(ps_hip_batch* b, uint32_t n_sys) {
     }
     const bool mt = e->trav == PS_HIP_TRAV_MT;
     PsTravState* st = reinterpret_cast<PsTravState*>(e->tstate_b.p);
-    const uint32_t arena_cap = (uint32_t)std::min<size_t>(e->arena_b.bytes / 4, 0xFFFFFFFFull);
+    const uint32_t arena_cap = list_cap_words(e->arena_b);
     const uint32_t mask_cap = (uint32_t)std::min<size_t>(e->masks_b.bytes / 8, 0xFFFFFFFFull);
     const uint32_t ub = (uint32_t)std::min<size_t>(cap, 0xFFFFFFFFull);
     ps_launch_trav_init(ls, mt ? 1 : 0, n_sys, sys, e->nend(), e->link(), e->entries(0), st,
@@ -1105,7 +1111,9 @@ int ps_hip_engine_create_in(ps_hip_engine_t** out, size_t max_particles, size_t 
     }
     std::memset(&e->st, 0, sizeof(e->st));
     e->max_p = std::max<size_t>(max_particles, 1);
-    e->node_cap = std::max<size_t>(std::min(max_nodes, worst_case_nodes(e->max_p)), 1);
+    // the traversal's 32-bit node offsets bound every tree
+    e->node_cap = std::max<size_t>(
+        std::min({max_nodes, worst_case_nodes(e->max_p), (size_t)PS_TRAV_MAX_NODES}), 1);
     e->theta = theta > 0.0f ? theta : 1.0f;
     e->trav = traversal == PS_HIP_TRAV_ST ? PS_HIP_TRAV_ST : PS_HIP_TRAV_MT;
     if (host_tail) {
@@ -1457,6 +1465,7 @@ int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z, float* ma
         PS_CK(hipMemcpyAsync(&hst, e->tstate_b.p, sizeof(hst), hipMemcpyDeviceToHost, s));
         PS_CK(hipMemcpyAsync(&nodes, e->totals(), 8, hipMemcpyDeviceToHost, s));
         PS_CK(wait_stream(s));
+        if (nodes > PS_TRAV_MAX_NODES) return PS_HIP_EOOM;  // past the traversal's offsets
         if (!hst.overflow) break;
         // an arena was too small: grow past the need and run the tick again
         // from the traversal (the sorted arrays and the forest stand; the

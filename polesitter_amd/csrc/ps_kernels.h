@@ -165,9 +165,16 @@ struct PsTravEntries {
 };
 // Device-resident bookkeeping of one traversal: every iteration is launched
 // without host synchronisation; sizes live here.
+// The traversal reads whole 64-word chunks of a list, past its end too:
+// the list arena keeps this many words past the capacity it reports.
+constexpr uint32_t PS_LIST_PAD = 64;
+// The traversal addresses node records with 32-bit byte offsets: a tick's
+// nodes (or the bound its launches are given) stay at most this many.
+constexpr uint32_t PS_TRAV_MAX_NODES = 1u << 28;
+
 struct PsTravState {
     uint32_t n_ent[16];            // entries of iteration k (n_ent[0] from the init)
-    uint32_t arena_cap;            // capacities of the arenas (words)
+    uint32_t arena_cap;            // capacities of the arenas (words; see PS_LIST_PAD)
     uint32_t mask_cap;
     uint32_t overflow;             // 1: a capacity was exceeded, the traversal is redone
     uint32_t pad;

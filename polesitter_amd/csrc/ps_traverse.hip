@@ -163,6 +163,31 @@ __global__ __launch_bounds__(kBlock) void trav_init_kernel(const uint32_t* __res
 
 __device__ __forceinline__ uint32_t chunks_of(uint32_t len) { return (len + 63u) >> 6; }
 
+// Near records: what a near source passes on to the next list, kept in LDS
+// between the count and the write phase -- the first id (bits 0..28) and the
+// count - 1 (bits 29..31) of consecutive ids (a leaf: itself; an internal
+// node: its children).  Node ids stay below PS_TRAV_MAX_NODES <= 2^29.
+constexpr uint32_t kRecShift = 29;
+constexpr uint32_t kRecMask = (1u << kRecShift) - 1u;
+// An entry's records: (start << 16 | count) in its group's pool space, or
+// kNoPiece when they did not fit (the write phase then reads the list again).
+constexpr uint32_t kNoPiece = 0xFFFFFFFFu;
+constexpr uint32_t kPoolWords = 24576;  // the block's record pool (96 KiB)
+
+// Node record i from a node array: the byte offset is a 32-bit product, so
+// the load takes the array's address from scalar registers and one 32-bit
+// offset register (the host keeps a tick's nodes at most PS_TRAV_MAX_NODES).
+template <class T>
+__device__ __forceinline__ T node_ld(const T* __restrict__ base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                       (size_t)(i * (uint32_t)sizeof(T)));
+}
+
+// link[i].y (child mask and leaf flag) alone
+__device__ __forceinline__ uint32_t node_flags(const uint2* __restrict__ link, uint32_t i) {
+    return node_ld(reinterpret_cast<const uint32_t*>(link), 2u * i + 1u);
+}
+
 
 // Count of one entry by its group of G lanes (8, 16, 32 or 64).  The list is
 // taken in chunks of 64 sources, one far-mask word each; lane gl holds the
@@ -176,7 +201,8 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
                                              const uint32_t* __restrict__ arena,
                                              unsigned long long* __restrict__ masks,
                                              uint32_t mask_cap, uint32_t e, uint32_t gl,
-                                             uint32_t gshift) {
+                                             uint32_t gshift, uint32_t* pool, uint32_t& bump,
+                                             uint32_t pool_end, uint32_t& piece) {
     constexpr int U = 64 / G;
     const uint32_t tn = ent.tgt[e];
     const uint32_t o = ent.off[e];
@@ -188,60 +214,79 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
     float cx = 0.f, cy = 0.f, cz = 0.f;
     uint32_t nfar = 0, nleaf = 0, nchild = 0;
     for (uint32_t k0 = 0; k0 < L; k0 += 64) {
+        // Every slot loads: past the list's end the word comes from the
+        // next list or the arena's padding (PS_LIST_PAD words past its
+        // capacity) and the slot then evaluates the target itself, which is
+        // discarded -- no branches around the loads.
+        // slot j runs when some lane of the wavefront has a source in it
+        // (a scalar branch: the other groups' lists may be longer)
+        bool need[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) need[j] = __ballot(k0 + j * G + gl < L) != 0ull;
         uint32_t s[U];
+        const uint32_t* lp = arena + (size_t)(o + k0 + gl);
 #pragma unroll
-        for (int j = 0; j < U; ++j) {
-            const uint32_t k = k0 + j * G + gl;
-            s[j] = k < L ? arena[o + k] : PS_NONE;
-        }
+        for (int j = 0; j < U; ++j) s[j] = need[j] ? lp[j * G] : tn;
         float4 sg[U], smp[U];
-        uint32_t sy[U];
+        uint2 sl[U];
 #pragma unroll
         for (int j = 0; j < U; ++j) {
-            if (s[j] < t.n_nodes) {
-                sg[j] = t.geom[s[j]];
-                smp[j] = t.mp[s[j]];
-                sy[j] = t.link[s[j]].y;
-            } else {
-                sg[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-                smp[j] = sg[j];
-                sy[j] = 0u;
-            }
+            if (!need[j]) continue;
+            const bool ok = k0 + j * G + gl < L && s[j] < t.n_nodes;
+            const uint32_t sv = ok ? s[j] : tn;
+            sg[j] = node_ld(t.geom, sv);
+            smp[j] = node_ld(t.mp, sv);
+            sl[j] = node_ld(t.link, sv);
         }
         unsigned long long word = 0;
+        uint32_t nrec = 0;  // records of the chunk so far (uniform in the group)
 #pragma unroll
         for (int j = 0; j < U; ++j) {
-            const uint32_t k = k0 + j * G + gl;
-            bool far = false, near_leaf = false;
-            if (k < L) {
-                if (s[j] < t.n_nodes) {  // always true when counts and writes agree
-                    float dx, dy, dz, d2;
-                    far = ps_far_d(tg, sg[j], theta2, dx, dy, dz, d2);
-                    if (far) {
-                        if (kKahan) {
-                            float px = 0.f, py = 0.f, pz = 0.f;
-                            ps_m2l_d(dx, dy, dz, d2, smp[j], px, py, pz);
-                            ps_kahan_add(ex, cx, px);
-                            ps_kahan_add(ey, cy, py);
-                            ps_kahan_add(ez, cz, pz);
-                        } else {
-                            ps_m2l_d(dx, dy, dz, d2, smp[j], ex, ey, ez);
-                        }
-                    } else if (sy[j] & PS_LEAF) {
-                        near_leaf = true;
-                    } else {
-                        nchild += (uint32_t)__popc(sy[j] & 0xFFu);
-                    }
-                } else {
-                    far = true;  // an invalid id is skipped by every consumer
+            if (!need[j]) continue;
+            const bool in = k0 + j * G + gl < L;
+            // an id past the nodes is never written when counts and writes
+            // agree; it is skipped (a far bit without M2L) by every consumer
+            const bool ok = in && s[j] < t.n_nodes;
+            float dx, dy, dz, d2;
+            const bool mac = ps_far_d(tg, sg[j], theta2, dx, dy, dz, d2);
+            const bool far = ok && mac;
+            if (kKahan) {
+                if (far) {
+                    float px = 0.f, py = 0.f, pz = 0.f;
+                    ps_m2l_d(dx, dy, dz, d2, smp[j], px, py, pz);
+                    ps_kahan_add(ex, cx, px);
+                    ps_kahan_add(ey, cy, py);
+                    ps_kahan_add(ez, cz, pz);
                 }
+            } else {  // no branch: most slots are far
+                ps_m2l_dm(dx, dy, dz, d2, smp[j], far, ex, ey, ez);
             }
-            const unsigned long long fb = group_bits<G>(far, gshift);
+            const bool near = ok && !mac;
+            const bool leaf = (sl[j].y & PS_LEAF) != 0u;
+            const uint32_t nk = (uint32_t)__popc(sl[j].y & 0xFFu);
+            nchild += (near && !leaf) ? nk : 0u;
+            nleaf += (near && leaf) ? 1u : 0u;  // this lane's; summed over the group below
+            const unsigned long long fb = group_bits<G>(far || (in && !ok), gshift);
             word |= fb << (j * G);
-            nleaf += near_leaf ? 1u : 0u;  // this lane's; summed over the group below
+            // the record of a near source, in list order after the chunk's
+            // earlier ones: a near leaf passes on itself, a near internal
+            // node its nk children (consecutive ids from its first child)
+            const unsigned long long nb = group_bits<G>(near, gshift);
+            const uint32_t pos = bump + nrec + (uint32_t)__popcll(nb & ((1ull << gl) - 1ull));
+            if (near && pos < pool_end)
+                pool[pos] = leaf ? s[j] : (sl[j].x | ((nk - 1u) << kRecShift));
+            nrec += (uint32_t)__popcll(nb);
         }
         if (gl == 0 && mo + (k0 >> 6) < mask_cap) masks[mo + (k0 >> 6)] = word;
         nfar += (uint32_t)__popcll(word);
+        if (piece != kNoPiece) {
+            if (bump + nrec <= pool_end) {
+                bump += nrec;
+            } else {  // no room: the write phase reads the list again
+                bump = piece;  // the entry's start
+                piece = kNoPiece;
+            }
+        }
     }
     nchild = group_sum_u32<G>(nchild);
     nleaf = group_sum_u32<G>(nleaf);
@@ -264,6 +309,7 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
     uint4 c = make_uint4(0u, 0u, 0u, nfar);
     if (tl.y & PS_LEAF) {
         if (nchild == 0) {  // final: the near entries of this list are the P2P sources
+            if (piece != kNoPiece) bump = piece;  // its records are not needed
             if (nleaf && gl == 0) {
                 t.p2p_off[tn] = o;
                 t.p2p_len[tn] = L;
@@ -284,6 +330,30 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
         t.loc[tn] = l;
     }
     return c;
+}
+
+// The next iteration's entries of a continuing target tn: itself (a leaf) or
+// its children, all with the next list at p.x of length out.
+template <int G>
+__device__ __forceinline__ void write_next(const PsTravEntries& next, uint32_t tn, uint2 tl,
+                                           uint32_t out, uint4 p, uint32_t gl) {
+    const uint32_t nch = chunks_of(out);
+    if (tl.y & PS_LEAF) {
+        if (gl == 0) {
+            next.tgt[p.y] = tn;
+            next.off[p.y] = p.x;
+            next.len[p.y] = out;
+            next.moff[p.y] = p.z;
+        }
+    } else {
+        const uint32_t nk = ps_child_count(tl);
+        for (uint32_t r = gl; r < nk; r += G) {
+            next.tgt[p.y + r] = tl.x + r;
+            next.off[p.y + r] = p.x;
+            next.len[p.y + r] = out;
+            next.moff[p.y + r] = p.z + r * nch;
+        }
+    }
 }
 
 // Write of one continuing entry by its group: out = its next list length
@@ -309,24 +379,26 @@ __device__ __forceinline__ void write_entry(const PsTravTree& t, const PsTravEnt
     const uint32_t limit = base + out;
     for (uint32_t k0 = 0; k0 < L; k0 += 64) {
         const unsigned long long fm = masks[mo + (k0 >> 6)];
+        bool need[U];  // as in count_entry
+#pragma unroll
+        for (int j = 0; j < U; ++j) need[j] = __ballot(k0 + j * G + gl < L) != 0ull;
         uint32_t s[U];
+        const uint32_t* lp = arena + (size_t)(o + k0 + gl);
 #pragma unroll
-        for (int j = 0; j < U; ++j) {
-            const uint32_t k = k0 + j * G + gl;
-            s[j] = k < L ? arena[o + k] : PS_NONE;
-        }
+        for (int j = 0; j < U; ++j) s[j] = need[j] ? lp[j * G] : PS_NONE;
         uint2 sl[U];
+        bool nr[U];
 #pragma unroll
         for (int j = 0; j < U; ++j) {
-            const bool near = s[j] != PS_NONE && !((fm >> (j * G + gl)) & 1ull);
-            if (!near) s[j] = PS_NONE;
-            sl[j] = (near && s[j] < t.n_nodes) ? t.link[s[j]] : make_uint2(0u, 0u);
+            // near (a clear far bit) implies a valid id: invalid ones carry a far bit
+            nr[j] = k0 + j * G + gl < L && !((fm >> (j * G + gl)) & 1ull) && s[j] < t.n_nodes;
+            sl[j] = make_uint2(0u, 0u);
+            if (nr[j]) sl[j] = node_ld(t.link, s[j]);  // the near sources only
         }
 #pragma unroll
         for (int j = 0; j < U; ++j) {
             const uint32_t nout =
-                s[j] == PS_NONE ? 0u
-                                : ((sl[j].y & PS_LEAF) ? 1u : (uint32_t)__popc(sl[j].y & 0xFFu));
+                !nr[j] ? 0u : ((sl[j].y & PS_LEAF) ? 1u : (uint32_t)__popc(sl[j].y & 0xFFu));
             uint32_t total;
             const uint32_t inc = group_scan_u32<G>(nout, gl, total);
             const uint32_t w = base + inc - nout;
@@ -342,23 +414,38 @@ __device__ __forceinline__ void write_entry(const PsTravTree& t, const PsTravEnt
             base += total;
         }
     }
-    const uint32_t nch = chunks_of(out);
-    if (tl.y & PS_LEAF) {
-        if (gl == 0) {
-            next.tgt[p.y] = tn;
-            next.off[p.y] = p.x;
-            next.len[p.y] = out;
-            next.moff[p.y] = p.z;
+    write_next<G>(next, tn, tl, out, p, gl);
+}
+
+// Write of one continuing entry from its near records in LDS (count of
+// them: n), the same list words in the same order as write_entry.
+template <int G>
+__device__ __forceinline__ void write_entry_rec(const PsTravTree& t, const PsTravEntries& ent,
+                                                uint32_t* __restrict__ arena,
+                                                const uint32_t* recs, uint32_t n, uint32_t e,
+                                                uint32_t out, uint4 p, const PsTravEntries& next,
+                                                uint32_t gl) {
+    const uint32_t tn = ent.tgt[e];
+    const uint2 tl = t.link[tn];
+    uint32_t base = p.x;
+    const uint32_t limit = base + out;
+    for (uint32_t r0 = 0; r0 < n; r0 += G) {
+        const uint32_t i = r0 + gl;
+        const uint32_t rc = i < n ? recs[i] : 0u;
+        const uint32_t nout = i < n ? (rc >> kRecShift) + 1u : 0u;
+        uint32_t total;
+        const uint32_t inc = group_scan_u32<G>(nout, gl, total);
+        const uint32_t w = base + inc - nout;
+        if (nout && w + nout <= limit) {
+            uint32_t* dst = arena + w;
+            const uint32_t v0 = rc & kRecMask;
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r)
+                if (r < nout) dst[r] = v0 + r;
         }
-    } else {
-        const uint32_t nk = ps_child_count(tl);
-        for (uint32_t r = gl; r < nk; r += G) {
-            next.tgt[p.y + r] = tl.x + r;
-            next.off[p.y + r] = p.x;
-            next.len[p.y + r] = out;
-            next.moff[p.y + r] = p.z + r * nch;
-        }
+        base += total;
     }
+    write_next<G>(next, tn, tl, out, p, gl);
 }
 
 // next item of a block-local work queue, the same value in every lane of the group
@@ -383,6 +470,8 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
     __shared__ uint32_t sbin[kBins][kSeg / 64];  // per 64 entries: counts, then offsets
     __shared__ uint16_t sord[kSeg];              // the segment's entries, longest lists first
     __shared__ uint32_t sseg[2];                 // the next segment: first entry, length
+    __shared__ uint32_t spool[kPoolWords];       // near records, kPoolWords / kGroups per group
+    __shared__ uint32_t spc[kSeg];               // per entry: its records (or kNoPiece)
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t grp = tid / G;
     const uint32_t gl = tid & (G - 1);
@@ -479,11 +568,18 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
         }
         __syncthreads();
         // count
+        constexpr uint32_t kPoolPer = kPoolWords / kGroups;
+        uint32_t bump = grp * kPoolPer;  // the group's pool space: [grp, grp + 1) * kPoolPer
         for (uint32_t i = group_take<G>(&queue[0], gl); i < segn; i = group_take<G>(&queue[0], gl)) {
             const uint32_t k = sord[i];
+            uint32_t piece = bump;
             const uint4 c = count_entry<kKahan, G>(t, theta2, ent, arena, masks, mask_cap, seg + k,
-                                                   gl, gshift);
-            if (gl == 0) scnt[k] = c;
+                                                   gl, gshift, spool, bump, (grp + 1u) * kPoolPer,
+                                                   piece);
+            if (gl == 0) {
+                scnt[k] = c;
+                spc[k] = piece == kNoPiece ? kNoPiece : (piece | ((bump - piece) << 16));
+            }
             far_acc += c.w;
         }
         __syncthreads();
@@ -559,8 +655,13 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
                 const uint32_t k = sord[i];
                 const uint4 q = scnt[k];
                 if (q.w == 0) continue;  // final or empty: nothing to pass on
-                write_entry<G>(t, ent, arena, masks, seg + k, q.w,
-                               make_uint4(la + q.x, le + q.y, lm + q.z, 0u), next, gl);
+                const uint4 pl = make_uint4(la + q.x, le + q.y, lm + q.z, 0u);
+                const uint32_t pc = spc[k];
+                if (pc != kNoPiece)
+                    write_entry_rec<G>(t, ent, arena, spool + (pc & 0xFFFFu), pc >> 16, seg + k,
+                                       q.w, pl, next, gl);
+                else
+                    write_entry<G>(t, ent, arena, masks, seg + k, q.w, pl, next, gl);
             }
         }
         __syncthreads();  // scnt and sbase are rewritten by the next segment

@@ -50,7 +50,10 @@ import numpy as np
 FAMILIES = {
     "trav_step": ("trav_step_kernel",),
     "p2p": ("p2p_kernel", "p2p_big_kernel", "p2p_collect_kernel"),
-    "sort": ("sort_upsweep_kernel", "sort_rowscan_kernel", "sort_downsweep_kernel"),
+    "sort": ("sort_upsweep_kernel", "sort_rowscan_kernel", "sort_downsweep_kernel",
+             "delta_mask_kernel", "delta_scan_kernel", "delta_compact_kernel",
+             "delta_pack_kernel", "delta_small_sort_kernel", "delta_merge_kernel",
+             "delta_scatter_kernel", "copy6_kernel"),
     "build": ("tree_count_kernel", "tree_levels_kernel", "tree_write_kernel", "tree_mask_kernel"),
     "upward": ("upward_level_kernel", "upward_level_dev_kernel"),
     "l2l": ("leaf_locals_kernel", "l2l_level_kernel", "l2l_level_dev_kernel"),

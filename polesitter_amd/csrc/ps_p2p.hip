@@ -343,6 +343,64 @@ __device__ __forceinline__ void p2p_big_round(uint32_t c0, uint32_t te, uint32_t
     }
 }
 
+// The leaf's last r < 64 targets, k = 2^lk lanes each (the largest power of
+// two with r k <= 64): lane t k + q takes target c0 + t and every k-th
+// source particle from the q-th; a butterfly over the k lanes adds the
+// partial sums.  More lanes are busy than with one target per lane (r / 64
+// of them), at the price of a vector load per source step (k addresses).
+template <int kMode>
+__device__ __forceinline__ void p2p_split_round(uint32_t c0, uint32_t te, uint32_t lane, size_t n,
+                                                uint32_t L, uint32_t o, uint32_t mo, float4 fl,
+                                                const float4* __restrict__ pos4,
+                                                const uint32_t* __restrict__ arena,
+                                                const unsigned long long* __restrict__ masks,
+                                                const uint2* __restrict__ link,
+                                                const uint32_t* __restrict__ nend,
+                                                float* __restrict__ fx, float* __restrict__ fy,
+                                                float* __restrict__ fz, uint32_t& pairs,
+                                                uint32_t& ws, uint32_t lk) {
+    const uint32_t k = 1u << lk;
+    const uint32_t q = lane & (k - 1u);
+    const uint32_t i = c0 + (lane >> lk);
+    const bool valid = i < te;
+    const float4 p = valid ? pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    uint32_t wp = 0;
+    for (uint32_t k0 = 0; k0 < L; k0 += 64) {
+        unsigned long long near = ~masks[mo + (k0 >> 6)];
+        if (L - k0 < 64) near &= (1ull << (L - k0)) - 1ull;
+        while (near) {
+            const uint32_t b = (uint32_t)__ffsll((long long)near) - 1u;
+            near &= near - 1ull;
+            ++ws;
+            const uint32_t s = arena[o + k0 + b];
+            const uint32_t j0 = link[s].x;
+            const uint32_t j1e = nend[s];
+            const uint32_t j1 = j1e < (uint32_t)n ? j1e : (uint32_t)n;
+            wp += j1e - j0;
+            uint32_t j = j0 + q;
+            for (; j + 3u * k < j1; j += 4u * k) {  // four loads in flight per lane
+                const float4 q0 = pos4[j], q1 = pos4[j + k], q2 = pos4[j + 2u * k],
+                             q3 = pos4[j + 3u * k];
+                ps_pair(p, q0, ax, ay, az);
+                ps_pair(p, q1, ax, ay, az);
+                ps_pair(p, q2, ax, ay, az);
+                ps_pair(p, q3, ax, ay, az);
+            }
+            for (; j < j1; j += k) ps_pair(p, pos4[j], ax, ay, az);
+        }
+    }
+    for (uint32_t d = 1; d < k; d <<= 1) {
+        ax += __shfl_xor(ax, (int)d, 64);
+        ay += __shfl_xor(ay, (int)d, 64);
+        az += __shfl_xor(az, (int)d, 64);
+    }
+    if (valid && q == 0) {
+        pairs += wp;
+        p2p_store<kMode>(i, p, fl, ax, ay, az, fx, fy, fz);
+    }
+}
+
 // One wavefront per big leaf (>= tmin particles), 64 targets per
 // round: the whole wave shares the leaf's source list, so the loop runs on
 // scalar control flow and each source particle is one scalar load broadcast
@@ -386,7 +444,26 @@ __global__ __launch_bounds__(kBlock) void p2p_big_kernel(
         for (uint32_t c0 = t0; c0 < te;) {
             const uint32_t rem = te - c0;
             uint32_t ws = 0;
-            if (rem > 192) {
+            if (!kKahan && rem < 64 && rem <= 32) {
+                // the last targets, several lanes each
+                const uint32_t lk = 31u - (uint32_t)__clz(64u / rem);
+                p2p_split_round<kMode>(c0, te, lane, n, L, o, mo, fl, pos4, arena, masks, link,
+                                       nend, fx, fy, fz, pairs, ws, lk);
+                c0 = te;
+            } else if (!kKahan && rem >= 64 && rem % 64u != 0u && rem < 256) {
+                // whole rounds of 64 targets per lane group, the rest split
+                const uint32_t T = rem >> 6;
+                if (T >= 3)
+                    p2p_big_round<kMode, kKahan, 3>(c0, te, lane, n, L, o, mo, fl, pos4, arena,
+                                                    masks, link, nend, fx, fy, fz, pairs, ws);
+                else if (T == 2)
+                    p2p_big_round<kMode, kKahan, 2>(c0, te, lane, n, L, o, mo, fl, pos4, arena,
+                                                    masks, link, nend, fx, fy, fz, pairs, ws);
+                else
+                    p2p_big_round<kMode, kKahan, 1>(c0, te, lane, n, L, o, mo, fl, pos4, arena,
+                                                    masks, link, nend, fx, fy, fz, pairs, ws);
+                c0 += 64u * T;
+            } else if (rem > 192) {
                 p2p_big_round<kMode, kKahan, 4>(c0, te, lane, n, L, o, mo, fl, pos4, arena, masks,
                                                 link, nend, fx, fy, fz, pairs, ws);
                 c0 += 256;

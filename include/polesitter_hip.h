@@ -64,6 +64,10 @@ extern "C" {
                                           from the last tick's sorted keys in place, sort
                                           those and merge (the same permutation); 0: always
                                           the full radix sort */
+#define PS_HIP_OPT_TRAV_RECORDS 12 /* 1 (default): the traversal keeps each entry's near
+                                      sources in LDS between counting and writing;
+                                      0: every next list is written from its list, far
+                                      mask and links (the same words) */
 
 typedef struct ps_hip_engine ps_hip_engine_t;
 

@@ -35,6 +35,7 @@ OPT_ARENA_WORDS = 8
 OPT_TRAV_BLOCKS = 9
 OPT_TRAV_SEGMENTS = 10
 OPT_SORT_INCREMENTAL = 11
+OPT_TRAV_RECORDS = 12
 BATCH_OPT_GRAPH = 100
 
 

@@ -209,6 +209,7 @@ struct ps_hip_engine {
     int prev_cur = 0;        // prev_b[prev_cur] holds them
     size_t prev_n = 0;       // their count (0: none)
     int incremental = 1;     // PS_HIP_OPT_SORT_INCREMENTAL
+    int trav_records = 1;    // PS_HIP_OPT_TRAV_RECORDS
     // the incremental sort's copy back into the caller's arrays runs on s2;
     // skey is where the sorted keys are meanwhile (the tree reads them)
     bool copy_pending = false;
@@ -584,6 +585,10 @@ uint32_t entries_bound(const ps_hip_engine* e, bool mt, int k) {
 // 1024 threads, else 8.  The small top iterations are chains of memory round
 // trips: with wide groups they spread over the whole chip, and each lane has
 // fewer of the entry's sources to test and expand.
+// words of near records a traversal group may keep in LDS (0: none, every
+// continuing entry is written from its list)
+uint32_t rec_cap_of(const ps_hip_engine* e) { return e->trav_records ? 0xFFFFFFFFu : 0u; }
+
 int trav_lanes_for(const ps_hip_engine* e, uint32_t ub) {
     if (e->trav_lanes) return e->trav_lanes;
     const unsigned long long lanes = (unsigned long long)e->trav_blocks * 1024ull;
@@ -619,7 +624,7 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
         const int lanes = trav_lanes_for(e, ub);
         ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
                             e->entries(nxt), far_part + far_n, e->kahan, lanes,
-                            e->trav_blocks, e->trav_spb, e->s);
+                            e->trav_blocks, e->trav_spb, rec_cap_of(e), e->s);
         far_n += ps_trav_blocks(ub, lanes, e->trav_blocks);
     }
     if (e->trav_ev) (void)hipEventRecord(e->ev_it[1], e->s);
@@ -847,7 +852,7 @@ int async_back(ps_hip_engine* e, hipStream_t cs) {
         const int lanes = trav_lanes_for(e, ub);
         ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
                             e->entries(nxt), far_part + far_n, e->kahan, lanes, e->trav_blocks,
-                            e->trav_spb, cs);
+                            e->trav_spb, rec_cap_of(e), cs);
         far_n += ps_trav_blocks(ub, lanes, e->trav_blocks);
     }
     ps_reduce_u64(far_part, far_n, &st->m2l_pairs, cs);
@@ -1010,7 +1015,7 @@ int batch_run_back(ps_hip_batch* b, uint32_t n_sys) {
         const int lanes = trav_lanes_for(e, ub);
         ps_launch_trav_step(t, theta2, e->entries(cur), e->arena(), e->masks(), st, k, ub,
                             e->entries(nxt), far_part + far_n, e->kahan, lanes,
-                            e->trav_blocks, e->trav_spb, s);
+                            e->trav_blocks, e->trav_spb, rec_cap_of(e), s);
         far_n += ps_trav_blocks(ub, lanes, e->trav_blocks);
     }
     ps_reduce_u64(far_part, far_n, &st->m2l_pairs, s);
@@ -1240,6 +1245,10 @@ int ps_hip_engine_set_option(ps_hip_engine_t* e, int key, double value) {
             if (!(value >= 1.0 && value <= 64.0) || value != (double)(int)value)
                 return PS_HIP_EINVAL;
             e->trav_spb = (uint32_t)value;
+            return PS_HIP_OK;
+        case PS_HIP_OPT_TRAV_RECORDS:
+            if (value != 0.0 && value != 1.0) return PS_HIP_EINVAL;
+            e->trav_records = (int)value;
             return PS_HIP_OK;
         case PS_HIP_OPT_SORT_INCREMENTAL:
             if (value != 0.0 && value != 1.0) return PS_HIP_EINVAL;

@@ -204,7 +204,7 @@ uint32_t ps_trav_blocks(uint32_t max_ent, int lanes, uint32_t max_blocks);
 void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t* arena,
                          unsigned long long* masks, PsTravState* st, int k, uint32_t max_ent,
                          PsTravEntries next, unsigned long long* far_part, int kahan, int lanes,
-                         uint32_t max_blocks, uint32_t spb, hipStream_t s);
+                         uint32_t max_blocks, uint32_t spb, uint32_t rec_cap, hipStream_t s);
 
 // ---- ps_p2p.hip: near field + L2P ------------------------------------------
 // mode 0: f = m*field + m*sum (fused L2P+P2P); 1: f = m*field (L2P only);

@@ -460,7 +460,8 @@ template <bool kKahan, int G>
 __global__ __launch_bounds__(kStep) void trav_step_kernel(
     PsTravTree t, float theta2, PsTravEntries ent, uint32_t* __restrict__ arena,
     unsigned long long* __restrict__ masks, PsTravState* __restrict__ st, int it,
-    PsTravEntries next, unsigned long long* __restrict__ far_part, uint32_t spb) {
+    PsTravEntries next, unsigned long long* __restrict__ far_part, uint32_t spb,
+    uint32_t rec_cap) {
     constexpr uint32_t kGroups = kStep / G;
     __shared__ uint4 scnt[kSeg];  // sizes of the segment's entries, then their offsets
     __shared__ uint4 wtot[kStep / 64];
@@ -569,13 +570,15 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
         __syncthreads();
         // count
         constexpr uint32_t kPoolPer = kPoolWords / kGroups;
-        uint32_t bump = grp * kPoolPer;  // the group's pool space: [grp, grp + 1) * kPoolPer
+        // the group's pool space: kPoolPer words from grp * kPoolPer (rec_cap
+        // < kPoolPer only to test the write from the lists)
+        uint32_t bump = grp * kPoolPer;
+        const uint32_t pool_end = bump + (rec_cap < kPoolPer ? rec_cap : kPoolPer);
         for (uint32_t i = group_take<G>(&queue[0], gl); i < segn; i = group_take<G>(&queue[0], gl)) {
             const uint32_t k = sord[i];
             uint32_t piece = bump;
             const uint4 c = count_entry<kKahan, G>(t, theta2, ent, arena, masks, mask_cap, seg + k,
-                                                   gl, gshift, spool, bump, (grp + 1u) * kPoolPer,
-                                                   piece);
+                                                   gl, gshift, spool, bump, pool_end, piece);
             if (gl == 0) {
                 scnt[k] = c;
                 spc[k] = piece == kNoPiece ? kNoPiece : (piece | ((bump - piece) << 16));
@@ -703,11 +706,11 @@ uint32_t ps_trav_blocks(uint32_t max_ent, int lanes, uint32_t max_blocks) {
 void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t* arena,
                          unsigned long long* masks, PsTravState* st, int k, uint32_t max_ent,
                          PsTravEntries next, unsigned long long* far_part, int kahan, int lanes,
-                         uint32_t max_blocks, uint32_t spb, hipStream_t s) {
+                         uint32_t max_blocks, uint32_t spb, uint32_t rec_cap, hipStream_t s) {
     const uint32_t nb = ps_trav_blocks(max_ent, lanes, max_blocks);
 #define PS_STEP(KH, G)                                                                    \
     trav_step_kernel<KH, G><<<nb, kStep, 0, s>>>(t, theta2, ent, arena, masks, st, k, next, \
-                                                  far_part, spb)
+                                                  far_part, spb, rec_cap)
     switch (lane_group(lanes) + (kahan ? 1 : 0)) {
         case 8: PS_STEP(false, 8); break;
         case 9: PS_STEP(true, 8); break;

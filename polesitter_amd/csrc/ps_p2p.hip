@@ -348,7 +348,7 @@ __device__ __forceinline__ void p2p_big_round(uint32_t c0, uint32_t te, uint32_t
 // source particle from the q-th; a butterfly over the k lanes adds the
 // partial sums.  More lanes are busy than with one target per lane (r / 64
 // of them), at the price of a vector load per source step (k addresses).
-template <int kMode>
+template <int kMode, bool kKahan>
 __device__ __forceinline__ void p2p_split_round(uint32_t c0, uint32_t te, uint32_t lane, size_t n,
                                                 uint32_t L, uint32_t o, uint32_t mo, float4 fl,
                                                 const float4* __restrict__ pos4,
@@ -365,6 +365,7 @@ __device__ __forceinline__ void p2p_split_round(uint32_t c0, uint32_t te, uint32
     const bool valid = i < te;
     const float4 p = valid ? pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float ax = 0.f, ay = 0.f, az = 0.f;
+    float cx = 0.f, cy = 0.f, cz = 0.f;  // compensations (Kahan)
     uint32_t wp = 0;
     for (uint32_t k0 = 0; k0 < L; k0 += 64) {
         unsigned long long near = ~masks[mo + (k0 >> 6)];
@@ -382,18 +383,29 @@ __device__ __forceinline__ void p2p_split_round(uint32_t c0, uint32_t te, uint32
             for (; j + 3u * k < j1; j += 4u * k) {  // four loads in flight per lane
                 const float4 q0 = pos4[j], q1 = pos4[j + k], q2 = pos4[j + 2u * k],
                              q3 = pos4[j + 3u * k];
-                ps_pair(p, q0, ax, ay, az);
-                ps_pair(p, q1, ax, ay, az);
-                ps_pair(p, q2, ax, ay, az);
-                ps_pair(p, q3, ax, ay, az);
+                p2p_acc<kKahan>(p, q0, ax, ay, az, cx, cy, cz);
+                p2p_acc<kKahan>(p, q1, ax, ay, az, cx, cy, cz);
+                p2p_acc<kKahan>(p, q2, ax, ay, az, cx, cy, cz);
+                p2p_acc<kKahan>(p, q3, ax, ay, az, cx, cy, cz);
             }
-            for (; j < j1; j += k) ps_pair(p, pos4[j], ax, ay, az);
+            for (; j < j1; j += k) p2p_acc<kKahan>(p, pos4[j], ax, ay, az, cx, cy, cz);
         }
     }
     for (uint32_t d = 1; d < k; d <<= 1) {
-        ax += __shfl_xor(ax, (int)d, 64);
-        ay += __shfl_xor(ay, (int)d, 64);
-        az += __shfl_xor(az, (int)d, 64);
+        if (kKahan) {  // compensated sums merged with their compensations
+            ps_kahan_merge(ax, cx, __shfl_xor(ax, (int)d, 64), __shfl_xor(cx, (int)d, 64));
+            ps_kahan_merge(ay, cy, __shfl_xor(ay, (int)d, 64), __shfl_xor(cy, (int)d, 64));
+            ps_kahan_merge(az, cz, __shfl_xor(az, (int)d, 64), __shfl_xor(cz, (int)d, 64));
+        } else {
+            ax += __shfl_xor(ax, (int)d, 64);
+            ay += __shfl_xor(ay, (int)d, 64);
+            az += __shfl_xor(az, (int)d, 64);
+        }
+    }
+    if (kKahan) {
+        ax -= cx;
+        ay -= cy;
+        az -= cz;
     }
     if (valid && q == 0) {
         pairs += wp;
@@ -444,13 +456,13 @@ __global__ __launch_bounds__(kBlock) void p2p_big_kernel(
         for (uint32_t c0 = t0; c0 < te;) {
             const uint32_t rem = te - c0;
             uint32_t ws = 0;
-            if (!kKahan && rem < 64 && rem <= 32) {
+            if (rem <= 32) {
                 // the last targets, several lanes each
                 const uint32_t lk = 31u - (uint32_t)__clz(64u / rem);
-                p2p_split_round<kMode>(c0, te, lane, n, L, o, mo, fl, pos4, arena, masks, link,
-                                       nend, fx, fy, fz, pairs, ws, lk);
+                p2p_split_round<kMode, kKahan>(c0, te, lane, n, L, o, mo, fl, pos4, arena, masks,
+                                               link, nend, fx, fy, fz, pairs, ws, lk);
                 c0 = te;
-            } else if (!kKahan && rem >= 64 && rem % 64u != 0u && rem < 256) {
+            } else if (rem >= 64 && rem % 64u != 0u && rem < 256) {
                 // whole rounds of 64 targets per lane group, the rest split
                 const uint32_t T = rem >> 6;
                 if (T >= 3)

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void trav_init_kernel(const uint32_t* __res
         if (k == 0) {
             PsTravState z = {};
             z.n_ent[0] = n_e;
-            z.arena_ctr = n_sys;  // the root lists
+            z.arena_ctr = (n_sys + 3u) & ~3u;  // the root lists; every list starts at a multiple of 4
             z.mask_ctr = n_e;     // one mask word per entry
             z.arena_cap = arena_cap;
             z.mask_cap = mask_cap;
@@ -172,7 +172,7 @@ constexpr uint32_t kRecMask = (1u << kRecShift) - 1u;
 // An entry's records: (start << 16 | count) in its group's pool space, or
 // kNoPiece when they did not fit (the write phase then reads the list again).
 constexpr uint32_t kNoPiece = 0xFFFFFFFFu;
-constexpr uint32_t kPoolWords = 24576;  // the block's record pool (96 KiB)
+constexpr uint32_t kPoolWords = 24064;  // the block's record pool (94 KiB)
 
 // Node record i from a node array: the byte offset is a 32-bit product, so
 // the load takes the array's address from scalar registers and one 32-bit
@@ -417,34 +417,68 @@ __device__ __forceinline__ void write_entry(const PsTravTree& t, const PsTravEnt
     write_next<G>(next, tn, tl, out, p, gl);
 }
 
+// LDS stores of some lanes seen by later LDS loads of other lanes of the
+// same wavefront: its LDS operations complete in order, and the fences keep
+// the compiler from moving them across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Staging words of a group in the write phase: a round's records expand to
+// at most 4 G words, plus up to 3 carried over from the round before.
+template <int G>
+constexpr uint32_t stage_words() {
+    return 4u * G + 4u;
+}
+
 // Write of one continuing entry from its near records in LDS (count of
-// them: n), the same list words in the same order as write_entry.
+// them: n), the same list words in the same order as write_entry.  Rounds of
+// G / 2 records are expanded into the group's staging space stg and stored
+// as whole 16-byte pieces, one per lane: a list starts on a 4-word boundary
+// and its reservation is padded to a multiple of 4 words, so the last piece
+// may carry up to 3 stale words past the list's end, into that padding.
 template <int G>
 __device__ __forceinline__ void write_entry_rec(const PsTravTree& t, const PsTravEntries& ent,
                                                 uint32_t* __restrict__ arena,
                                                 const uint32_t* recs, uint32_t n, uint32_t e,
                                                 uint32_t out, uint4 p, const PsTravEntries& next,
-                                                uint32_t gl) {
+                                                uint32_t gl, uint32_t* stg) {
+    constexpr uint32_t H = G / 2;
     const uint32_t tn = ent.tgt[e];
     const uint2 tl = t.link[tn];
-    uint32_t base = p.x;
-    const uint32_t limit = base + out;
-    for (uint32_t r0 = 0; r0 < n; r0 += G) {
+    uint4* dst = reinterpret_cast<uint4*>(arena + p.x);  // p.x is a multiple of 4
+    const uint4* sq = reinterpret_cast<const uint4*>(stg);
+    const uint32_t nq = (out + 3u) >> 2;  // pieces of the padded list
+    uint32_t have = 0u;                   // staged words not stored yet (< 4 between rounds)
+    uint32_t done = 0u;                   // pieces stored
+    for (uint32_t r0 = 0; r0 < n; r0 += H) {
         const uint32_t i = r0 + gl;
-        const uint32_t rc = i < n ? recs[i] : 0u;
-        const uint32_t nout = i < n ? (rc >> kRecShift) + 1u : 0u;
+        const bool v = gl < H && i < n;
+        const uint32_t rc = v ? recs[i] : 0u;
+        const uint32_t nout = v ? (rc >> kRecShift) + 1u : 0u;
         uint32_t total;
         const uint32_t inc = group_scan_u32<G>(nout, gl, total);
-        const uint32_t w = base + inc - nout;
-        if (nout && w + nout <= limit) {
-            uint32_t* dst = arena + w;
-            const uint32_t v0 = rc & kRecMask;
+        const uint32_t w = have + inc - nout;
+        const uint32_t v0 = rc & kRecMask;
 #pragma unroll
-            for (uint32_t r = 0; r < 8; ++r)
-                if (r < nout) dst[r] = v0 + r;
-        }
-        base += total;
+        for (uint32_t r = 0; r < 8; ++r)
+            if (r < nout) stg[w + r] = v0 + r;
+        wave_lds_sync();
+        const uint32_t staged = have + total;
+        const uint32_t full = staged >> 2;  // whole pieces (at most G)
+        if (gl < full && done + gl < nq) dst[done + gl] = sq[gl];
+        const uint32_t tail = staged & 3u;
+        uint32_t tw = 0u;
+        if (gl < tail) tw = stg[(full << 2) + gl];
+        wave_lds_sync();
+        if (gl < tail) stg[gl] = tw;
+        wave_lds_sync();
+        have = tail;
+        done += full;
     }
+    if (have && gl == 0 && done < nq) dst[done] = sq[0];
     write_next<G>(next, tn, tl, out, p, gl);
 }
 
@@ -473,6 +507,7 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
     __shared__ uint32_t sseg[2];                 // the next segment: first entry, length
     __shared__ uint32_t spool[kPoolWords];       // near records, kPoolWords / kGroups per group
     __shared__ uint32_t spc[kSeg];               // per entry: its records (or kNoPiece)
+    __shared__ __align__(16) uint32_t sstg[kGroups * stage_words<G>()];  // write-phase staging
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t grp = tid / G;
     const uint32_t gl = tid & (G - 1);
@@ -593,7 +628,8 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
             const uint32_t j = j0 + tid;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (j < segn) v = scnt[j];
-            uint4 inc = v;
+            const uint32_t vx = (v.x + 3u) & ~3u;  // lists padded to whole 16-byte pieces
+            uint4 inc = make_uint4(vx, v.y, v.z, 0u);
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
                 const uint32_t ox = __shfl_up(inc.x, d, 64);
@@ -617,7 +653,7 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
                 all.x += tw.x; all.y += tw.y; all.z += tw.z;
             }
             if (j < segn)
-                scnt[j] = make_uint4(run.x + inc.x - v.x, run.y + inc.y - v.y,
+                scnt[j] = make_uint4(run.x + inc.x - vx, run.y + inc.y - v.y,
                                      run.z + inc.z - v.z, v.x);
             carry = all;
             __syncthreads();  // wtot is rewritten by the next round
@@ -662,7 +698,7 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
                 const uint32_t pc = spc[k];
                 if (pc != kNoPiece)
                     write_entry_rec<G>(t, ent, arena, spool + (pc & 0xFFFFu), pc >> 16, seg + k,
-                                       q.w, pl, next, gl);
+                                       q.w, pl, next, gl, sstg + grp * stage_words<G>());
                 else
                     write_entry<G>(t, ent, arena, masks, seg + k, q.w, pl, next, gl);
             }

@@ -216,7 +216,7 @@ struct ps_hip_engine {
     hipEvent_t ev_sorted = nullptr, ev_copied = nullptr;
     uint32_t* skey = nullptr;
     // nodes
-    DevBuf geom_b, mp_b, loc_b, link_b, nend_b, parent_b, p2p_off_b, p2p_len_b, p2p_moff_b;
+    DevBuf geom_b, tpk_b, mp_b, loc_b, link_b, nend_b, parent_b, p2p_off_b, p2p_len_b, p2p_moff_b;
     DevBuf ltot_b;  // summed locals of the leaves (the tick's L2L)
     DevBuf oct_b;   // each node's slot in its parent (tree write scratch)
     // traversal: entries (tgt, off, len, moff) ping-pong; the list arena and
@@ -269,6 +269,7 @@ struct ps_hip_engine {
     float4* pos4() { return reinterpret_cast<float4*>(pos4_b.p); }
     uint32_t* leaf_of() { return reinterpret_cast<uint32_t*>(leaf_of_b.p); }
     float4* geom() { return reinterpret_cast<float4*>(geom_b.p); }
+    float4* tpk() { return reinterpret_cast<float4*>(tpk_b.p); }
     float4* mp() { return reinterpret_cast<float4*>(mp_b.p); }
     float4* loc() { return reinterpret_cast<float4*>(loc_b.p); }
     float4* ltot() { return reinterpret_cast<float4*>(ltot_b.p); }
@@ -317,7 +318,7 @@ struct ps_hip_engine {
     void free_all() {
         for (auto& b : stage) release(b);
         for (auto& b : prev_b) release(b);
-        DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b,
+        DevBuf* all[] = {&sort_t,  &sort_tmp,  &tree_tmp,  &pos4_b,    &leaf_of_b, &geom_b, &tpk_b,
                          &mp_b,    &loc_b,     &link_b,    &nend_b,    &parent_b,  &p2p_off_b,
                          &p2p_len_b, &p2p_moff_b, &arena_b, &masks_b,  &far_b,  &ltot_b, &oct_b,
                          &small_b,   &block_pairs_b, &tstate_b, &big_b,  &mlp_b, &mlp_pk_b, &mlp_tmp};
@@ -371,7 +372,7 @@ void for_each_tick_buffer(ps_hip_engine* e, size_t P, size_t M, uint32_t arena_w
     f(e->leaf_of_b, 4 * P);
     f(e->prev_b[0], 4 * P);
     f(e->prev_b[1], 4 * P);
-    DevBuf* node16[] = {&e->geom_b, &e->mp_b, &e->loc_b, &e->ltot_b};
+    DevBuf* node16[] = {&e->geom_b, &e->tpk_b, &e->mp_b, &e->loc_b, &e->ltot_b};
     for (DevBuf* d : node16) f(*d, 16 * M);
     f(e->oct_b, M);
     f(e->link_b, 8 * M);
@@ -436,6 +437,7 @@ hipError_t alloc_fixed(ps_hip_engine* e) {
     PS_AL(prev_b[0], 4 * P);
     PS_AL(prev_b[1], 4 * P);
     PS_AL(geom_b, 16 * M);
+    PS_AL(tpk_b, 16 * M);
     PS_AL(mp_b, 16 * M);
     PS_AL(loc_b, 16 * M);
     PS_AL(ltot_b, 16 * M);
@@ -564,8 +566,8 @@ int stage_build(ps_hip_engine* e) {
 
 int stage_upward(ps_hip_engine* e) {
     for (int d = (int)e->max_depth; d >= 0; --d)
-        ps_launch_upward_level(e->ls[d], e->ls[d + 1], e->pos4(), e->geom(), e->link(), e->nend(),
-                               e->mp(), e->s);
+        ps_launch_upward_level(e->ls[d], e->ls[d + 1], d, e->tpk(), e->pos4(), e->geom(),
+                               e->link(), e->nend(), e->mp(), e->s);
     return map_err(hipGetLastError());
 }
 
@@ -608,6 +610,7 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
     PsTravTree t;
     t.n_nodes = n_nodes;
     t.geom = e->geom();
+    t.tpk = e->tpk();
     t.mp = e->mp();
     t.link = e->link();
     t.nend = e->nend();
@@ -783,7 +786,7 @@ int run_tick(ps_hip_engine* e) {
 int async_ensure(ps_hip_engine* e, size_t n) {
     hipStream_t s = e->s;
     const size_t M = worst_case_nodes(n);
-    DevBuf* node16[] = {&e->geom_b, &e->mp_b, &e->loc_b};
+    DevBuf* node16[] = {&e->geom_b, &e->tpk_b, &e->mp_b, &e->loc_b};
     for (DevBuf* d : node16) PS_CK(grow(*d, 16 * M, s));
     PS_CK(grow(e->link_b, 8 * M, s));
     PS_CK(grow(e->oct_b, M, s));
@@ -819,7 +822,7 @@ int async_front(ps_hip_engine* e, hipStream_t cs) {
                         e->totals(), e->leaf_cap, cs));
     ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
                        e->totals(), M, cs);
-    ps_launch_upward_dev(ls, e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), cs);
+    ps_launch_upward_dev(ls, e->tpk(), e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), cs);
     return map_err(hipGetLastError());
 }
 
@@ -838,6 +841,7 @@ int async_back(ps_hip_engine* e, hipStream_t cs) {
     PsTravTree t;
     t.n_nodes = ub;
     t.geom = e->geom();
+    t.tpk = e->tpk();
     t.mp = e->mp();
     t.link = e->link();
     t.nend = e->nend();
@@ -921,7 +925,7 @@ size_t forest_node_bound(const std::vector<uint32_t>& off) {
 int batch_ensure(ps_hip_batch* b, size_t n, size_t cap) {
     ps_hip_engine* e = b->e;
     hipStream_t s = e->s;
-    DevBuf* node16[] = {&e->geom_b, &e->mp_b, &e->loc_b};
+    DevBuf* node16[] = {&e->geom_b, &e->tpk_b, &e->mp_b, &e->loc_b};
     for (DevBuf* d : node16) PS_CK(grow(*d, 16 * cap, s));
     PS_CK(grow(e->link_b, 8 * cap, s));
     PS_CK(grow(e->oct_b, cap, s));
@@ -971,7 +975,7 @@ int batch_run_front(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
                         cap, e->totals(), e->leaf_cap, s, caps));
     ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
                        e->totals(), cap, s);
-    ps_launch_upward_dev(ls, e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), s);
+    ps_launch_upward_dev(ls, e->tpk(), e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), s);
     return map_err(hipGetLastError());
 }
 
@@ -1001,6 +1005,7 @@ int batch_run_back(ps_hip_batch* b, uint32_t n_sys) {
     PsTravTree t;
     t.n_nodes = ub;
     t.geom = e->geom();
+    t.tpk = e->tpk();
     t.mp = e->mp();
     t.link = e->link();
     t.nend = e->nend();

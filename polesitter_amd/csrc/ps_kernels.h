@@ -129,11 +129,13 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
                          size_t n_nodes, const unsigned long long* n_nodes_dev, int leaf_cap,
                          hipStream_t s, const uint8_t* sys_cap = nullptr);
 // Upward (depths 10..0) and L2L (depths 1..10) with the level starts read
-// from device memory ls[0..11]; fixed grids, no host-side sizes.
-void ps_launch_upward_dev(const uint32_t* ls, const float4* pos4, const float4* geom,
+// from device memory ls[0..11]; fixed grids, no host-side sizes.  The upward
+// pass also writes each node's traversal record tpk (centre; depth, child
+// slot mask and leaf flag in the fourth word's bits 0..3, 4..11 and 12).
+void ps_launch_upward_dev(const uint32_t* ls, float4* tpk, const float4* pos4, const float4* geom,
                           const uint2* link, const uint32_t* nend, float4* mp, hipStream_t s);
 void ps_launch_l2l_dev(const uint32_t* ls, const uint32_t* parent, float4* loc, hipStream_t s);
-void ps_launch_upward_level(uint32_t begin, uint32_t end, const float4* pos4,
+void ps_launch_upward_level(uint32_t begin, uint32_t end, int d, float4* tpk, const float4* pos4,
                             const float4* geom, const uint2* link, const uint32_t* nend,
                             float4* mp, hipStream_t s);
 void ps_launch_l2l_level(uint32_t begin, uint32_t end, const uint32_t* parent, float4* loc,
@@ -147,6 +149,7 @@ void ps_launch_leaf_locals(uint32_t n_nodes, const uint2* link, const uint32_t* 
 struct PsTravTree {
     uint32_t n_nodes;  // list entries are checked against it
     const float4* geom;
+    const float4* tpk;  // traversal node records (from the upward pass)
     const float4* mp;
     const uint2* link;
     const uint32_t* nend;

@@ -164,11 +164,14 @@ __global__ __launch_bounds__(kBlock) void trav_init_kernel(const uint32_t* __res
 __device__ __forceinline__ uint32_t chunks_of(uint32_t len) { return (len + 63u) >> 6; }
 
 // Near records: what a near source passes on to the next list, kept in LDS
-// between the count and the write phase -- the first id (bits 0..28) and the
-// count - 1 (bits 29..31) of consecutive ids (a leaf: itself; an internal
-// node: its children).  Node ids stay below PS_TRAV_MAX_NODES <= 2^29.
+// between the count and the write phase -- the source's id (bits 0..27), a
+// flag for an internal node (bit 28) and the count - 1 (bits 29..31) of the
+// consecutive ids it stands for (a leaf: itself; an internal node: its
+// children, from its first child, which the write phase looks up).  Node
+// ids stay below PS_TRAV_MAX_NODES <= 2^28.
 constexpr uint32_t kRecShift = 29;
-constexpr uint32_t kRecMask = (1u << kRecShift) - 1u;
+constexpr uint32_t kRecInternal = 1u << 28;
+constexpr uint32_t kRecMask = kRecInternal - 1u;
 // An entry's records: (start << 16 | count) in its group's pool space, or
 // kNoPiece when they did not fit (the write phase then reads the list again).
 constexpr uint32_t kNoPiece = 0xFFFFFFFFu;
@@ -210,6 +213,7 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
     const uint32_t mo = ent.moff[e];
     const float4 tg = t.geom[tn];
     const uint2 tl = t.link[tn];
+    const int dt = (int)(__float_as_uint(t.tpk[tn].w) & 0xFu);  // the target's depth
     float ex = 0.f, ey = 0.f, ez = 0.f;
     float cx = 0.f, cy = 0.f, cz = 0.f;
     uint32_t nfar = 0, nleaf = 0, nchild = 0;
@@ -227,16 +231,14 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
         const uint32_t* lp = arena + (size_t)(o + k0 + gl);
 #pragma unroll
         for (int j = 0; j < U; ++j) s[j] = need[j] ? lp[j * G] : tn;
-        float4 sg[U], smp[U];
-        uint2 sl[U];
+        float4 sg[U], smp[U];  // traversal record (centre, depth and flags), multipole
 #pragma unroll
         for (int j = 0; j < U; ++j) {
             if (!need[j]) continue;
             const bool ok = k0 + j * G + gl < L && s[j] < t.n_nodes;
             const uint32_t sv = ok ? s[j] : tn;
-            sg[j] = node_ld(t.geom, sv);
+            sg[j] = node_ld(t.tpk, sv);
             smp[j] = node_ld(t.mp, sv);
-            sl[j] = node_ld(t.link, sv);
         }
         unsigned long long word = 0;
         uint32_t nrec = 0;  // records of the chunk so far (uniform in the group)
@@ -247,8 +249,13 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
             // an id past the nodes is never written when counts and writes
             // agree; it is skipped (a far bit without M2L) by every consumer
             const bool ok = in && s[j] < t.n_nodes;
+            // the source's half width: the root's times 2^-depth, so the
+            // target's scaled by 2^(its depth - the source's), exactly
+            const uint32_t pw = __float_as_uint(sg[j].w);
+            const float4 sgj = make_float4(sg[j].x, sg[j].y, sg[j].z,
+                                           __builtin_amdgcn_ldexpf(tg.w, dt - (int)(pw & 0xFu)));
             float dx, dy, dz, d2;
-            const bool mac = ps_far_d(tg, sg[j], theta2, dx, dy, dz, d2);
+            const bool mac = ps_far_d(tg, sgj, theta2, dx, dy, dz, d2);
             const bool far = ok && mac;
             if (kKahan) {
                 if (far) {
@@ -262,8 +269,8 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
                 ps_m2l_dm(dx, dy, dz, d2, smp[j], far, ex, ey, ez);
             }
             const bool near = ok && !mac;
-            const bool leaf = (sl[j].y & PS_LEAF) != 0u;
-            const uint32_t nk = (uint32_t)__popc(sl[j].y & 0xFFu);
+            const bool leaf = (pw & (1u << 12)) != 0u;
+            const uint32_t nk = (uint32_t)__popc((pw >> 4) & 0xFFu);
             nchild += (near && !leaf) ? nk : 0u;
             nleaf += (near && leaf) ? 1u : 0u;  // this lane's; summed over the group below
             const unsigned long long fb = group_bits<G>(far || (in && !ok), gshift);
@@ -274,7 +281,7 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
             const unsigned long long nb = group_bits<G>(near, gshift);
             const uint32_t pos = bump + nrec + (uint32_t)__popcll(nb & ((1ull << gl) - 1ull));
             if (near && pos < pool_end)
-                pool[pos] = leaf ? s[j] : (sl[j].x | ((nk - 1u) << kRecShift));
+                pool[pos] = leaf ? s[j] : (s[j] | kRecInternal | ((nk - 1u) << kRecShift));
             nrec += (uint32_t)__popcll(nb);
         }
         if (gl == 0 && mo + (k0 >> 6) < mask_cap) masks[mo + (k0 >> 6)] = word;
@@ -458,10 +465,12 @@ __device__ __forceinline__ void write_entry_rec(const PsTravTree& t, const PsTra
         const bool v = gl < H && i < n;
         const uint32_t rc = v ? recs[i] : 0u;
         const uint32_t nout = v ? (rc >> kRecShift) + 1u : 0u;
+        const uint32_t id = rc & kRecMask;
+        // an internal node stands for its children, from its first child
+        const uint32_t v0 = (rc & kRecInternal) ? node_ld(t.link, id).x : id;
         uint32_t total;
         const uint32_t inc = group_scan_u32<G>(nout, gl, total);
         const uint32_t w = have + inc - nout;
-        const uint32_t v0 = rc & kRecMask;
 #pragma unroll
         for (uint32_t r = 0; r < 8; ++r)
             if (r < nout) stg[w + r] = v0 + r;

@@ -276,7 +276,15 @@ __device__ __forceinline__ void m2m_children(const float4* __restrict__ geom,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, uint32_t end,
+// The traversal's node record: the centre, then depth (bits 0..3), child
+// slot mask (4..11) and leaf flag (12) in the fourth word.
+__device__ __forceinline__ float4 trav_node(float4 g, uint2 l, int d) {
+    const uint32_t w = (uint32_t)d | ((l.y & 0xFFu) << 4) | ((l.y & PS_LEAF) ? (1u << 12) : 0u);
+    return make_float4(g.x, g.y, g.z, __uint_as_float(w));
+}
+
+__global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, uint32_t end, int d,
+                                                              float4* __restrict__ tpk,
                                                               const float4* __restrict__ pos4,
                                                               const float4* __restrict__ geom,
                                                               const uint2* __restrict__ link,
@@ -287,6 +295,7 @@ __global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, ui
     if (nidx >= end) return;
     const float4 g = geom[nidx];
     const uint2 l = link[nidx];
+    tpk[nidx] = trav_node(g, l, d);
     float m0 = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
     if (l.y & PS_LEAF) {  // P2M about the cell centre, H:978-1004
         p2m_leaf(pos4, l.x, nend[nidx], g, m0, mx, my, mz);
@@ -364,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void leaf_locals_kernel(uint32_t n_nodes,
 // (ls[d], ls[d+1]) and a fixed grid: nothing depends on host-side sizes, so
 // the launches can be recorded in a graph
 __global__ __launch_bounds__(kBlock) void upward_level_dev_kernel(
-    const uint32_t* __restrict__ ls, int d, const float4* __restrict__ pos4,
+    const uint32_t* __restrict__ ls, int d, float4* __restrict__ tpk, const float4* __restrict__ pos4,
     const float4* __restrict__ geom, const uint2* __restrict__ link,
     const uint32_t* __restrict__ nend, float4* __restrict__ mp) {
 #pragma clang fp contract(off)
@@ -373,6 +382,7 @@ __global__ __launch_bounds__(kBlock) void upward_level_dev_kernel(
          nidx += gridDim.x * kBlock) {
         const float4 g = geom[nidx];
         const uint2 l = link[nidx];
+        tpk[nidx] = trav_node(g, l, d);
         float m0 = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
         if (l.y & PS_LEAF) {
             p2m_leaf(pos4, l.x, nend[nidx], g, m0, mx, my, mz);
@@ -403,10 +413,11 @@ constexpr uint32_t kDevGrid = 2048;  // blocks of the device-range launches
 
 }  // namespace
 
-void ps_launch_upward_dev(const uint32_t* ls, const float4* pos4, const float4* geom,
+void ps_launch_upward_dev(const uint32_t* ls, float4* tpk, const float4* pos4, const float4* geom,
                           const uint2* link, const uint32_t* nend, float4* mp, hipStream_t s) {
     for (int d = PS_MAX_DEPTH; d >= 0; --d)
-        upward_level_dev_kernel<<<kDevGrid, kBlock, 0, s>>>(ls, d, pos4, geom, link, nend, mp);
+        upward_level_dev_kernel<<<kDevGrid, kBlock, 0, s>>>(ls, d, tpk, pos4, geom, link, nend,
+                                                            mp);
 }
 
 void ps_launch_l2l_dev(const uint32_t* ls, const uint32_t* parent, float4* loc, hipStream_t s) {
@@ -460,12 +471,12 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
     return hipGetLastError();
 }
 
-void ps_launch_upward_level(uint32_t begin, uint32_t end, const float4* pos4,
+void ps_launch_upward_level(uint32_t begin, uint32_t end, int d, float4* tpk, const float4* pos4,
                             const float4* geom, const uint2* link, const uint32_t* nend,
                             float4* mp, hipStream_t s) {
     if (end <= begin) return;
     uint32_t blocks = (end - begin + kBlock - 1) / kBlock;
-    upward_level_kernel<<<blocks, kBlock, 0, s>>>(begin, end, pos4, geom, link, nend, mp);
+    upward_level_kernel<<<blocks, kBlock, 0, s>>>(begin, end, d, tpk, pos4, geom, link, nend, mp);
 }
 
 void ps_launch_leaf_locals(uint32_t n_nodes, const uint2* link, const uint32_t* parent,

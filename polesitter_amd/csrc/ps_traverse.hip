@@ -46,7 +46,7 @@ namespace {
 constexpr int kBlock = 256;            // init kernel
 constexpr int kStep = 1024;            // threads of an iteration block
 constexpr uint32_t kSeg = 2048;        // entries per LDS segment (one reservation each)
-constexpr uint32_t kBins = 4;          // list-length classes of the segment order
+constexpr uint32_t kOrdBins = 32;      // list-length bins of the segment order
 
 // Segment length of an iteration of n_e entries over nblk blocks: about spb
 // segments per block, a multiple of the groups per block up to kSeg (small
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
     __shared__ uint32_t sbase[4];  // list, entry and mask bases; 1 = reserved
     __shared__ uint32_t queue[2];  // work queues of the count and write phases
     __shared__ unsigned long long sfar[kGroups];
-    __shared__ uint32_t sbin[kBins][kSeg / 64];  // per 64 entries: counts, then offsets
+    __shared__ uint32_t shist[kOrdBins];         // entries per length bin, then bin offsets
     __shared__ uint16_t sord[kSeg];              // the segment's entries, longest lists first
     __shared__ uint32_t sseg[2];                 // the next segment: first entry, length
     __shared__ uint32_t spool[kPoolWords];       // near records, kPoolWords / kGroups per group
@@ -566,52 +566,43 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
         queue[0] = 0u;
         queue[1] = 0u;
     }
+    if (tid < kOrdBins) shist[tid] = 0u;
     __syncthreads();
     for (;;) {
         const uint32_t seg = sseg[0], segn = sseg[1];
         if (seg == PS_NONE) break;  // uniform per block
         if (tid == 0) queue[1] = 0u;  // the write phase of the last segment is over
-        // order: a group's work is the chunk count of its entry's list, and a
-        // wavefront runs as many chunk rounds as the longest list among its
-        // groups, so the queues hand out the entries by class of chunk count
-        // (3+, 2, 1, 0), each class in entry order.  Which group takes an
-        // entry changes nothing in what the entry computes.
-        uint32_t crk[kSeg / kStep];  // class << 16 | rank among the class in its 64 entries
+        // order: a wavefront runs as many slot rows (G sources each, one per
+        // lane of a group) as the longest list among its groups needs, so
+        // the queues hand out the entries longest first, in bins of G
+        // sources (the first bin: longer still).  Which group takes an
+        // entry, and in which order a bin's entries come, changes nothing in
+        // what an entry computes.
+        uint32_t rk[kSeg / kStep];  // bin << 16 | rank in the bin
 #pragma unroll
         for (uint32_t r = 0; r < kSeg / kStep; ++r) {
             const uint32_t j = r * kStep + tid;
-            uint32_t c = kBins;  // past the segment: no class
+            rk[r] = PS_NONE;
             if (j < segn) {
-                const uint32_t ch = (ent.len[seg + j] + 63u) >> 6;
-                c = ch < kBins - 1 ? kBins - 1 - ch : 0u;
-            }
-            crk[r] = c << 16;
-#pragma unroll
-            for (uint32_t b = 0; b < kBins; ++b) {
-                const uint64_t m = __ballot(c == b);
-                if (c == b) crk[r] |= (uint32_t)__popcll(m & ps_lanemask_lt());
-                if (lane == 0) sbin[b][j >> 6] = (uint32_t)__popcll(m);
+                const uint32_t rows = (ent.len[seg + j] + (uint32_t)G - 1u) / (uint32_t)G;
+                const uint32_t b = kOrdBins - 1u - (rows < kOrdBins - 1u ? rows : kOrdBins - 1u);
+                rk[r] = (b << 16) | atomicAdd(&shist[b], 1u);
             }
         }
         __syncthreads();
-        if (wave == 0) {  // exclusive prefix over (class, 64-entry block), class major
-            constexpr uint32_t kW = kSeg / 64;
-            static_assert(kBins * kW == 128, "two counts per lane");
-            const uint32_t a0 = sbin[(2 * lane) / kW][(2 * lane) % kW];
-            const uint32_t a1 = sbin[(2 * lane + 1) / kW][(2 * lane + 1) % kW];
+        if (wave == 0) {  // exclusive prefix over the bins
+            const uint32_t v = lane < kOrdBins ? shist[lane] : 0u;
             uint32_t tot;
-            const uint32_t ex = ps_wave_excl_scan(a0 + a1, &tot);
-            sbin[(2 * lane) / kW][(2 * lane) % kW] = ex;
-            sbin[(2 * lane + 1) / kW][(2 * lane + 1) % kW] = ex + a0;
+            const uint32_t ex = ps_wave_excl_scan(v, &tot);
+            if (lane < kOrdBins) shist[lane] = ex;
         }
         __syncthreads();
 #pragma unroll
-        for (uint32_t r = 0; r < kSeg / kStep; ++r) {
-            const uint32_t j = r * kStep + tid;
-            const uint32_t c = crk[r] >> 16;
-            if (c < kBins) sord[sbin[c][j >> 6] + (crk[r] & 0xFFFFu)] = (uint16_t)j;
-        }
+        for (uint32_t r = 0; r < kSeg / kStep; ++r)
+            if (rk[r] != PS_NONE)
+                sord[shist[rk[r] >> 16] + (rk[r] & 0xFFFFu)] = (uint16_t)(r * kStep + tid);
         __syncthreads();
+        if (tid < kOrdBins) shist[tid] = 0u;  // for the next segment (barriers before its use)
         // count
         constexpr uint32_t kPoolPer = kPoolWords / kGroups;
         // the group's pool space: kPoolPer words from grp * kPoolPer (rec_cap

@@ -169,6 +169,28 @@ __device__ __forceinline__ void ps_kahan_add(float& sum, float& comp, float term
     sum = t;
 }
 
+// ---------------------------------------------------------------------------
+// The traversal's node record (written by the upward pass): the centre, and
+// in the fourth word
+//   bits 0..4  -depth, 5-bit two's complement (the half width is the root's
+//              times 2^-depth, exactly: each level halves it)
+//   bits 5..8  the node's list-record bits: internal flag, id count - 1 (a
+//              leaf stands for itself, an internal node for its children)
+//   bits 9..   what the node adds to a target's next list when near: its id
+//              count | internal << 16
+// ---------------------------------------------------------------------------
+constexpr uint32_t PS_TNODE_OUT_SHIFT = 9;
+__device__ __forceinline__ uint32_t ps_tnode_word(int depth, bool leaf, uint32_t nk) {
+    const uint32_t intl = leaf ? 0u : 1u;
+    const uint32_t cnt = leaf ? 1u : nk;
+    return ((uint32_t)(-depth) & 0x1Fu) | (intl << 5) | ((cnt - 1u) << 6) |
+           ((cnt | (intl << 16)) << PS_TNODE_OUT_SHIFT);
+}
+// -depth
+__device__ __forceinline__ int ps_tnode_nexp(uint32_t w) { return ((int)(w << 27)) >> 27; }
+// bits 28..31 of a list record: internal flag, id count - 1
+__device__ __forceinline__ uint32_t ps_tnode_rec_bits(uint32_t w) { return (w & 0x1E0u) << 23; }
+
 __device__ __forceinline__ uint32_t ps_child_count(uint2 link) {
     return (link.y & PS_LEAF) ? 0u : (uint32_t)__popc(link.y & 0xFFu);
 }

@@ -130,8 +130,7 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
                          hipStream_t s, const uint8_t* sys_cap = nullptr);
 // Upward (depths 10..0) and L2L (depths 1..10) with the level starts read
 // from device memory ls[0..11]; fixed grids, no host-side sizes.  The upward
-// pass also writes each node's traversal record tpk (centre; depth, child
-// slot mask and leaf flag in the fourth word's bits 0..3, 4..11 and 12).
+// pass also writes each node's traversal record tpk (ps_tnode_word).
 void ps_launch_upward_dev(const uint32_t* ls, float4* tpk, const float4* pos4, const float4* geom,
                           const uint2* link, const uint32_t* nend, float4* mp, hipStream_t s);
 void ps_launch_l2l_dev(const uint32_t* ls, const uint32_t* parent, float4* loc, hipStream_t s);

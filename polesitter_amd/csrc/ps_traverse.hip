@@ -169,6 +169,9 @@ __device__ __forceinline__ uint32_t chunks_of(uint32_t len) { return (len + 63u)
 // consecutive ids it stands for (a leaf: itself; an internal node: its
 // children, from its first child, which the write phase looks up).  Node
 // ids stay below PS_TRAV_MAX_NODES <= 2^28.
+// s_waitcnt immediate for vmcnt(0) alone (gfx9 fields: vmcnt 15:14 and 3:0,
+// expcnt 6:4, lgkmcnt 11:8 -- the last two at their no-wait maximum)
+constexpr int kWaitVmem0 = 0x0F70;
 constexpr uint32_t kRecShift = 29;
 constexpr uint32_t kRecInternal = 1u << 28;
 constexpr uint32_t kRecMask = kRecInternal - 1u;
@@ -213,30 +216,36 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
     const uint32_t mo = ent.moff[e];
     const float4 tg = t.geom[tn];
     const uint2 tl = t.link[tn];
-    const int dt = (int)(__float_as_uint(t.tpk[tn].w) & 0xFu);  // the target's depth
+    // the root's half width: the target's times 2^depth, exactly
+    const float root_hw = __builtin_amdgcn_ldexpf(tg.w, -ps_tnode_nexp(__float_as_uint(t.tpk[tn].w)));
     float ex = 0.f, ey = 0.f, ez = 0.f;
     float cx = 0.f, cy = 0.f, cz = 0.f;
-    uint32_t nfar = 0, nleaf = 0, nchild = 0;
+    uint32_t nfar = 0;
+    uint32_t acc = 0;  // this lane's near sources: output ids | near internal nodes << 16
     for (uint32_t k0 = 0; k0 < L; k0 += 64) {
-        // Every slot loads: past the list's end the word comes from the
+        // slot j holds list entry k0 + j G + gl: in the list iff j G < rem
+        const int rem = (int)(L - k0) - (int)gl;
+        // slot j runs when some lane of the wavefront has a source in it
+        // (a scalar test: the other groups' lists may be longer).  Every
+        // running slot loads: past the list's end the word comes from the
         // next list or the arena's padding (PS_LIST_PAD words past its
         // capacity) and the slot then evaluates the target itself, which is
         // discarded -- no branches around the loads.
-        // slot j runs when some lane of the wavefront has a source in it
-        // (a scalar branch: the other groups' lists may be longer)
         bool need[U];
 #pragma unroll
-        for (int j = 0; j < U; ++j) need[j] = __ballot(k0 + j * G + gl < L) != 0ull;
+        for (int j = 0; j < U; ++j) need[j] = __ballot(j * G < rem) != 0ull;
         uint32_t s[U];
         const uint32_t* lp = arena + (size_t)(o + k0 + gl);
 #pragma unroll
         for (int j = 0; j < U; ++j) s[j] = need[j] ? lp[j * G] : tn;
+        // every list word is in before the node loads start (one wait here
+        // instead of one per slot, ahead of each slot's loads)
+        __builtin_amdgcn_s_waitcnt(kWaitVmem0);
         float4 sg[U], smp[U];  // traversal record (centre, depth and flags), multipole
 #pragma unroll
         for (int j = 0; j < U; ++j) {
             if (!need[j]) continue;
-            const bool ok = k0 + j * G + gl < L && s[j] < t.n_nodes;
-            const uint32_t sv = ok ? s[j] : tn;
+            const uint32_t sv = j * G < rem ? s[j] : tn;
             sg[j] = node_ld(t.tpk, sv);
             smp[j] = node_ld(t.mp, sv);
         }
@@ -245,18 +254,14 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
 #pragma unroll
         for (int j = 0; j < U; ++j) {
             if (!need[j]) continue;
-            const bool in = k0 + j * G + gl < L;
-            // an id past the nodes is never written when counts and writes
-            // agree; it is skipped (a far bit without M2L) by every consumer
-            const bool ok = in && s[j] < t.n_nodes;
-            // the source's half width: the root's times 2^-depth, so the
-            // target's scaled by 2^(its depth - the source's), exactly
+            const bool in = j * G < rem;
+            // the source's half width: the root's times 2^-depth, exactly
             const uint32_t pw = __float_as_uint(sg[j].w);
             const float4 sgj = make_float4(sg[j].x, sg[j].y, sg[j].z,
-                                           __builtin_amdgcn_ldexpf(tg.w, dt - (int)(pw & 0xFu)));
+                                           __builtin_amdgcn_ldexpf(root_hw, ps_tnode_nexp(pw)));
             float dx, dy, dz, d2;
             const bool mac = ps_far_d(tg, sgj, theta2, dx, dy, dz, d2);
-            const bool far = ok && mac;
+            const bool far = in && mac;
             if (kKahan) {
                 if (far) {
                     float px = 0.f, py = 0.f, pz = 0.f;
@@ -268,20 +273,15 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
             } else {  // no branch: most slots are far
                 ps_m2l_dm(dx, dy, dz, d2, smp[j], far, ex, ey, ez);
             }
-            const bool near = ok && !mac;
-            const bool leaf = (pw & (1u << 12)) != 0u;
-            const uint32_t nk = (uint32_t)__popc((pw >> 4) & 0xFFu);
-            nchild += (near && !leaf) ? nk : 0u;
-            nleaf += (near && leaf) ? 1u : 0u;  // this lane's; summed over the group below
-            const unsigned long long fb = group_bits<G>(far || (in && !ok), gshift);
-            word |= fb << (j * G);
+            const bool near = in && !mac;
+            acc += near ? (pw >> PS_TNODE_OUT_SHIFT) : 0u;  // summed over the group below
+            word |= group_bits<G>(far, gshift) << (j * G);
             // the record of a near source, in list order after the chunk's
             // earlier ones: a near leaf passes on itself, a near internal
-            // node its nk children (consecutive ids from its first child)
+            // node its children (consecutive ids from its first child)
             const unsigned long long nb = group_bits<G>(near, gshift);
             const uint32_t pos = bump + nrec + (uint32_t)__popcll(nb & ((1ull << gl) - 1ull));
-            if (near && pos < pool_end)
-                pool[pos] = leaf ? s[j] : (s[j] | kRecInternal | ((nk - 1u) << kRecShift));
+            if (near && pos < pool_end) pool[pos] = s[j] | ps_tnode_rec_bits(pw);
             nrec += (uint32_t)__popcll(nb);
         }
         if (gl == 0 && mo + (k0 >> 6) < mask_cap) masks[mo + (k0 >> 6)] = word;
@@ -295,8 +295,9 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
             }
         }
     }
-    nchild = group_sum_u32<G>(nchild);
-    nleaf = group_sum_u32<G>(nleaf);
+    acc = group_sum_u32<G>(acc);
+    const uint32_t out = acc & 0xFFFFu;  // next list words
+    const uint32_t nint = acc >> 16;      // near internal sources
     if (kKahan) {
 #pragma unroll
         for (int d = G / 2; d > 0; d >>= 1) {
@@ -312,12 +313,11 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
         ey = group_sum<G>(ey);
         ez = group_sum<G>(ez);
     }
-    const uint32_t out = nleaf + nchild;
     uint4 c = make_uint4(0u, 0u, 0u, nfar);
     if (tl.y & PS_LEAF) {
-        if (nchild == 0) {  // final: the near entries of this list are the P2P sources
+        if (nint == 0) {  // final: the near entries of this list are the P2P sources
             if (piece != kNoPiece) bump = piece;  // its records are not needed
-            if (nleaf && gl == 0) {
+            if (out && gl == 0) {
                 t.p2p_off[tn] = o;
                 t.p2p_len[tn] = L;
                 t.p2p_moff[tn] = mo;

@@ -276,10 +276,9 @@ __device__ __forceinline__ void m2m_children(const float4* __restrict__ geom,
     }
 }
 
-// The traversal's node record: the centre, then depth (bits 0..3), child
-// slot mask (4..11) and leaf flag (12) in the fourth word.
+// the traversal's node record (ps_tnode_word)
 __device__ __forceinline__ float4 trav_node(float4 g, uint2 l, int d) {
-    const uint32_t w = (uint32_t)d | ((l.y & 0xFFu) << 4) | ((l.y & PS_LEAF) ? (1u << 12) : 0u);
+    const uint32_t w = ps_tnode_word(d, (l.y & PS_LEAF) != 0u, ps_child_count(l));
     return make_float4(g.x, g.y, g.z, __uint_as_float(w));
 }
 

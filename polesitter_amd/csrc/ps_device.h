@@ -275,6 +275,12 @@ __device__ __forceinline__ unsigned long long group_bits(bool pred, uint32_t gsh
     const unsigned long long b = __ballot(pred) >> gshift;
     return G == 64 ? b : (b & ((1ull << G) - 1ull));
 }
+// the same from a wavefront mask already at hand
+template <int G>
+__device__ __forceinline__ unsigned long long group_bits_of(unsigned long long m, uint32_t gshift) {
+    const unsigned long long b = m >> gshift;
+    return G == 64 ? b : (b & ((1ull << G) - 1ull));
+}
 
 template <int G>
 __device__ __forceinline__ uint32_t group_sum_u32(uint32_t v) {

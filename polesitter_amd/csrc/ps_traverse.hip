@@ -232,8 +232,12 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
         // capacity) and the slot then evaluates the target itself, which is
         // discarded -- no branches around the loads.
         bool need[U];
+        unsigned long long inm[U];  // lanes with slot j in their list
 #pragma unroll
-        for (int j = 0; j < U; ++j) need[j] = __ballot(j * G < rem) != 0ull;
+        for (int j = 0; j < U; ++j) {
+            inm[j] = __ballot(j * G < rem);
+            need[j] = inm[j] != 0ull;
+        }
         uint32_t s[U];
         const uint32_t* lp = arena + (size_t)(o + k0 + gl);
 #pragma unroll
@@ -262,6 +266,10 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
             float dx, dy, dz, d2;
             const bool mac = ps_far_d(tg, sgj, theta2, dx, dy, dz, d2);
             const bool far = in && mac;
+            // the wavefront's far and near masks straight from the compares
+            // (a ballot of the combined flags would rebuild them lane by lane)
+            const unsigned long long mmac = __ballot(mac);
+            const unsigned long long mfar = inm[j] & mmac, mnear = inm[j] & ~mmac;
             if (kKahan) {
                 if (far) {
                     float px = 0.f, py = 0.f, pz = 0.f;
@@ -275,11 +283,11 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
             }
             const bool near = in && !mac;
             acc += near ? (pw >> PS_TNODE_OUT_SHIFT) : 0u;  // summed over the group below
-            word |= group_bits<G>(far, gshift) << (j * G);
+            word |= group_bits_of<G>(mfar, gshift) << (j * G);
             // the record of a near source, in list order after the chunk's
             // earlier ones: a near leaf passes on itself, a near internal
             // node its children (consecutive ids from its first child)
-            const unsigned long long nb = group_bits<G>(near, gshift);
+            const unsigned long long nb = group_bits_of<G>(mnear, gshift);
             const uint32_t pos = bump + nrec + (uint32_t)__popcll(nb & ((1ull << gl) - 1ull));
             if (near && pos < pool_end) pool[pos] = s[j] | ps_tnode_rec_bits(pw);
             nrec += (uint32_t)__popcll(nb);

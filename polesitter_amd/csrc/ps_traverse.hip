@@ -189,31 +189,29 @@ __device__ __forceinline__ T node_ld(const T* __restrict__ base, uint32_t i) {
                                        (size_t)(i * (uint32_t)sizeof(T)));
 }
 
-// link[i].y (child mask and leaf flag) alone
-__device__ __forceinline__ uint32_t node_flags(const uint2* __restrict__ link, uint32_t i) {
-    return node_ld(reinterpret_cast<const uint32_t*>(link), 2u * i + 1u);
-}
-
-
 // Count of one entry by its group of G lanes (8, 16, 32 or 64).  The list is
 // taken in chunks of 64 sources, one far-mask word each; lane gl holds the
 // U = 64 / G sources k0 + j G + gl, whose list words and then node records
-// (geometry, multipole, link) are all requested before any is used, so a
-// chunk costs two dependent memory round trips.  Returns (list words, next
-// entries, mask words, far pairs) of the entry's output, valid in every lane.
+// (traversal record and multipole) are all requested before any is used, so a
+// chunk costs two dependent memory round trips.  me = the entry (target,
+// list offset, length, mask offset), staged in LDS by the segment's order
+// pass.  Returns (list words, next entries, mask words, far pairs) of the
+// entry's output, valid in every lane.
 template <bool kKahan, int G>
-__device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
-                                             const PsTravEntries& ent,
+__device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2, uint4 me,
                                              const uint32_t* __restrict__ arena,
                                              unsigned long long* __restrict__ masks,
-                                             uint32_t mask_cap, uint32_t e, uint32_t gl,
+                                             uint32_t mask_cap, uint32_t gl,
                                              uint32_t gshift, uint32_t* pool, uint32_t& bump,
                                              uint32_t pool_end, uint32_t& piece) {
     constexpr int U = 64 / G;
-    const uint32_t tn = ent.tgt[e];
-    const uint32_t o = ent.off[e];
-    const uint32_t L = ent.len[e];
-    const uint32_t mo = ent.moff[e];
+    const uint32_t tn = me.x;
+    const uint32_t o = me.y;
+    const uint32_t L = me.z;
+    const uint32_t mo = me.w;
+    // the target's local, read now (it is only updated at the end): its load
+    // overlaps the list's instead of following the sums
+
     const float4 tg = t.geom[tn];
     const uint2 tl = t.link[tn];
     // the root's half width: the target's times 2^depth, exactly
@@ -236,7 +234,9 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
 #pragma unroll
         for (int j = 0; j < U; ++j) {
             inm[j] = __ballot(j * G < rem);
-            need[j] = inm[j] != 0ull;
+            // slot 0 always runs (a group still in the loop has a source in
+            // it): its loads are then issued first and waited for alone
+            need[j] = j == 0 || inm[j] != 0ull;
         }
         uint32_t s[U];
         const uint32_t* lp = arena + (size_t)(o + k0 + gl);
@@ -338,11 +338,11 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2,
         c = make_uint4(out, nk, nk * chunks_of(out), nfar);
     }
     if (gl == 0 && nfar) {
-        float4 l = t.loc[tn];
-        l.y += ex;
-        l.z += ey;
-        l.w += ez;
-        t.loc[tn] = l;
+        float* lp3 = reinterpret_cast<float*>(t.loc + tn) + 1;  // (Ex, Ey, Ez)
+        const float3 lt = make_float3(lp3[0], lp3[1], lp3[2]);
+        lp3[0] = lt.x + ex;
+        lp3[1] = lt.y + ey;
+        lp3[2] = lt.z + ez;
     }
     return c;
 }
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
     PsTravEntries next, unsigned long long* __restrict__ far_part, uint32_t spb,
     uint32_t rec_cap) {
     constexpr uint32_t kGroups = kStep / G;
-    __shared__ uint4 scnt[kSeg];  // sizes of the segment's entries, then their offsets
+    __shared__ uint4 scnt[kSeg];  // the segment's entries, then their sizes, then their offsets
     __shared__ uint4 wtot[kStep / 64];
     __shared__ uint32_t sbase[4];  // list, entry and mask bases; 1 = reserved
     __shared__ uint32_t queue[2];  // work queues of the count and write phases
@@ -592,7 +592,12 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
             const uint32_t j = r * kStep + tid;
             rk[r] = PS_NONE;
             if (j < segn) {
-                const uint32_t rows = (ent.len[seg + j] + (uint32_t)G - 1u) / (uint32_t)G;
+                // the entry itself goes to LDS for the count phase (coalesced
+                // here, where the count phase would gather it per entry)
+                const uint4 me = make_uint4(ent.tgt[seg + j], ent.off[seg + j], ent.len[seg + j],
+                                            ent.moff[seg + j]);
+                scnt[j] = me;
+                const uint32_t rows = (me.z + (uint32_t)G - 1u) / (uint32_t)G;
                 const uint32_t b = kOrdBins - 1u - (rows < kOrdBins - 1u ? rows : kOrdBins - 1u);
                 rk[r] = (b << 16) | atomicAdd(&shist[b], 1u);
             }
@@ -620,8 +625,8 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
         for (uint32_t i = group_take<G>(&queue[0], gl); i < segn; i = group_take<G>(&queue[0], gl)) {
             const uint32_t k = sord[i];
             uint32_t piece = bump;
-            const uint4 c = count_entry<kKahan, G>(t, theta2, ent, arena, masks, mask_cap, seg + k,
-                                                   gl, gshift, spool, bump, pool_end, piece);
+            const uint4 c = count_entry<kKahan, G>(t, theta2, scnt[k], arena, masks, mask_cap, gl,
+                                                   gshift, spool, bump, pool_end, piece);
             if (gl == 0) {
                 scnt[k] = c;
                 spc[k] = piece == kNoPiece ? kNoPiece : (piece | ((bump - piece) << 16));

@@ -174,6 +174,11 @@ struct ps_hip_engine {
     hipStream_t s = nullptr;
     hipStream_t s2 = nullptr;  // second stream: the big-leaf P2P kernel beside the other one
     hipEvent_t fork = nullptr, join = nullptr;
+    // the second stream's side work of a tick: the big-leaf list once the
+    // tree is written (ev_tree), the traversal's pair count and state once it
+    // is done (ev_trav); joined before the tick's last event (ev_side)
+    hipEvent_t ev_tree = nullptr, ev_trav = nullptr, ev_side = nullptr;
+    bool collected = false;  // this tick's big-leaf list is built (on s2)
     size_t max_p = 0;
     size_t node_cap = 0;
     float theta = 1.0f;
@@ -345,6 +350,10 @@ struct ps_hip_engine {
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
         fork = join = nullptr;
+        for (hipEvent_t* v : {&ev_tree, &ev_trav, &ev_side}) {
+            if (*v) (void)hipEventDestroy(*v);
+            *v = nullptr;
+        }
         if (s2) (void)hipStreamDestroy(s2);
         s2 = nullptr;
         if (s) (void)hipStreamDestroy(s);
@@ -549,6 +558,7 @@ int stage_build(ps_hip_engine* e) {
     ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
                        e->totals(), e->node_cap, e->s);
     e->l2l_pending = false;
+    e->collected = false;  // a new tree: its big-leaf list is still to be built
     PS_CK(wait_event(e->ev_counts));
     const uint32_t flags = e->h_small[12];
     const unsigned long long total =
@@ -631,7 +641,11 @@ int launch_traverse(ps_hip_engine* e, float theta2, bool mt, int iters) {
         far_n += ps_trav_blocks(ub, lanes, e->trav_blocks);
     }
     if (e->trav_ev) (void)hipEventRecord(e->ev_it[1], e->s);
-    ps_reduce_u64(far_part, far_n, &st->m2l_pairs, e->s);
+    // the pair count and the state's copy to the host run beside the rest
+    // of the tick (joined before its end)
+    PS_CK(hipEventRecord(e->ev_trav, e->s));
+    PS_CK(hipStreamWaitEvent(e->s2, e->ev_trav, 0));
+    ps_reduce_u64(far_part, far_n, &st->m2l_pairs, e->s2);
     return map_err(hipGetLastError());
 }
 
@@ -659,7 +673,7 @@ int stage_traverse(ps_hip_engine* e) {
     PS_CK(grow(e->masks_b, 8 * std::max<size_t>(4 * M + n0 + 4096, n0), e->s));
     PS_RC(launch_traverse(e, theta2, mt, iters));
     PS_CK(hipMemcpyAsync(e->h_tstate, e->tstate_b.p, sizeof(PsTravState), hipMemcpyDeviceToHost,
-                         e->s));
+                         e->s2));
     return PS_HIP_OK;
 }
 
@@ -729,8 +743,30 @@ int stage_p2p(ps_hip_engine* e, int mode, int ev_before = -1, const float4* leaf
                   e->fx, e->fy, e->fz, mode, e->kahan, e->small_u32(), bound, e->p2p_big,
                   reinterpret_cast<uint32_t*>(e->big_b.p),
                   reinterpret_cast<unsigned long long*>(e->block_pairs_b.p), pc, e->s, e->s2,
-                  e->fork, e->join);
+                  e->fork, e->join, e->collected);
     return map_err(hipGetLastError());
+}
+
+// Later work on the engine stream waits for the second stream's side work
+// (the traversal's pair count and state copy, the big-leaf list).
+hipError_t join_side(ps_hip_engine* e) {
+    hipError_t r = hipEventRecord(e->ev_side, e->s2);
+    return r == hipSuccess ? hipStreamWaitEvent(e->s, e->ev_side, 0) : r;
+}
+
+// The big-leaf list on the second stream, as soon as the tree is written
+// (the P2P stage forks its big-leaf kernel onto the same stream after it).
+int stage_collect(ps_hip_engine* e) {
+    e->collected = false;
+    if (!e->p2p_big || !e->n_nodes) return PS_HIP_OK;
+    PS_CK(grow(e->big_b, 4 * ((size_t)e->n_nodes + 2), e->s));
+    PS_CK(hipEventRecord(e->ev_tree, e->s));
+    PS_CK(hipStreamWaitEvent(e->s2, e->ev_tree, 0));
+    ps_launch_p2p_collect(e->small_u32(), e->n_nodes, e->p2p_big, e->link(), e->nend(),
+                          reinterpret_cast<uint32_t*>(e->big_b.p), e->s2);
+    PS_CK(hipGetLastError());
+    e->collected = true;
+    return PS_HIP_OK;
 }
 
 // the tick from the traversal on (also its re-run after an arena overflow)
@@ -745,6 +781,7 @@ int run_from_traverse(ps_hip_engine* e) {
         PS_RC(stage_leaf_locals(e));
         PS_RC(stage_p2p(e, 0, 6, e->ltot()));
     }
+    PS_CK(join_side(e));
     e->record(7);
     if (e->mlp_on) {
         ps_launch_mlp_apply(e->n, e->mlp_pk_b.p, e->pos4(),
@@ -761,11 +798,13 @@ int run_from_traverse(ps_hip_engine* e) {
 // Launches the whole tick; one host wait inside (the tree's counts, while
 // the GPU writes the tree).
 int run_tick(ps_hip_engine* e) {
+    e->collected = false;
     e->record(1);
     PS_RC(stage_sort(e));
     e->record(2);
     int rc = stage_build(e);
     e->record(3);
+    if (rc == PS_HIP_OK) rc = stage_collect(e);
     if (rc == PS_HIP_OK) rc = stage_upward(e);
     if (rc == PS_HIP_OK) rc = run_from_traverse(e);
     const int rj = finish_copy(e);  // joined already unless a stage failed
@@ -1144,6 +1183,8 @@ int ps_hip_engine_create_in(ps_hip_engine_t** out, size_t max_particles, size_t 
     if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_async, hipEventDisableTiming);
     if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_sorted, hipEventDisableTiming);
     if (r == hipSuccess) r = hipEventCreateWithFlags(&e->ev_copied, hipEventDisableTiming);
+    for (hipEvent_t* v : {&e->ev_tree, &e->ev_trav, &e->ev_side})
+        if (r == hipSuccess) r = hipEventCreateWithFlags(v, hipEventDisableTiming);
     for (int k = 0; k < 12 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev[k]);
     for (int k = 0; k < 64 && r == hipSuccess; ++k) r = hipEventCreate(&e->ev_it[k]);
     int rc = PS_HIP_OK;
@@ -1797,6 +1838,7 @@ int ps_hip_debug_stage(ps_hip_engine_t* e, int stage) {
             for (int attempt = 0;; ++attempt) {
                 rc = stage_traverse(e);
                 if (rc != PS_HIP_OK) break;
+                PS_CK(join_side(e));
                 PS_CK(wait_stream(e->s));
                 bool redo = false;
                 rc = trav_finish(e, attempt, &redo);

@@ -222,13 +222,19 @@ void ps_launch_trav_step(PsTravTree t, float theta2, PsTravEntries ent, uint32_t
 // (>= 2 * ps_p2p_blocks(n, bound) entries), reduced into pair_count[0]
 // (particle pairs) and pair_count[1] (target leaf x source leaf pairs).
 uint32_t ps_p2p_blocks(size_t n, uint32_t bound);
+// The list of the big leaves (>= tmin particles; big = count, queue counter,
+// leaves) for the wavefront-per-leaf kernel: it reads only the tree, so a
+// caller may build it early on another stream and tell ps_launch_p2p
+// (collected), which otherwise builds it on s first.
+void ps_launch_p2p_collect(const uint32_t* ls, uint32_t bound, uint32_t tmin, const uint2* link,
+                           const uint32_t* nend, uint32_t* big, hipStream_t s);
 void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const float4* loc,
                    const uint32_t* p2p_off, const uint32_t* p2p_len, const uint32_t* p2p_moff,
                    const uint32_t* arena, const unsigned long long* masks, const uint2* link,
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
                    const uint32_t* ls, uint32_t bound, uint32_t tmin, uint32_t* big,
                    unsigned long long* block_pairs, unsigned long long* pair_count,
-                   hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join);
+                   hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join, bool collected = false);
 
 // ---- ps_direct.hip: FP64 direct sum (the error oracle on the device) -------
 void ps_launch_direct_f64(const float4* pos4, size_t n, const uint32_t* targets, size_t nt,

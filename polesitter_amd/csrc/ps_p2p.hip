@@ -537,6 +537,15 @@ void launch_mode(size_t n, const float4* pos4, const uint32_t* leaf_of, const fl
 
 }  // namespace
 
+void ps_launch_p2p_collect(const uint32_t* ls, uint32_t bound, uint32_t tmin, const uint2* link,
+                           const uint32_t* nend, uint32_t* big, hipStream_t s) {
+    if (!big || bound == 0 || tmin == 0) return;
+    ps_fill_u32(big, 0u, 2, s);  // count, queue
+    const uint32_t cb = (bound + kCollect - 1) / kCollect;
+    p2p_collect_kernel<<<cb < kCollectBlocks ? cb : kCollectBlocks, kCollect, 0, s>>>(
+        ls, tmin, link, nend, big);
+}
+
 uint32_t ps_p2p_blocks(size_t n, uint32_t bound) {
     return (uint32_t)((n + kBlock - 1) / kBlock) + big_blocks(bound);
 }
@@ -547,16 +556,12 @@ void ps_launch_p2p(size_t n, const float4* pos4, const uint32_t* leaf_of, const 
                    const uint32_t* nend, float* fx, float* fy, float* fz, int mode, int kahan,
                    const uint32_t* ls, uint32_t bound, uint32_t tmin, uint32_t* big,
                    unsigned long long* block_pairs, unsigned long long* pair_count,
-                   hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join) {
+                   hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join,
+                   bool collected) {
     if (n == 0) return;
     if (!big || bound == 0) tmin = 0;  // no big-leaf kernel
     unsigned long long* bp = pair_count ? block_pairs : nullptr;
-    if (tmin) {
-        ps_fill_u32(big, 0u, 2, s);  // count, queue
-        uint32_t cb = (bound + kCollect - 1) / kCollect;
-        p2p_collect_kernel<<<cb < kCollectBlocks ? cb : kCollectBlocks, kCollect, 0, s>>>(
-            ls, tmin, link, nend, big);
-    }
+    if (tmin && !collected) ps_launch_p2p_collect(ls, bound, tmin, link, nend, big, s);
 #define PS_P2P(M, K)                                                                           \
     launch_mode<M, K>(n, pos4, leaf_of, loc, p2p_off, p2p_len, p2p_moff, arena, masks, link, \
                       nend, fx, fy, fz, bound, tmin, big, bp, s, s_big, fork, join)

@@ -50,14 +50,24 @@ struct PartDepths {
 // window of leaf_cap + 1 consecutive particles containing i shares d digits,
 // so i's leaf depth is 1 + the largest lcp(j, j + leaf_cap), j in
 // [i - leaf_cap, i]; with leaf_cap = 1 that is the rule above.
+// The particle's own code is loaded here too, with its neighbours' (and
+// their systems): all requested at once, clamped to the array's ends.
 __device__ __forceinline__ PartDepths part_depths(const uint32_t* codes,
                                                   const uint32_t* __restrict__ sys, size_t n,
-                                                  size_t i, uint32_t c, int leaf_cap,
+                                                  size_t i, uint32_t& c, int leaf_cap,
                                                   const uint8_t* __restrict__ sys_cap) {
-    const uint32_t si = sys ? sys[i] : 0u;
+    const size_t ip = i > 0 ? i - 1 : i, in = i + 1 < n ? i + 1 : i;
+    c = codes[i];
+    const uint32_t cp = codes[ip], cn = codes[in];
+    uint32_t si = 0u, sp = 0u, sn = 0u;
+    if (sys) {
+        si = sys[i];
+        sp = sys[ip];
+        sn = sys[in];
+    }
     if (sys && sys_cap) leaf_cap = sys_cap[si];
-    int kp = (i > 0 && (!sys || sys[i - 1] == si)) ? octal_lcp(codes[i - 1], c) : -1;
-    int kn = (i + 1 < n && (!sys || sys[i + 1] == si)) ? octal_lcp(c, codes[i + 1]) : -1;
+    int kp = (i > 0 && sp == si) ? octal_lcp(cp, c) : -1;
+    int kn = (i + 1 < n && sn == si) ? octal_lcp(c, cn) : -1;
     int K = kp > kn ? kp : kn;
     if (leaf_cap > 1) {
         K = -1;
@@ -86,9 +96,9 @@ __global__ __launch_bounds__(kBlock) void tree_count_kernel(const uint32_t* __re
     const bool valid = i < n;
     PartDepths pd = {-1, kDepths, kDepths};
     if (valid) {
-        uint32_t c = codes[i];
-        if (c >> 30) atomicOr(flags, 1u);
+        uint32_t c;
         pd = part_depths(codes, sys, n, i, c, leaf_cap, sys_cap);
+        if (c >> 30) atomicOr(flags, 1u);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -124,10 +134,7 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
     const bool valid = i < n;
     uint32_t c = 0;
     PartDepths pd = {-1, kDepths, kDepths};
-    if (valid) {
-        c = codes[i];
-        pd = part_depths(codes, sys, n, i, c, leaf_cap, sys_cap);
-    }
+    if (valid) pd = part_depths(codes, sys, n, i, c, leaf_cap, sys_cap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t le = ps_lanemask_lt() | (1ull << lane);
     uint32_t inc[kDepths];  // this lane's inclusive start count per depth

@@ -432,70 +432,67 @@ __device__ __forceinline__ void write_entry(const PsTravTree& t, const PsTravEnt
     write_next<G>(next, tn, tl, out, p, gl);
 }
 
-// LDS stores of some lanes seen by later LDS loads of other lanes of the
-// same wavefront: its LDS operations complete in order, and the fences keep
-// the compiler from moving them across.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Staging words of a group in the write phase: a round's records expand to
-// at most 4 G words, plus up to 3 carried over from the round before.
-template <int G>
-constexpr uint32_t stage_words() {
-    return 4u * G + 4u;
+// Stores of a run of consecutive ids v0 .. v0 + n - 1 (n <= 8) from word
+// dst: 16-, 8- and 4-byte stores (dst is only 4-byte aligned)
+struct __attribute__((aligned(4))) PsRun4 {
+    uint32_t a, b, c, d;
+};
+struct __attribute__((aligned(4))) PsRun2 {
+    uint32_t a, b;
+};
+__device__ __forceinline__ void store_run(uint32_t* dst, uint32_t v0, uint32_t n) {
+    uint32_t r = 0;
+    if (n >= 4u) {
+        *reinterpret_cast<PsRun4*>(dst) = PsRun4{v0, v0 + 1u, v0 + 2u, v0 + 3u};
+        r = 4u;
+    }
+    if (n >= 8u) {
+        *reinterpret_cast<PsRun4*>(dst + 4) = PsRun4{v0 + 4u, v0 + 5u, v0 + 6u, v0 + 7u};
+        r = 8u;
+    }
+    if (n - r >= 2u) {
+        *reinterpret_cast<PsRun2*>(dst + r) = PsRun2{v0 + r, v0 + r + 1u};
+        r += 2u;
+    }
+    if (n > r) dst[r] = v0 + r;
 }
 
 // Write of one continuing entry from its near records in LDS (count of
-// them: n), the same list words in the same order as write_entry.  Rounds of
-// G / 2 records are expanded into the group's staging space stg and stored
-// as whole 16-byte pieces, one per lane: a list starts on a 4-word boundary
-// and its reservation is padded to a multiple of 4 words, so the last piece
-// may carry up to 3 stale words past the list's end, into that padding.
+// them: n), the same list words in the same order as write_entry: per round
+// each lane takes two records (their first ids: a leaf itself, an internal
+// node's first child, looked up together) and stores their ids straight to
+// the list, its positions from a group prefix sum -- no staging in LDS.
 template <int G>
 __device__ __forceinline__ void write_entry_rec(const PsTravTree& t, const PsTravEntries& ent,
-                                                uint32_t* __restrict__ arena,
-                                                const uint32_t* recs, uint32_t n, uint32_t e,
-                                                uint32_t out, uint4 p, const PsTravEntries& next,
-                                                uint32_t gl, uint32_t* stg) {
-    constexpr uint32_t H = G / 2;
+                                                uint32_t* __restrict__ arena, uint32_t* recs,
+                                                uint32_t n, uint32_t e, uint32_t out, uint4 p,
+                                                const PsTravEntries& next, uint32_t gl) {
     const uint32_t tn = ent.tgt[e];
     const uint2 tl = t.link[tn];
-    uint4* dst = reinterpret_cast<uint4*>(arena + p.x);  // p.x is a multiple of 4
-    const uint4* sq = reinterpret_cast<const uint4*>(stg);
-    const uint32_t nq = (out + 3u) >> 2;  // pieces of the padded list
-    uint32_t have = 0u;                   // staged words not stored yet (< 4 between rounds)
-    uint32_t done = 0u;                   // pieces stored
-    for (uint32_t r0 = 0; r0 < n; r0 += H) {
-        const uint32_t i = r0 + gl;
-        const bool v = gl < H && i < n;
-        const uint32_t rc = v ? recs[i] : 0u;
-        const uint32_t nout = v ? (rc >> kRecShift) + 1u : 0u;
-        const uint32_t id = rc & kRecMask;
-        // an internal node stands for its children, from its first child
-        const uint32_t v0 = (rc & kRecInternal) ? node_ld(t.link, id).x : id;
-        uint32_t total;
-        const uint32_t inc = group_scan_u32<G>(nout, gl, total);
-        const uint32_t w = have + inc - nout;
+    uint32_t base = p.x;
+    for (uint32_t r0 = 0; r0 < n; r0 += 2u * G) {
+        // two records per lane: ids and counts, internal ones resolved
+        uint32_t rv[2], v0[2], nout[2];
 #pragma unroll
-        for (uint32_t r = 0; r < 8; ++r)
-            if (r < nout) stg[w + r] = v0 + r;
-        wave_lds_sync();
-        const uint32_t staged = have + total;
-        const uint32_t full = staged >> 2;  // whole pieces (at most G)
-        if (gl < full && done + gl < nq) dst[done + gl] = sq[gl];
-        const uint32_t tail = staged & 3u;
-        uint32_t tw = 0u;
-        if (gl < tail) tw = stg[(full << 2) + gl];
-        wave_lds_sync();
-        if (gl < tail) stg[gl] = tw;
-        wave_lds_sync();
-        have = tail;
-        done += full;
+        for (uint32_t q = 0; q < 2; ++q) {
+            const uint32_t i = r0 + q * G + gl;
+            rv[q] = i < n ? recs[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 2; ++q) {
+            const uint32_t i = r0 + q * G + gl;
+            nout[q] = i < n ? (rv[q] >> kRecShift) + 1u : 0u;
+            const uint32_t id = rv[q] & kRecMask;
+            v0[q] = (rv[q] & kRecInternal) ? node_ld(t.link, id).x : id;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 2; ++q) {
+            uint32_t total;
+            const uint32_t inc = group_scan_u32<G>(nout[q], gl, total);
+            if (nout[q]) store_run(arena + base + inc - nout[q], v0[q], nout[q]);
+            base += total;
+        }
     }
-    if (have && gl == 0 && done < nq) dst[done] = sq[0];
     write_next<G>(next, tn, tl, out, p, gl);
 }
 
@@ -524,7 +521,6 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
     __shared__ uint32_t sseg[2];                 // the next segment: first entry, length
     __shared__ uint32_t spool[kPoolWords];       // near records, kPoolWords / kGroups per group
     __shared__ uint32_t spc[kSeg];               // per entry: its records (or kNoPiece)
-    __shared__ __align__(16) uint32_t sstg[kGroups * stage_words<G>()];  // write-phase staging
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t grp = tid / G;
     const uint32_t gl = tid & (G - 1);
@@ -641,7 +637,7 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
             const uint32_t j = j0 + tid;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (j < segn) v = scnt[j];
-            const uint32_t vx = (v.x + 3u) & ~3u;  // lists padded to whole 16-byte pieces
+            const uint32_t vx = (v.x + 3u) & ~3u;  // every list starts 16-byte aligned
             uint4 inc = make_uint4(vx, v.y, v.z, 0u);
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
@@ -711,7 +707,7 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
                 const uint32_t pc = spc[k];
                 if (pc != kNoPiece)
                     write_entry_rec<G>(t, ent, arena, spool + (pc & 0xFFFFu), pc >> 16, seg + k,
-                                       q.w, pl, next, gl, sstg + grp * stage_words<G>());
+                                       q.w, pl, next, gl);
                 else
                     write_entry<G>(t, ent, arena, masks, seg + k, q.w, pl, next, gl);
             }

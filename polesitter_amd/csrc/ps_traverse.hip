@@ -163,6 +163,13 @@ __global__ __launch_bounds__(kBlock) void trav_init_kernel(const uint32_t* __res
 
 __device__ __forceinline__ uint32_t chunks_of(uint32_t len) { return (len + 63u) >> 6; }
 
+// What a continuing target passes to the next iteration, packed by the
+// count phase: id << 4 | k -- itself (id = the target, k = 0) for a leaf,
+// its k children (id = the first child) for an internal node.
+__device__ __forceinline__ uint32_t next_word(uint32_t tn, uint2 tl) {
+    return (tl.y & PS_LEAF) ? tn << 4 : (tl.x << 4) | ps_child_count(tl);
+}
+
 // Near records: what a near source passes on to the next list, kept in LDS
 // between the count and the write phase -- the source's id (bits 0..27), a
 // flag for an internal node (bit 28) and the count - 1 (bits 29..31) of the
@@ -203,7 +210,7 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2, 
                                              unsigned long long* __restrict__ masks,
                                              uint32_t mask_cap, uint32_t gl,
                                              uint32_t gshift, uint32_t* pool, uint32_t& bump,
-                                             uint32_t pool_end, uint32_t& piece) {
+                                             uint32_t pool_end, uint32_t& piece, uint32_t& nx) {
     constexpr int U = 64 / G;
     const uint32_t tn = me.x;
     const uint32_t o = me.y;
@@ -322,6 +329,7 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2, 
         ez = group_sum<G>(ez);
     }
     uint4 c = make_uint4(0u, 0u, 0u, nfar);
+    nx = next_word(tn, tl);
     if (tl.y & PS_LEAF) {
         if (nint == 0) {  // final: the near entries of this list are the P2P sources
             if (piece != kNoPiece) bump = piece;  // its records are not needed
@@ -347,23 +355,23 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2, 
     return c;
 }
 
-// The next iteration's entries of a continuing target tn: itself (a leaf) or
-// its children, all with the next list at p.x of length out.
+// The next iteration's entries of a continuing target (its next_word nx),
+// all with the next list at p.x of length out.
 template <int G>
-__device__ __forceinline__ void write_next(const PsTravEntries& next, uint32_t tn, uint2 tl,
-                                           uint32_t out, uint4 p, uint32_t gl) {
+__device__ __forceinline__ void write_next(const PsTravEntries& next, uint32_t nx, uint32_t out,
+                                           uint4 p, uint32_t gl) {
     const uint32_t nch = chunks_of(out);
-    if (tl.y & PS_LEAF) {
+    const uint32_t id = nx >> 4, nk = nx & 15u;
+    if (nk == 0) {
         if (gl == 0) {
-            next.tgt[p.y] = tn;
+            next.tgt[p.y] = id;
             next.off[p.y] = p.x;
             next.len[p.y] = out;
             next.moff[p.y] = p.z;
         }
     } else {
-        const uint32_t nk = ps_child_count(tl);
         for (uint32_t r = gl; r < nk; r += G) {
-            next.tgt[p.y + r] = tl.x + r;
+            next.tgt[p.y + r] = id + r;
             next.off[p.y + r] = p.x;
             next.len[p.y + r] = out;
             next.moff[p.y + r] = p.z + r * nch;
@@ -380,14 +388,12 @@ template <int G>
 __device__ __forceinline__ void write_entry(const PsTravTree& t, const PsTravEntries& ent,
                                             uint32_t* __restrict__ arena,
                                             const unsigned long long* __restrict__ masks,
-                                            uint32_t e, uint32_t out, uint4 p,
+                                            uint32_t e, uint32_t nx, uint32_t out, uint4 p,
                                             const PsTravEntries& next, uint32_t gl) {
     constexpr int U = 64 / G;
-    const uint32_t tn = ent.tgt[e];
     const uint32_t o = ent.off[e];
     const uint32_t L = ent.len[e];
     const uint32_t mo = ent.moff[e];
-    const uint2 tl = t.link[tn];
     // the count pass's masks and link reads give exactly `out` outputs; the
     // bound only makes that an invariant of the stores
     uint32_t base = p.x;
@@ -429,7 +435,7 @@ __device__ __forceinline__ void write_entry(const PsTravTree& t, const PsTravEnt
             base += total;
         }
     }
-    write_next<G>(next, tn, tl, out, p, gl);
+    write_next<G>(next, nx, out, p, gl);
 }
 
 // Stores of a run of consecutive ids v0 .. v0 + n - 1 (n <= 8) from word
@@ -463,12 +469,10 @@ __device__ __forceinline__ void store_run(uint32_t* dst, uint32_t v0, uint32_t n
 // node's first child, looked up together) and stores their ids straight to
 // the list, its positions from a group prefix sum -- no staging in LDS.
 template <int G>
-__device__ __forceinline__ void write_entry_rec(const PsTravTree& t, const PsTravEntries& ent,
-                                                uint32_t* __restrict__ arena, uint32_t* recs,
-                                                uint32_t n, uint32_t e, uint32_t out, uint4 p,
-                                                const PsTravEntries& next, uint32_t gl) {
-    const uint32_t tn = ent.tgt[e];
-    const uint2 tl = t.link[tn];
+__device__ __forceinline__ void write_entry_rec(const PsTravTree& t, uint32_t* __restrict__ arena,
+                                                uint32_t* recs, uint32_t n, uint32_t nx,
+                                                uint32_t out, uint4 p, const PsTravEntries& next,
+                                                uint32_t gl) {
     uint32_t base = p.x;
     for (uint32_t r0 = 0; r0 < n; r0 += 2u * G) {
         // two records per lane: ids and counts, internal ones resolved
@@ -493,7 +497,7 @@ __device__ __forceinline__ void write_entry_rec(const PsTravTree& t, const PsTra
             base += total;
         }
     }
-    write_next<G>(next, tn, tl, out, p, gl);
+    write_next<G>(next, nx, out, p, gl);
 }
 
 // next item of a block-local work queue, the same value in every lane of the group
@@ -521,6 +525,7 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
     __shared__ uint32_t sseg[2];                 // the next segment: first entry, length
     __shared__ uint32_t spool[kPoolWords];       // near records, kPoolWords / kGroups per group
     __shared__ uint32_t spc[kSeg];               // per entry: its records (or kNoPiece)
+    __shared__ uint32_t snx[kSeg];               // per entry: what it passes on (next_word)
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t grp = tid / G;
     const uint32_t gl = tid & (G - 1);
@@ -620,12 +625,13 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
         const uint32_t pool_end = bump + (rec_cap < kPoolPer ? rec_cap : kPoolPer);
         for (uint32_t i = group_take<G>(&queue[0], gl); i < segn; i = group_take<G>(&queue[0], gl)) {
             const uint32_t k = sord[i];
-            uint32_t piece = bump;
+            uint32_t piece = bump, nx = 0;
             const uint4 c = count_entry<kKahan, G>(t, theta2, scnt[k], arena, masks, mask_cap, gl,
-                                                   gshift, spool, bump, pool_end, piece);
+                                                   gshift, spool, bump, pool_end, piece, nx);
             if (gl == 0) {
                 scnt[k] = c;
                 spc[k] = piece == kNoPiece ? kNoPiece : (piece | ((bump - piece) << 16));
+                snx[k] = nx;
             }
             far_acc += c.w;
         }
@@ -705,11 +711,12 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
                 if (q.w == 0) continue;  // final or empty: nothing to pass on
                 const uint4 pl = make_uint4(la + q.x, le + q.y, lm + q.z, 0u);
                 const uint32_t pc = spc[k];
+                const uint32_t nx = snx[k];
                 if (pc != kNoPiece)
-                    write_entry_rec<G>(t, ent, arena, spool + (pc & 0xFFFFu), pc >> 16, seg + k,
-                                       q.w, pl, next, gl);
+                    write_entry_rec<G>(t, arena, spool + (pc & 0xFFFFu), pc >> 16, nx, q.w, pl,
+                                       next, gl);
                 else
-                    write_entry<G>(t, ent, arena, masks, seg + k, q.w, pl, next, gl);
+                    write_entry<G>(t, ent, arena, masks, seg + k, nx, q.w, pl, next, gl);
             }
         }
         __syncthreads();  // scnt and sbase are rewritten by the next segment

@@ -119,16 +119,25 @@ __device__ __forceinline__ bool ps_far_d(float4 t, float4 s, float theta2, float
     return d2 > theta2 * (hs * hs);
 }
 
+// inclusive prefix sum over the wavefront with DPP lane moves (no LDS
+// round trip): row shifts by 1, 2, 4, 8 give each row of 16 lanes its own
+// prefix (lanes shifted in from outside the row read 0), then the last lane
+// of row 0 (row 2) is broadcast into row 1 (row 3), and lane 31 into rows 2
+// and 3 (rows outside a broadcast's row mask take 0)
+__device__ __forceinline__ uint32_t ps_wave_incl_scan(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return v;
+}
+
 // exclusive prefix over the wave; *total gets the wave-wide sum
 __device__ __forceinline__ uint32_t ps_wave_excl_scan(uint32_t v, uint32_t* total) {
-    const int lane = __lane_id();
-    uint32_t inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += o;
-    }
-    *total = __shfl(inc, 63, 64);
+    const uint32_t inc = ps_wave_incl_scan(v);
+    *total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     return inc - v;
 }
 

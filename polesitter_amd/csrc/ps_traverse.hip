@@ -644,18 +644,8 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (j < segn) v = scnt[j];
             const uint32_t vx = (v.x + 3u) & ~3u;  // every list starts 16-byte aligned
-            uint4 inc = make_uint4(vx, v.y, v.z, 0u);
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t ox = __shfl_up(inc.x, d, 64);
-                const uint32_t oy = __shfl_up(inc.y, d, 64);
-                const uint32_t oz = __shfl_up(inc.z, d, 64);
-                if (lane >= (uint32_t)d) {
-                    inc.x += ox;
-                    inc.y += oy;
-                    inc.z += oz;
-                }
-            }
+            const uint4 inc =
+                make_uint4(ps_wave_incl_scan(vx), ps_wave_incl_scan(v.y), ps_wave_incl_scan(v.z), 0u);
             if (lane == 63) wtot[wave] = inc;
             __syncthreads();
             uint4 run = carry, all = carry;

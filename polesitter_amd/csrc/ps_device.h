@@ -228,8 +228,49 @@ __device__ __forceinline__ float dpp_f32(float v) {
 constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
 constexpr int kDppShr1 = 0x111, kDppShr2 = 0x112, kDppShr4 = 0x114, kDppShr8 = 0x118;
 
-// sum over the G lanes of one group (G a power of two <= 64); groups of 8 and
-// 16 with DPP, wider ones with shuffles; the same value in every lane
+// Inclusive prefix sums over groups of 32 or 64 lanes with DPP (see
+// ps_wave_incl_scan): rows of 16 by shifts, row 0's (row 2's) last lane into
+// row 1 (row 3), and for 64 lanes lane 31 into rows 2 and 3.  The group's
+// total is its last lane's, read into a scalar.
+template <int G>
+__device__ __forceinline__ uint32_t wide_incl_scan_u32(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kDppShr1, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kDppShr2, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kDppShr4, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kDppShr8, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    if (G == 64)
+        v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return v;
+}
+// one DPP step of a float scan: x moved by CTRL within the rows ROWS (0 in
+// lanes that take nothing)
+template <int CTRL, int ROWS, bool BOUND>
+__device__ __forceinline__ float dpp_step_f32(float x) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, ROWS, 0xF, BOUND));
+}
+template <int G>
+__device__ __forceinline__ float wide_incl_scan_f32(float v) {
+    v += dpp_step_f32<kDppShr1, 0xF, true>(v);
+    v += dpp_step_f32<kDppShr2, 0xF, true>(v);
+    v += dpp_step_f32<kDppShr4, 0xF, true>(v);
+    v += dpp_step_f32<kDppShr8, 0xF, true>(v);
+    v += dpp_step_f32<0x142, 0xA, false>(v);  // row_bcast:15
+    if (G == 64) v += dpp_step_f32<0x143, 0xC, false>(v);  // row_bcast:31
+    return v;
+}
+// the last lane's value of each group of 32 or 64 lanes, in all its lanes
+template <int G>
+__device__ __forceinline__ uint32_t wide_last_u32(uint32_t v) {
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+    if (G == 64) return hi;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+    return __lane_id() < 32 ? lo : hi;
+}
+
+// sum over the G lanes of one group (G a power of two <= 64), all with DPP
+// lane moves; the same value in every lane
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
     if (G <= 16) {
@@ -239,9 +280,8 @@ __device__ __forceinline__ float group_sum(float v) {
         if (G == 16) v += dpp_f32<kDppMirror>(v);
         return v;
     }
-#pragma unroll
-    for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
-    return v;
+    return __builtin_bit_cast(float,
+                              wide_last_u32<G>(__builtin_bit_cast(uint32_t, wide_incl_scan_f32<G>(v))));
 }
 
 // inclusive prefix sum over the lanes of one group (gl: lane in the group)
@@ -270,12 +310,8 @@ __device__ __forceinline__ uint32_t group_scan_u32(uint32_t v, uint32_t gl, uint
         total = s;
         return inc;
     }
-#pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-        const uint32_t o = __shfl_up(inc, d, G);
-        if (gl >= (uint32_t)d) inc += o;
-    }
-    total = __shfl(inc, G - 1, G);
+    inc = wide_incl_scan_u32<G>(inc);
+    total = wide_last_u32<G>(inc);
     return inc;
 }
 
@@ -300,9 +336,7 @@ __device__ __forceinline__ uint32_t group_sum_u32(uint32_t v) {
         if (G == 16) v += dpp_u32<kDppMirror>(v);
         return v;
     }
-#pragma unroll
-    for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
-    return v;
+    return wide_last_u32<G>(wide_incl_scan_u32<G>(v));
 }
 
 // orders the LDS accesses of one wavefront (its lanes run in step: no

@@ -29,10 +29,15 @@
 //   place  block prefix sum of the sizes; one atomic add per counter (list
 //          words, next entries, mask words) reserves the segment's output,
 //          checked against the arena capacities.
-//   write  for continuing entries only, while the list is still in cache:
-//          the mask picks the near sources, near leaves are copied and near
-//          internal nodes expanded to their children, compacted with a group
-//          prefix sum into the reserved list space; next entries likewise.
+//   write  for continuing entries only: the near records the count left in
+//          LDS (a near leaf's id, a near internal node's id and child count)
+//          become runs of consecutive ids stored straight to the reserved
+//          list space, at positions from a group prefix sum; an entry whose
+//          records did not fit the LDS pool reads its list again (the far
+//          mask picks the near sources, their links give the children): the
+//          same words either way.  Next entries from the word the count
+//          phase kept per entry (the target itself, or its first child and
+//          child count).
 // Where a segment's output lands depends on the order of the atomics; what
 // each entry computes does not (its list order is fixed by its producer), so
 // the results are deterministic.
@@ -216,9 +221,6 @@ __device__ __forceinline__ uint4 count_entry(const PsTravTree& t, float theta2, 
     const uint32_t o = me.y;
     const uint32_t L = me.z;
     const uint32_t mo = me.w;
-    // the target's local, read now (it is only updated at the end): its load
-    // overlaps the list's instead of following the sums
-
     const float4 tg = t.geom[tn];
     const uint2 tl = t.link[tn];
     // the root's half width: the target's times 2^depth, exactly

@@ -769,8 +769,8 @@ int stage_collect(ps_hip_engine* e) {
     return PS_HIP_OK;
 }
 
-// the tick from the traversal on (also its re-run after an arena overflow)
-int run_from_traverse(ps_hip_engine* e) {
+// traversal, L2L (or the leaves' locals) and P2P
+int traverse_to_p2p(ps_hip_engine* e) {
     e->record(4);
     PS_RC(stage_traverse(e));
     e->record(5);
@@ -781,7 +781,17 @@ int run_from_traverse(ps_hip_engine* e) {
         PS_RC(stage_leaf_locals(e));
         PS_RC(stage_p2p(e, 0, 6, e->ltot()));
     }
-    PS_CK(join_side(e));
+    return PS_HIP_OK;
+}
+
+// the tick from the traversal on (also its re-run after an arena overflow)
+int run_from_traverse(ps_hip_engine* e) {
+    const int rc = traverse_to_p2p(e);
+    // the second stream's work joins the engine stream on every path, so
+    // nothing of this tick still runs there once the engine stream drains
+    const hipError_t rj = join_side(e);
+    if (rc != PS_HIP_OK) return rc;
+    PS_CK(rj);
     e->record(7);
     if (e->mlp_on) {
         ps_launch_mlp_apply(e->n, e->mlp_pk_b.p, e->pos4(),
@@ -806,7 +816,10 @@ int run_tick(ps_hip_engine* e) {
     e->record(3);
     if (rc == PS_HIP_OK) rc = stage_collect(e);
     if (rc == PS_HIP_OK) rc = stage_upward(e);
-    if (rc == PS_HIP_OK) rc = run_from_traverse(e);
+    if (rc == PS_HIP_OK)
+        rc = run_from_traverse(e);
+    else
+        (void)join_side(e);  // a failed stage: the side stream's work still joins
     const int rj = finish_copy(e);  // joined already unless a stage failed
     return rc != PS_HIP_OK ? rc : rj;
 }

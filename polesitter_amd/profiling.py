@@ -97,21 +97,6 @@ def run(args):
     from . import workloads as W
 
     L = api.hip_lib()
-    nb = args.cal_mb << 20
-    a = api.DeviceArray(nb // 4, np.float32)
-    b = api.DeviceArray(nb // 4, np.float32)
-    g = gather_elems(args.cal_mb)
-    rng = np.random.default_rng(5)
-    idx = api.DeviceArray.from_numpy(rng.integers(0, nb // 4, g, dtype=np.uint32))
-    out = api.DeviceArray(g, np.uint32)
-    for _ in range(3):
-        api.check("fill", L.ps_hip_op_bandwidth(2, None, a.ptr, None, nb // 16))
-        api.check("read", L.ps_hip_op_bandwidth(1, a.ptr, b.ptr, None, nb // 16))
-        api.check("copy", L.ps_hip_op_bandwidth(0, a.ptr, b.ptr, None, nb // 32))
-        api.check("gather", L.ps_hip_op_bandwidth(3, a.ptr, out.ptr, idx.ptr, g))
-    print(f"profiling: calibration kernels over {nb} bytes x3", flush=True)
-    del a, b, idx, out
-
     if args.workload == "plummer1m":
         n = 1 << 20
         x, y, z, m = W.plummer(n, seed=1)
@@ -126,10 +111,32 @@ def run(args):
     s.load(x, y, z, m)
     vel = [api.DeviceArray.from_numpy(vv) for vv in v]
     tmp = api.DeviceArray(3 * n, np.float32)
-    for k in range(args.steps):  # the bench's step: tick, then the fused frame step
+
+    def step():  # the bench's step: tick, then the fused frame step
         s.tick(api.MORTON_AVX2)
         s.frame_step(vel, tmp, 1e-3)
         api.check("sync", L.ps_hip_synchronize())
+
+    # the first tick (a full sort; later ones sort incrementally) comes before
+    # the calibration kernels, and the summary counts the ticks after them
+    step()
+    print("profiling: warm-up step done", flush=True)
+    nb = args.cal_mb << 20
+    a = api.DeviceArray(nb // 4, np.float32)
+    b = api.DeviceArray(nb // 4, np.float32)
+    g = gather_elems(args.cal_mb)
+    rng = np.random.default_rng(5)
+    idx = api.DeviceArray.from_numpy(rng.integers(0, nb // 4, g, dtype=np.uint32))
+    out = api.DeviceArray(g, np.uint32)
+    for _ in range(3):
+        api.check("fill", L.ps_hip_op_bandwidth(2, None, a.ptr, None, nb // 16))
+        api.check("read", L.ps_hip_op_bandwidth(1, a.ptr, b.ptr, None, nb // 16))
+        api.check("copy", L.ps_hip_op_bandwidth(0, a.ptr, b.ptr, None, nb // 32))
+        api.check("gather", L.ps_hip_op_bandwidth(3, a.ptr, out.ptr, idx.ptr, g))
+    print(f"profiling: calibration kernels over {nb} bytes x3", flush=True)
+    del a, b, idx, out
+    for k in range(args.steps):
+        step()
         print(f"profiling: step {k + 1}/{args.steps} done", flush=True)
     s.close()
     return 0
@@ -139,13 +146,26 @@ def _csvs(d, suffix):
     return sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
 
 
+def _after_warmup(rows):
+    """rows: [(dispatch id, kernel name, row)] of one file.  The tick
+    dispatches before the first calibration kernel (the warm-up step of a
+    `run`) are dropped; a file without calibration kernels is kept whole."""
+    first_cal = min((d for d, name, _ in rows if family_of(name) in CAL), default=None)
+    if first_cal is None:
+        return rows
+    return [(d, name, r) for d, name, r in rows
+            if d >= first_cal or family_of(name) in CAL]
+
+
 def read_all(dirs):
-    """{(path, dispatch): (kernel, {counter: value})} over the passes in dirs."""
+    """{(path, dispatch): (kernel, {counter: value})} over the passes in dirs
+    (without a run's warm-up step)."""
     out = {}
     for d in dirs:
         for path in _csvs(d, "counter_collection.csv"):
             with open(path, newline="") as f:
-                for r in csv.DictReader(f):
+                rows = [(int(r["Dispatch_Id"]), r["Kernel_Name"], r) for r in csv.DictReader(f)]
+                for _, _, r in _after_warmup(rows):
                     key = (path, int(r["Dispatch_Id"]))
                     name, vals = out.get(key, (r["Kernel_Name"], {}))
                     c = r["Counter_Name"]
@@ -176,7 +196,8 @@ def family_ms_per_tick(trace_dir):
     ticks = 0
     for path in _csvs(trace_dir, "kernel_trace.csv"):
         with open(path, newline="") as f:
-            for r in csv.DictReader(f):
+            rows = [(int(r["Dispatch_Id"]), r["Kernel_Name"], r) for r in csv.DictReader(f)]
+            for _, _, r in _after_warmup(rows):
                 name = r["Kernel_Name"]
                 if TICK_MARK in name:
                     ticks += 1
@@ -342,7 +363,8 @@ def counters(args):
     for d in args.dirs:
         for path in _csvs(d, "counter_collection.csv"):
             with open(path, newline="") as f:
-                for r in csv.DictReader(f):
+                rows = [(int(r["Dispatch_Id"]), r["Kernel_Name"], r) for r in csv.DictReader(f)]
+                for _, _, r in _after_warmup(rows):
                     key = (path, int(r["Dispatch_Id"]))
                     c = r["Counter_Name"]
                     disp[key][c] = disp[key].get(c, 0.0) + float(r["Counter_Value"])

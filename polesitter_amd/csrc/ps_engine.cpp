@@ -941,6 +941,9 @@ struct ps_hip_batch {
     DevBuf aux_b;                // two u32 arrays: the second key's sort scratch
     DevBuf prm_b, roots_b;       // float4 per system
     DevBuf caps_b;               // u8 per system: leaf capacities (when caps_n > 0)
+    DevBuf seg_b;                // the segmented sort's tile tables (vstart, vseg, seg_tile)
+    std::vector<uint32_t> seg_h;  // their host copy (the upload's source)
+    PsSortSegs segs{};           // n_tiles 0: the sort takes the system id as a fourth pass
     size_t caps_n = 0;           // systems covered by caps_b (0: the engine option)
     size_t node_cap = 0;         // guaranteed bound for the current layout
     int use_graph = 1;
@@ -970,6 +973,40 @@ size_t forest_node_bound(const std::vector<uint32_t>& off) {
     size_t total = 0;
     for (size_t k = 0; k + 1 < off.size(); ++k) total += worst_case_nodes(off[k + 1] - off[k]);
     return total;
+}
+
+// Systems up to this many sort segment by segment (three passes, tile tables
+// and 4 KiB of digit starts per system); more take the system id as a
+// fourth pass.
+constexpr uint32_t kSegSortMaxSys = 4096;
+
+// The segmented sort's tables for a layout (off: n_sys + 1 offsets), on the
+// device; the sort scratch grown to fit.
+int batch_segments(ps_hip_batch* b, const std::vector<uint32_t>& off, uint32_t n_sys) {
+    ps_hip_engine* e = b->e;
+    hipStream_t s = e->s;
+    b->segs = PsSortSegs{};
+    if (n_sys == 0 || n_sys > kSegSortMaxSys) return PS_HIP_OK;
+    const size_t t = ps_sort_seg_tiles(off.data(), n_sys, nullptr, nullptr, nullptr);
+    if (t == 0 || t > 0xFFFFFFFEull) return PS_HIP_OK;
+    b->seg_h.assign((t + 1) + t + (n_sys + 1), 0u);
+    uint32_t* vstart = b->seg_h.data();
+    uint32_t* vseg = vstart + t + 1;
+    uint32_t* seg_tile = vseg + t;
+    ps_sort_seg_tiles(off.data(), n_sys, vstart, vseg, seg_tile);
+    PS_CK(grow(b->seg_b, 4 * b->seg_h.size(), s));
+    PS_CK(grow(e->sort_tmp, ps_sort_seg_tmp_bytes(t, n_sys), s));
+    PS_CK(hipMemcpyAsync(b->seg_b.p, b->seg_h.data(), 4 * b->seg_h.size(),
+                         hipMemcpyHostToDevice, s));
+    // the copy reads seg_h: it stays unchanged until the next layout
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(b->seg_b.p);
+    b->segs.off = reinterpret_cast<const uint32_t*>(b->off_b.p);
+    b->segs.n_sys = n_sys;
+    b->segs.vstart = d;
+    b->segs.vseg = d + t + 1;
+    b->segs.seg_tile = d + 2 * t + 1;
+    b->segs.n_tiles = (uint32_t)t;
+    return PS_HIP_OK;
 }
 
 // every buffer of a batch tick over n particles and `cap` nodes; no
@@ -1016,9 +1053,13 @@ int batch_run_front(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
                           s);
     PsSortArrays io = {e->code, e->id, e->x, e->y, e->z, e->m, sys};
     PsSortArrays t0 = e->sort_set(0), t1 = e->sort_set(1);
-    t0.aux = reinterpret_cast<uint32_t*>(b->aux_b.p);
-    t1.aux = t0.aux + e->max_p;
-    PS_CK(ps_radix_sort(io, t0, t1, e->pos4(), n, e->sort_tmp.p, s));
+    if (b->segs.n_tiles) {  // segment by segment: every particle stays in its system
+        PS_CK(ps_radix_sort_seg(io, t0, t1, e->pos4(), n, b->segs, e->sort_tmp.p, s));
+    } else {
+        t0.aux = reinterpret_cast<uint32_t*>(b->aux_b.p);
+        t1.aux = t0.aux + e->max_p;
+        PS_CK(ps_radix_sort(io, t0, t1, e->pos4(), n, e->sort_tmp.p, s));
+    }
     const uint8_t* caps = b->caps_n ? reinterpret_cast<const uint8_t*>(b->caps_b.p) : nullptr;
     PS_CK(ps_tree_count(e->code, sys, n, ls, ls + 12, e->totals(), e->tree_tmp.p, e->leaf_cap,
                         s, caps));
@@ -1442,7 +1483,8 @@ int ps_hip_batch_destroy(ps_hip_batch_t* b) {
     if (!b) return PS_HIP_EINVAL;
     if (b->e && b->e->s) (void)hipStreamSynchronize(b->e->s);
     b->drop_graph();
-    DevBuf* bufs[] = {&b->off_b, &b->sys_b, &b->aux_b, &b->prm_b, &b->roots_b, &b->caps_b};
+    DevBuf* bufs[] = {&b->off_b, &b->sys_b, &b->aux_b, &b->prm_b, &b->roots_b, &b->caps_b,
+                      &b->seg_b};
     for (DevBuf* d : bufs) release(*d);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1524,6 +1566,7 @@ int ps_hip_batch_tick(ps_hip_batch_t* b, float* x, float* y, float* z, float* ma
         b->node_cap = forest_node_bound(off);
         PS_RC(batch_ensure(b, n, b->node_cap));
         PS_CK(hipMemcpyAsync(b->off_b.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, s));
+        PS_RC(batch_segments(b, off, nsys));
         PS_RC(batch_run(b, nsys, morton_mode));
     }
     PS_CK(hipEventRecord(b->ev1, s));

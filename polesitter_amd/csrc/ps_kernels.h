@@ -81,6 +81,26 @@ size_t ps_sort_tmp_bytes(size_t n);
 // (and with it y, z, m) may be null in all three sets: keys and id only.
 hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
                          size_t n, void* tmp, hipStream_t s, uint32_t* kcopy = nullptr);
+// Many systems at once, each a contiguous segment [off[s], off[s + 1]) of
+// the arrays (s < n_sys, off[0] = 0, off[n_sys] = n): each segment sorted by
+// key bits 0..29 in place, the same permutation as ps_radix_sort with the
+// segment index as a second key, in three passes.  The tiles are cut at the
+// segments' ends: ps_sort_seg_tiles fills (host arrays) vstart[0..T] (tile
+// starts, then n), vseg[0..T) (a tile's segment) and seg_tile[0..n_sys]
+// (a segment's first tile, then T), and returns T (with null arrays: only T).
+struct PsSortSegs {
+    const uint32_t* off;       // device, n_sys + 1
+    uint32_t n_sys;
+    const uint32_t* vstart;    // device, n_tiles + 1
+    const uint32_t* vseg;      // device, n_tiles
+    const uint32_t* seg_tile;  // device, n_sys + 1
+    uint32_t n_tiles;
+};
+size_t ps_sort_seg_tiles(const uint32_t* off, uint32_t n_sys, uint32_t* vstart, uint32_t* vseg,
+                         uint32_t* seg_tile);
+size_t ps_sort_seg_tmp_bytes(size_t n_tiles, uint32_t n_sys);
+hipError_t ps_radix_sort_seg(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
+                             size_t n, PsSortSegs segs, void* tmp, hipStream_t s);
 // The incremental sort.  prev: the keys the last sort produced, in place
 // (sorted).  Step 1 (count) flags the positions whose key (low 30 bits)
 // differs from prev, writes their keys and positions in order to ckey and

@@ -20,9 +20,13 @@
 // Tiles are 4096 keys (upsweep: 256 threads x 16; downsweep: 1024 threads
 // x 4, sixteen wavefronts of 256 keys each ranked in four rounds).
 // Ping-pong: io -> t0 -> t1 -> io, so the sorted data ends in the caller's
-// arrays (H:1688-1690).  Many systems
-// sorted at once carry their system id as a second key: a fourth, stable pass
-// by it groups the systems again, each in code order.
+// arrays (H:1688-1690).  Many systems sorted at once (contiguous segments,
+// one per system) are sorted segment by segment in the same three passes:
+// the tiles are cut at the segments' ends, and a tile's digit runs go to
+// its segment's start + the digit's start inside the segment + the tile's
+// offset in it, so no key leaves its segment (ps_radix_sort_seg).  With the
+// system id as a second key instead, a fourth, stable pass by it groups the
+// systems again.
 #include "ps_device.h"
 #include "ps_kernels.h"
 
@@ -41,18 +45,34 @@ static_assert(kDigitsPer >= 1 && kRadix % kDBlock == 0, "digits per thread");
 static_assert(kRadix == kDBlock, "the downsweep scans the digit totals one per thread");
 constexpr int kRowBlock = 256;  // row scan
 
+// Tile t of the keys: [vstart[t], vstart[t + 1]) when the tiles are cut at
+// segment ends (vstart set), else [t kTile, min(n, (t + 1) kTile)).
+__device__ __forceinline__ void tile_range(const uint32_t* vstart, size_t n, uint32_t t,
+                                           size_t& base, uint32_t& cnt) {
+    if (vstart) {
+        base = vstart[t];
+        cnt = vstart[t + 1] - vstart[t];
+    } else {
+        base = (size_t)t * kTile;
+        cnt = (uint32_t)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void sort_upsweep_kernel(const uint32_t* __restrict__ key,
                                                               size_t n, int shift,
                                                               uint32_t* __restrict__ table,
-                                                              uint32_t n_tiles) {
+                                                              uint32_t n_tiles,
+                                                              const uint32_t* __restrict__ vstart) {
     __shared__ uint32_t h[kRadix];
     for (int d = threadIdx.x; d < kRadix; d += kBlock) h[d] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kTile;
+    size_t base;
+    uint32_t cnt;
+    tile_range(vstart, n, blockIdx.x, base, cnt);
 #pragma unroll 4
     for (int r = 0; r < kItems; ++r) {
-        size_t i = base + (size_t)r * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&h[(key[i] >> shift) & (kRadix - 1)], 1u);
+        const uint32_t j = (uint32_t)(r * kBlock) + threadIdx.x;
+        if (j < cnt) atomicAdd(&h[(key[base + j] >> shift) & (kRadix - 1)], 1u);
     }
     __syncthreads();
     for (int d = threadIdx.x; d < kRadix; d += kBlock)
@@ -94,6 +114,29 @@ __global__ __launch_bounds__(kRowBlock) void sort_rowscan_kernel(uint32_t* __res
     if (threadIdx.x == 0) dtot[blockIdx.x] = all;
 }
 
+// Segment s's digit starts, one digit per thread: its tiles are
+// [seg_tile[s], seg_tile[s + 1]), so digit d has P(b, d) - P(a, d) keys in it
+// (P: the scanned table, P(n_tiles, d) = dtot[d]); the digit's run starts at
+// off[s] + (the segment's keys of smaller digits).  sbase[s][d] is that start
+// minus P(a, d), so a tile of the segment adds its own P(t, d).
+__global__ __launch_bounds__(kRadix) void sort_segbase_kernel(
+    const uint32_t* __restrict__ table, uint32_t n_tiles, const uint32_t* __restrict__ dtot,
+    const uint32_t* __restrict__ seg_tile, const uint32_t* __restrict__ off,
+    uint32_t* __restrict__ sbase) {
+    __shared__ uint32_t wsum[kRadix / 64];
+    const uint32_t sg = blockIdx.x, d = threadIdx.x;
+    const uint32_t lane = d & 63u, wave = d >> 6;
+    const uint32_t a = seg_tile[sg], b = seg_tile[sg + 1];
+    const uint32_t pa = a < n_tiles ? table[(size_t)d * n_tiles + a] : dtot[d];
+    const uint32_t pb = b < n_tiles ? table[(size_t)d * n_tiles + b] : dtot[d];
+    uint32_t wt;
+    uint32_t ex = ps_wave_excl_scan(pb - pa, &wt);
+    if (lane == 0) wsum[wave] = wt;
+    __syncthreads();
+    for (uint32_t w = 0; w < wave; ++w) ex += wsum[w];
+    sbase[(size_t)sg * kRadix + d] = off[sg] + ex - pa;
+}
+
 __global__ __launch_bounds__(kDBlock) void sort_downsweep_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ iin,
     const float* __restrict__ xin, const float* __restrict__ yin, const float* __restrict__ zin,
@@ -101,7 +144,9 @@ __global__ __launch_bounds__(kDBlock) void sort_downsweep_kernel(
     float* __restrict__ xout, float* __restrict__ yout, float* __restrict__ zout,
     float* __restrict__ mout, float4* __restrict__ pos4, const uint32_t* __restrict__ ain,
     uint32_t* __restrict__ aout, size_t n, int shift, const uint32_t* __restrict__ offs,
-    const uint32_t* __restrict__ dtot, uint32_t n_tiles, uint32_t* __restrict__ kcopy) {
+    const uint32_t* __restrict__ dtot, uint32_t n_tiles, uint32_t* __restrict__ kcopy,
+    const uint32_t* __restrict__ vstart, const uint32_t* __restrict__ vseg,
+    const uint32_t* __restrict__ sbase) {
     __shared__ uint32_t keys[kTile];
     __shared__ uint16_t sidx[kTile];
     __shared__ uint32_t tstart[kRadix];
@@ -112,24 +157,30 @@ __global__ __launch_bounds__(kDBlock) void sort_downsweep_kernel(
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const size_t base = (size_t)blockIdx.x * kTile;
-    const uint32_t valid_n = (uint32_t)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    size_t base;
+    uint32_t valid_n;
+    tile_range(vstart, n, blockIdx.x, base, valid_n);
 
     // where digit tid's run of this tile goes: the digit's start (exclusive
-    // scan of the totals, one digit per thread) + the tile's offset in it
+    // scan of the totals, one digit per thread; or the tile's segment's
+    // start for the digit) + the tile's offset in it
     {
         const uint32_t d = (uint32_t)tid;
         const uint32_t own = offs[(size_t)d * n_tiles + blockIdx.x];
-        const uint32_t c = dtot[d];
-        uint32_t wt;
-        uint32_t ex = ps_wave_excl_scan(c, &wt);
-        if (lane == 0) wsum[wave] = wt;
         tstart[d] = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) wcnt[w][d] = 0;
-        __syncthreads();
-        for (int w = 0; w < wave; ++w) ex += wsum[w];
-        goff[d] = ex + own;
+        if (sbase) {
+            goff[d] = sbase[(size_t)vseg[blockIdx.x] * kRadix + d] + own;
+        } else {
+            const uint32_t c = dtot[d];
+            uint32_t wt;
+            uint32_t ex = ps_wave_excl_scan(c, &wt);
+            if (lane == 0) wsum[wave] = wt;
+            __syncthreads();
+            for (int w = 0; w < wave; ++w) ex += wsum[w];
+            goff[d] = ex + own;
+        }
     }
     __syncthreads();
 
@@ -581,12 +632,64 @@ hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, floa
         uint32_t* kout = by_aux ? b.aux : b.key;
         const uint32_t* ain = two ? (by_aux ? a.key : a.aux) : nullptr;
         uint32_t* aout = two ? (by_aux ? b.key : b.aux) : nullptr;
-        sort_upsweep_kernel<<<tiles, kBlock, 0, s>>>(kin, n, shift, table, tiles);
+        sort_upsweep_kernel<<<tiles, kBlock, 0, s>>>(kin, n, shift, table, tiles, nullptr);
         sort_rowscan_kernel<<<kRadix, kRowBlock, 0, s>>>(table, tiles, dtot);
         sort_downsweep_kernel<<<tiles, kDBlock, 0, s>>>(
             kin, a.id, a.x, a.y, a.z, a.m, kout, b.id, b.x, b.y, b.z, b.m,
             pass == passes - 1 ? pos4 : nullptr, ain, aout, n, shift, table, dtot, tiles,
-            pass == passes - 1 ? kcopy : nullptr);
+            pass == passes - 1 ? kcopy : nullptr, nullptr, nullptr, nullptr);
+    }
+    return hipGetLastError();
+}
+
+size_t ps_sort_seg_tiles(const uint32_t* off, uint32_t n_sys, uint32_t* vstart, uint32_t* vseg,
+                         uint32_t* seg_tile) {
+    size_t t = 0;
+    for (uint32_t sg = 0; sg < n_sys; ++sg) {
+        if (seg_tile) seg_tile[sg] = (uint32_t)t;
+        for (uint32_t b = off[sg]; b < off[sg + 1]; b += (uint32_t)kTile, ++t) {
+            if (vstart) vstart[t] = b;
+            if (vseg) vseg[t] = sg;
+        }
+    }
+    if (seg_tile) seg_tile[n_sys] = (uint32_t)t;
+    if (vstart) vstart[t] = n_sys ? off[n_sys] : 0u;
+    return t;
+}
+
+size_t ps_sort_seg_tmp_bytes(size_t n_tiles, uint32_t n_sys) {
+    const size_t table = (size_t)kRadix * (n_tiles ? n_tiles : 1);
+    return align256(table * sizeof(uint32_t)) + align256(kRadix * sizeof(uint32_t)) +
+           align256((size_t)kRadix * (n_sys ? n_sys : 1) * sizeof(uint32_t));
+}
+
+hipError_t ps_radix_sort_seg(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
+                             size_t n, PsSortSegs segs, void* tmp, hipStream_t s) {
+    if (n == 0 || segs.n_tiles == 0) return hipSuccess;
+    const uint32_t tiles = segs.n_tiles;
+    const size_t table_n = (size_t)kRadix * tiles;
+    char* p = reinterpret_cast<char*>(tmp);
+    uint32_t* table = reinterpret_cast<uint32_t*>(p);
+    p += align256(table_n * sizeof(uint32_t));
+    uint32_t* dtot = reinterpret_cast<uint32_t*>(p);
+    p += align256(kRadix * sizeof(uint32_t));
+    uint32_t* sbase = reinterpret_cast<uint32_t*>(p);
+    // three passes over the code, io -> t0 -> t1 -> io; every key stays in
+    // its segment (the segments' other arrays, aux included, need no move)
+    const PsSortArrays* src[3] = {&io, &t0, &t1};
+    const PsSortArrays* dst[3] = {&t0, &t1, &io};
+    for (int pass = 0; pass < 3; ++pass) {
+        const int shift = pass * kRadixBits;
+        const PsSortArrays& a = *src[pass];
+        const PsSortArrays& b = *dst[pass];
+        sort_upsweep_kernel<<<tiles, kBlock, 0, s>>>(a.key, n, shift, table, tiles, segs.vstart);
+        sort_rowscan_kernel<<<kRadix, kRowBlock, 0, s>>>(table, tiles, dtot);
+        sort_segbase_kernel<<<segs.n_sys, kRadix, 0, s>>>(table, tiles, dtot, segs.seg_tile,
+                                                          segs.off, sbase);
+        sort_downsweep_kernel<<<tiles, kDBlock, 0, s>>>(
+            a.key, a.id, a.x, a.y, a.z, a.m, b.key, b.id, b.x, b.y, b.z, b.m,
+            pass == 2 ? pos4 : nullptr, nullptr, nullptr, n, shift, table, dtot, tiles, nullptr,
+            segs.vstart, segs.vseg, sbase);
     }
     return hipGetLastError();
 }

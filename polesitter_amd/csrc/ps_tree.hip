@@ -193,8 +193,8 @@ __global__ __launch_bounds__(kBlock) void tree_write_kernel(
 
 // Child masks without atomics: a parent's children are consecutive nodes
 // (one level, Morton order), so the first child of each run ORs the octants
-// of its <= 8 siblings and stores the parent's mask (link.y, zero until
-// here for internal nodes).  m = min(*count, cap) nodes, or n_nodes.
+// of its <= 8 siblings and stores the parent's mask (link.y of an internal
+// node, not written before).  m = min(*count, cap) nodes, or n_nodes.
 __global__ __launch_bounds__(kBlock) void tree_mask_kernel(const uint32_t* __restrict__ parent,
                                                            const uint8_t* __restrict__ octant,
                                                            uint2* __restrict__ link,
@@ -460,10 +460,8 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
                          hipStream_t s, const uint8_t* sys_cap) {
     const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
     const uint32_t* off = reinterpret_cast<const uint32_t*>(tmp);
-    if (n_nodes_dev)
-        ps_fill_u32_count(reinterpret_cast<uint32_t*>(link), 0u, n_nodes_dev, 2, n_nodes, s);
-    else
-        ps_fill_u32(reinterpret_cast<uint32_t*>(link), 0u, 2 * n_nodes, s);
+    // no clearing of link: the write stores a leaf's whole word and an
+    // internal node's first child, the mask pass its child mask
     const uint32_t cap = n_nodes < 0xFFFFFFFFull ? (uint32_t)n_nodes : 0xFFFFFFFFu;
     tree_write_kernel<<<nb, kBlock, 0, s>>>(codes, sys, n, off, nb, root, roots, geom, link,
                                             nend, parent, leaf_of, octant, leaf_cap, cap,

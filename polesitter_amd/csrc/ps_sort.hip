@@ -503,6 +503,71 @@ __global__ __launch_bounds__(1024) void delta_small_sort_kernel(const uint32_t* 
     for (uint32_t i = threadIdx.x; i < nc; i += 1024) cpk[i] = a[i];
 }
 
+// A whole sort of n <= kSmallSort particles in one block: (key bits 0..29,
+// position) packed into distinct 64-bit values, one bitonic sort in LDS,
+// then every particle moved once, in place -- all reads before a barrier,
+// all writes after it.  The order by (key, position) is the stable order,
+// the permutation of the LSD passes.
+__global__ __launch_bounds__(1024) void small_sort_kernel(PsSortArrays io, float4* __restrict__ pos4,
+                                                          uint32_t* __restrict__ kcopy,
+                                                          uint32_t n) {
+    __shared__ unsigned long long a[kSmallSort];
+    uint32_t P = 2;
+    while (P < n) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += 1024)
+        a[i] = i < n ? packed_key(io.key[i], i) : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += 1024) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long u = a[i], v = a[ixj];
+                    const bool asc = (i & k) == 0;
+                    if ((u > v) == asc) {
+                        a[i] = v;
+                        a[ixj] = u;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    constexpr uint32_t R = kSmallSort / 1024;
+    uint32_t kk[R], id[R];
+    float x[R], y[R], z[R], m[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t p = r * 1024 + threadIdx.x;
+        if (p >= n) continue;
+        const uint32_t src = (uint32_t)a[p];
+        kk[r] = io.key[src];
+        id[r] = io.id[src];
+        if (io.x) {
+            x[r] = io.x[src];
+            y[r] = io.y[src];
+            z[r] = io.z[src];
+            m[r] = io.m[src];
+        }
+    }
+    __syncthreads();  // every particle is read before any is written
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t p = r * 1024 + threadIdx.x;
+        if (p >= n) continue;
+        io.key[p] = kk[r];
+        io.id[p] = id[r];
+        if (kcopy) kcopy[p] = kk[r];
+        if (io.x) {
+            io.x[p] = x[r];
+            io.y[p] = y[r];
+            io.z[p] = z[r];
+            io.m[p] = m[r];
+            if (pos4) pos4[p] = make_float4(x[r], y[r], z[r], m[r]);
+        }
+    }
+}
+
 __device__ __forceinline__ void move_particle(const PsSortArrays& in, const PsSortArrays& out,
                                               float4* pos4, uint32_t* kcopy, size_t src,
                                               size_t dst) {
@@ -612,6 +677,10 @@ size_t ps_sort_tmp_bytes(size_t n) {
 hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
                          size_t n, void* tmp, hipStream_t s, uint32_t* kcopy) {
     if (n == 0) return hipSuccess;
+    if (n <= kSmallSort && !(io.aux && t0.aux && t1.aux)) {  // one block, one launch
+        small_sort_kernel<<<1, 1024, 0, s>>>(io, pos4, kcopy, (uint32_t)n);
+        return hipGetLastError();
+    }
     const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
     const size_t table_n = (size_t)kRadix * tiles;
     uint32_t* table = reinterpret_cast<uint32_t*>(tmp);

@@ -108,14 +108,17 @@ __global__ __launch_bounds__(kBlock) void scan_reduce_dev_kernel(
     }
 }
 
-// rescan each tile from its offset
+// rescan each tile from its offset; total (a single tile only): its exact
+// 64-bit sums, stored
 template <class Ops>
 __global__ __launch_bounds__(kBlock) void scan_down_kernel(const typename Ops::T* in,
                                                            typename Ops::T* out, size_t n,
-                                                           const typename Ops::T* boff) {
+                                                           const typename Ops::T* boff,
+                                                           unsigned long long* total = nullptr) {
     using T = typename Ops::T;
     __shared__ T tile[kTile];
     __shared__ T wsum[kWaves];
+    __shared__ unsigned long long tsum[Ops::C][kWaves];
     const size_t base = (size_t)blockIdx.x * kTile;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
@@ -133,6 +136,17 @@ __global__ __launch_bounds__(kBlock) void scan_down_kernel(const typename Ops::T
     // inclusive wave scan of the per-thread sums
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    if (total) {  // block-uniform
+#pragma unroll
+        for (int c = 0; c < Ops::C; ++c) {
+            unsigned long long v = 0;
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) v += Ops::comp(local[k], c);
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+            if (lane == 0) tsum[c][wave] = v;
+        }
+    }
     T inc = s;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -151,6 +165,11 @@ __global__ __launch_bounds__(kBlock) void scan_down_kernel(const typename Ops::T
         run = Ops::add(run, local[k]);
     }
     __syncthreads();
+    if (total && threadIdx.x < (unsigned)Ops::C) {
+        unsigned long long v = 0;
+        for (int w = 0; w < kWaves; ++w) v += tsum[threadIdx.x][w];
+        total[threadIdx.x] = v;
+    }
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         size_t i = base + (size_t)k * kBlock + threadIdx.x;
@@ -290,17 +309,16 @@ template <class Ops>
 hipError_t scan_impl(const typename Ops::T* in, typename Ops::T* out, size_t n,
                      unsigned long long* total, void* tmp, hipStream_t s) {
     using T = typename Ops::T;
-    if (total) fill_u64_kernel<<<1, 64, 0, s>>>(total, Ops::C);
-    if (n == 0) return hipGetLastError();
     size_t nb = (n + kTile - 1) / kTile;
-    if (nb == 1) {
-        if (total) {
-            T* bs = reinterpret_cast<T*>(tmp);
-            scan_reduce_kernel<Ops><<<1, kBlock, 0, s>>>(in, n, bs, total);
+    if (nb <= 1) {  // one launch: the tile's scan and its totals
+        if (n == 0) {
+            if (total) fill_u64_kernel<<<1, 64, 0, s>>>(total, Ops::C);
+            return hipGetLastError();
         }
-        scan_down_kernel<Ops><<<1, kBlock, 0, s>>>(in, out, n, nullptr);
+        scan_down_kernel<Ops><<<1, kBlock, 0, s>>>(in, out, n, nullptr, total);
         return hipGetLastError();
     }
+    if (total) fill_u64_kernel<<<1, 64, 0, s>>>(total, Ops::C);
     T* bsum = reinterpret_cast<T*>(tmp);
     T* boff = reinterpret_cast<T*>(reinterpret_cast<char*>(tmp) + align256(nb * sizeof(T)));
     void* next = reinterpret_cast<char*>(tmp) + 2 * align256(nb * sizeof(T));

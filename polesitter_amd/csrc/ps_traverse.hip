@@ -589,36 +589,49 @@ __global__ __launch_bounds__(kStep) void trav_step_kernel(
         // sources (the first bin: longer still).  Which group takes an
         // entry, and in which order a bin's entries come, changes nothing in
         // what an entry computes.
-        uint32_t rk[kSeg / kStep];  // bin << 16 | rank in the bin
-#pragma unroll
-        for (uint32_t r = 0; r < kSeg / kStep; ++r) {
-            const uint32_t j = r * kStep + tid;
-            rk[r] = PS_NONE;
-            if (j < segn) {
-                // the entry itself goes to LDS for the count phase (coalesced
-                // here, where the count phase would gather it per entry)
-                const uint4 me = make_uint4(ent.tgt[seg + j], ent.off[seg + j], ent.len[seg + j],
-                                            ent.moff[seg + j]);
-                scnt[j] = me;
-                const uint32_t rows = (me.z + (uint32_t)G - 1u) / (uint32_t)G;
-                const uint32_t b = kOrdBins - 1u - (rows < kOrdBins - 1u ? rows : kOrdBins - 1u);
-                rk[r] = (b << 16) | atomicAdd(&shist[b], 1u);
+        if (segn <= kGroups) {
+            // at most one entry per group: they all run at once, so the
+            // entries keep their order (no histogram, two barriers fewer)
+            if (tid < segn) {
+                scnt[tid] = make_uint4(ent.tgt[seg + tid], ent.off[seg + tid],
+                                       ent.len[seg + tid], ent.moff[seg + tid]);
+                sord[tid] = (uint16_t)tid;
             }
-        }
-        __syncthreads();
-        if (wave == 0) {  // exclusive prefix over the bins
-            const uint32_t v = lane < kOrdBins ? shist[lane] : 0u;
-            uint32_t tot;
-            const uint32_t ex = ps_wave_excl_scan(v, &tot);
-            if (lane < kOrdBins) shist[lane] = ex;
-        }
-        __syncthreads();
+            __syncthreads();
+        } else {
+            uint32_t rk[kSeg / kStep];  // bin << 16 | rank in the bin
 #pragma unroll
-        for (uint32_t r = 0; r < kSeg / kStep; ++r)
-            if (rk[r] != PS_NONE)
-                sord[shist[rk[r] >> 16] + (rk[r] & 0xFFFFu)] = (uint16_t)(r * kStep + tid);
-        __syncthreads();
-        if (tid < kOrdBins) shist[tid] = 0u;  // for the next segment (barriers before its use)
+            for (uint32_t r = 0; r < kSeg / kStep; ++r) {
+                const uint32_t j = r * kStep + tid;
+                rk[r] = PS_NONE;
+                if (j < segn) {
+                    // the entry itself goes to LDS for the count phase
+                    // (coalesced here, where the count phase would gather it
+                    // per entry)
+                    const uint4 me = make_uint4(ent.tgt[seg + j], ent.off[seg + j],
+                                                ent.len[seg + j], ent.moff[seg + j]);
+                    scnt[j] = me;
+                    const uint32_t rows = (me.z + (uint32_t)G - 1u) / (uint32_t)G;
+                    const uint32_t b =
+                        kOrdBins - 1u - (rows < kOrdBins - 1u ? rows : kOrdBins - 1u);
+                    rk[r] = (b << 16) | atomicAdd(&shist[b], 1u);
+                }
+            }
+            __syncthreads();
+            if (wave == 0) {  // exclusive prefix over the bins
+                const uint32_t v = lane < kOrdBins ? shist[lane] : 0u;
+                uint32_t tot;
+                const uint32_t ex = ps_wave_excl_scan(v, &tot);
+                if (lane < kOrdBins) shist[lane] = ex;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t r = 0; r < kSeg / kStep; ++r)
+                if (rk[r] != PS_NONE)
+                    sord[shist[rk[r] >> 16] + (rk[r] & 0xFFFFu)] = (uint16_t)(r * kStep + tid);
+            __syncthreads();
+            if (tid < kOrdBins) shist[tid] = 0u;  // for the next segment (barriers before its use)
+        }
         // count
         constexpr uint32_t kPoolPer = kPoolWords / kGroups;
         // the group's pool space: kPoolPer words from grp * kPoolPer (rec_cap

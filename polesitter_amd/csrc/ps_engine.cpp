@@ -574,10 +574,20 @@ int stage_build(ps_hip_engine* e) {
     return PS_HIP_OK;
 }
 
+// Levels of at most this many nodes, from the root down to the first larger
+// one, go to one launch of one block (a launch per level costs more there
+// than the level's work).
+constexpr uint32_t kUpwardTopNodes = 2048;
+
 int stage_upward(ps_hip_engine* e) {
-    for (int d = (int)e->max_depth; d >= 0; --d)
+    int d_top = -1;
+    while (d_top < (int)e->max_depth && e->ls[d_top + 2] - e->ls[d_top + 1] <= kUpwardTopNodes)
+        ++d_top;
+    for (int d = (int)e->max_depth; d > d_top; --d)
         ps_launch_upward_level(e->ls[d], e->ls[d + 1], d, e->tpk(), e->pos4(), e->geom(),
                                e->link(), e->nend(), e->mp(), e->s);
+    ps_launch_upward_top(e->ls, d_top, e->tpk(), e->pos4(), e->geom(), e->link(), e->nend(),
+                         e->mp(), e->s);
     return map_err(hipGetLastError());
 }
 

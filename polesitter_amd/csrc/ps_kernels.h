@@ -154,6 +154,11 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
 void ps_launch_upward_dev(const uint32_t* ls, float4* tpk, const float4* pos4, const float4* geom,
                           const uint2* link, const uint32_t* nend, float4* mp, hipStream_t s);
 void ps_launch_l2l_dev(const uint32_t* ls, const uint32_t* parent, float4* loc, hipStream_t s);
+// Depths d_top .. 0 in one launch of one block (small levels; ls: the level
+// starts on the host, PS_MAX_DEPTH + 2 entries).
+void ps_launch_upward_top(const uint32_t* ls, int d_top, float4* tpk, const float4* pos4,
+                          const float4* geom, const uint2* link, const uint32_t* nend, float4* mp,
+                          hipStream_t s);
 void ps_launch_upward_level(uint32_t begin, uint32_t end, int d, float4* tpk, const float4* pos4,
                             const float4* geom, const uint2* link, const uint32_t* nend,
                             float4* mp, hipStream_t s);

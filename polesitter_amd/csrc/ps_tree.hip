@@ -289,16 +289,14 @@ __device__ __forceinline__ float4 trav_node(float4 g, uint2 l, int d) {
     return make_float4(g.x, g.y, g.z, __uint_as_float(w));
 }
 
-__global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, uint32_t end, int d,
-                                                              float4* __restrict__ tpk,
-                                                              const float4* __restrict__ pos4,
-                                                              const float4* __restrict__ geom,
-                                                              const uint2* __restrict__ link,
-                                                              const uint32_t* __restrict__ nend,
-                                                              float4* __restrict__ mp) {
+// P2M or M2M of node nidx at depth d, and its traversal record
+__device__ __forceinline__ void upward_node(uint32_t nidx, int d, float4* __restrict__ tpk,
+                                            const float4* __restrict__ pos4,
+                                            const float4* __restrict__ geom,
+                                            const uint2* __restrict__ link,
+                                            const uint32_t* __restrict__ nend,
+                                            float4* __restrict__ mp) {
 #pragma clang fp contract(off)
-    const uint32_t nidx = begin + blockIdx.x * kBlock + threadIdx.x;
-    if (nidx >= end) return;
     const float4 g = geom[nidx];
     const uint2 l = link[nidx];
     tpk[nidx] = trav_node(g, l, d);
@@ -309,6 +307,37 @@ __global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, ui
         m2m_children(geom, mp, l, g, m0, mx, my, mz);
     }
     mp[nidx] = make_float4(m0, mx, my, mz);
+}
+
+__global__ __launch_bounds__(kBlock) void upward_level_kernel(uint32_t begin, uint32_t end, int d,
+                                                              float4* __restrict__ tpk,
+                                                              const float4* __restrict__ pos4,
+                                                              const float4* __restrict__ geom,
+                                                              const uint2* __restrict__ link,
+                                                              const uint32_t* __restrict__ nend,
+                                                              float4* __restrict__ mp) {
+    const uint32_t nidx = begin + blockIdx.x * kBlock + threadIdx.x;
+    if (nidx < end) upward_node(nidx, d, tpk, pos4, geom, link, nend, mp);
+}
+
+// Depths d_top .. 0 in one block, level by level (a barrier between levels:
+// a node's children, one level down, are written before it reads them).
+constexpr int kTopBlock = 1024;
+struct PsLevels {
+    uint32_t start[PS_MAX_DEPTH + 2];
+};
+__global__ __launch_bounds__(kTopBlock) void upward_top_kernel(PsLevels ls, int d_top,
+                                                               float4* __restrict__ tpk,
+                                                               const float4* __restrict__ pos4,
+                                                               const float4* __restrict__ geom,
+                                                               const uint2* __restrict__ link,
+                                                               const uint32_t* __restrict__ nend,
+                                                               float4* __restrict__ mp) {
+    for (int d = d_top; d >= 0; --d) {
+        for (uint32_t nidx = ls.start[d] + threadIdx.x; nidx < ls.start[d + 1]; nidx += kTopBlock)
+            upward_node(nidx, d, tpk, pos4, geom, link, nend, mp);
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void l2l_level_kernel(uint32_t begin, uint32_t end,
@@ -473,6 +502,15 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
                                                          (size_t)cap);
     }
     return hipGetLastError();
+}
+
+void ps_launch_upward_top(const uint32_t* ls, int d_top, float4* tpk, const float4* pos4,
+                          const float4* geom, const uint2* link, const uint32_t* nend, float4* mp,
+                          hipStream_t s) {
+    if (d_top < 0) return;
+    PsLevels lv;
+    for (int d = 0; d <= PS_MAX_DEPTH + 1; ++d) lv.start[d] = ls[d];
+    upward_top_kernel<<<1, kTopBlock, 0, s>>>(lv, d_top, tpk, pos4, geom, link, nend, mp);
 }
 
 void ps_launch_upward_level(uint32_t begin, uint32_t end, int d, float4* tpk, const float4* pos4,

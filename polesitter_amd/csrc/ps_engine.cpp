@@ -552,11 +552,11 @@ int stage_build(ps_hip_engine* e) {
                         e->leaf_cap, e->s));
     PS_CK(hipMemcpyAsync(e->h_small, e->small_b.p, 96, hipMemcpyDeviceToHost, e->s));
     PS_CK(hipEventRecord(e->ev_counts, e->s));
+    // the mask pass also clears the locals and the P2P list lengths
     PS_CK(ps_tree_write(keys, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
                         e->link(), e->nend(), e->parent(), e->leaf_of(), e->oct(),
-                        e->node_cap, e->totals(), e->leaf_cap, e->s));
-    ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
-                       e->totals(), e->node_cap, e->s);
+                        e->node_cap, e->totals(), e->leaf_cap, e->s, nullptr, e->loc(),
+                        e->p2p_len()));
     e->l2l_pending = false;
     e->collected = false;  // a new tree: its big-leaf list is still to be built
     PS_CK(wait_event(e->ev_counts));
@@ -881,9 +881,7 @@ int async_front(ps_hip_engine* e, hipStream_t cs) {
                         e->leaf_cap, cs));
     PS_CK(ps_tree_write(e->code, nullptr, e->n, e->root, nullptr, e->tree_tmp.p, e->geom(),
                         e->link(), e->nend(), e->parent(), e->leaf_of(), e->oct(), M,
-                        e->totals(), e->leaf_cap, cs));
-    ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
-                       e->totals(), M, cs);
+                        e->totals(), e->leaf_cap, cs, nullptr, e->loc(), e->p2p_len()));
     ps_launch_upward_dev(ls, e->tpk(), e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), cs);
     return map_err(hipGetLastError());
 }
@@ -1075,9 +1073,7 @@ int batch_run_front(ps_hip_batch* b, uint32_t n_sys, int morton_mode) {
                         s, caps));
     PS_CK(ps_tree_write(e->code, sys, n, make_float4(0.f, 0.f, 0.f, 0.f), roots, e->tree_tmp.p,
                         e->geom(), e->link(), e->nend(), e->parent(), e->leaf_of(), e->oct(),
-                        cap, e->totals(), e->leaf_cap, s, caps));
-    ps_fill2_u32_count(reinterpret_cast<uint32_t*>(e->loc()), 4, e->p2p_len(), 1, 0u,
-                       e->totals(), cap, s);
+                        cap, e->totals(), e->leaf_cap, s, caps, e->loc(), e->p2p_len()));
     ps_launch_upward_dev(ls, e->tpk(), e->pos4(), e->geom(), e->link(), e->nend(), e->mp(), s);
     return map_err(hipGetLastError());
 }

@@ -147,7 +147,8 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
                          const float4* roots, void* tmp, float4* geom, uint2* link,
                          uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, uint8_t* octant,
                          size_t n_nodes, const unsigned long long* n_nodes_dev, int leaf_cap,
-                         hipStream_t s, const uint8_t* sys_cap = nullptr);
+                         hipStream_t s, const uint8_t* sys_cap = nullptr, float4* loc_clear = nullptr,
+                         uint32_t* len_clear = nullptr);
 // Upward (depths 10..0) and L2L (depths 1..10) with the level starts read
 // from device memory ls[0..11]; fixed grids, no host-side sizes.  The upward
 // pass also writes each node's traversal record tpk (ps_tnode_word).

@@ -199,11 +199,16 @@ __global__ __launch_bounds__(kBlock) void tree_mask_kernel(const uint32_t* __res
                                                            const uint8_t* __restrict__ octant,
                                                            uint2* __restrict__ link,
                                                            const unsigned long long* count,
-                                                           size_t n_nodes) {
+                                                           size_t n_nodes,
+                                                           float4* __restrict__ loc_clear,
+                                                           uint32_t* __restrict__ len_clear) {
     const size_t m = count ? (size_t)(*count < n_nodes ? *count : n_nodes) : n_nodes;
     uint32_t* link32 = reinterpret_cast<uint32_t*>(link);
     for (size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x; c < m;
          c += (size_t)gridDim.x * kBlock) {
+        // what the traversal accumulates starts at zero (optional)
+        if (loc_clear) loc_clear[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (len_clear) len_clear[c] = 0u;
         const uint32_t p = parent[c];
         if (p == PS_NONE || (c > 0 && parent[c - 1] == p)) continue;
         // the next seven nodes' parents and octants are loaded at once; the
@@ -486,7 +491,8 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
                          const float4* roots, void* tmp, float4* geom, uint2* link,
                          uint32_t* nend, uint32_t* parent, uint32_t* leaf_of, uint8_t* octant,
                          size_t n_nodes, const unsigned long long* n_nodes_dev, int leaf_cap,
-                         hipStream_t s, const uint8_t* sys_cap) {
+                         hipStream_t s, const uint8_t* sys_cap, float4* loc_clear,
+                         uint32_t* len_clear) {
     const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
     const uint32_t* off = reinterpret_cast<const uint32_t*>(tmp);
     // no clearing of link: the write stores a leaf's whole word and an
@@ -499,7 +505,7 @@ hipError_t ps_tree_write(const uint32_t* codes, const uint32_t* sys, size_t n, f
         size_t mb = (n_nodes + kBlock - 1) / kBlock;
         if (mb > 4096) mb = 4096;
         tree_mask_kernel<<<(unsigned)mb, kBlock, 0, s>>>(parent, octant, link, n_nodes_dev,
-                                                         (size_t)cap);
+                                                         (size_t)cap, loc_clear, len_clear);
     }
     return hipGetLastError();
 }

@@ -1676,8 +1676,10 @@ int ps_hip_prepare(const float* x, const float* y, const float* z, size_t n, uin
         dcode = reinterpret_cast<uint32_t*>(w->code.p);
     }
     float* prm = reinterpret_cast<float*>(w->prm.p);
-    ps_launch_bbox(d[0], d[1], d[2], n, reinterpret_cast<float2*>(w->part.p), prm, w->s);
-    ps_launch_morton(d[0], d[1], d[2], n, prm, dcode, morton_mode, w->s);
+    if (!ps_launch_prepare_small(d[0], d[1], d[2], n, prm, dcode, morton_mode, w->s)) {
+        ps_launch_bbox(d[0], d[1], d[2], n, reinterpret_cast<float2*>(w->part.p), prm, w->s);
+        ps_launch_morton(d[0], d[1], d[2], n, prm, dcode, morton_mode, w->s);
+    }
     PS_CK(hipMemcpyAsync(w->h_out, prm, 16, hipMemcpyDeviceToHost, w->s));
     if (host_codes) PS_CK(hipMemcpyAsync(codes, dcode, n * 4, hipMemcpyDeviceToHost, w->s));
     PS_CK(wait_stream(w->s));

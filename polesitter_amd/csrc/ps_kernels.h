@@ -39,6 +39,10 @@ void ps_reduce_u64(const unsigned long long* in, size_t n, unsigned long long* o
 
 // ---- ps_prepare.hip: bounding cube + Morton codes, H:1911-2170 --------------
 // prm (device, 4 floats): min, max, range, 1023/range.  part: >= 1024 float2.
+// The whole prepare pass in one block for a small n (prm and codes as
+// ps_launch_bbox + ps_launch_morton); false (nothing launched) above it.
+bool ps_launch_prepare_small(const float* x, const float* y, const float* z, size_t n,
+                             float* prm, uint32_t* codes, int mode, hipStream_t s);
 void ps_launch_bbox(const float* x, const float* y, const float* z, size_t n, float2* part,
                     float* prm, hipStream_t s);
 // mode 0: scalar truncation for every particle (H:2132-2157);

@@ -51,6 +51,17 @@ __global__ __launch_bounds__(kBlock) void bbox_partial_kernel(const float* __res
     }
 }
 
+// the cube's parameters from its min and max: (min, max, range, 1023/range)
+__device__ __forceinline__ void cube_params(float lo, float hi, float* prm) {
+#pragma clang fp contract(off)
+    float range = hi - lo;  // H:2008-2011
+    if (range < 0.001f) range = 0.001f;
+    prm[0] = lo;
+    prm[1] = hi;
+    prm[2] = range;
+    prm[3] = 1023.0f / range;  // H:2017
+}
+
 __global__ __launch_bounds__(kBlock) void bbox_final_kernel(const float2* part, int np,
                                                             float* prm) {
     __shared__ float slo[kBlock / 64], shi[kBlock / 64];
@@ -66,17 +77,11 @@ __global__ __launch_bounds__(kBlock) void bbox_final_kernel(const float2* part, 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-#pragma clang fp contract(off)
         for (int w = 1; w < kBlock / 64; ++w) {
             lo = fminf(lo, slo[w]);
             hi = fmaxf(hi, shi[w]);
         }
-        float range = hi - lo;  // H:2008-2011
-        if (range < 0.001f) range = 0.001f;
-        prm[0] = lo;
-        prm[1] = hi;
-        prm[2] = range;
-        prm[3] = 1023.0f / range;  // H:2017
+        cube_params(lo, hi, prm);
     }
 }
 
@@ -99,20 +104,11 @@ __device__ __forceinline__ uint32_t q_rne(float v, float lo, float scale) {
     return (uint32_t)(int)rintf(t);
 }
 
-__global__ __launch_bounds__(kBlock) void morton_kernel(const float* __restrict__ x,
-                                                        const float* __restrict__ y,
-                                                        const float* __restrict__ z, size_t n,
-                                                        const float* __restrict__ prm,
-                                                        uint32_t* __restrict__ codes, int mode,
-                                                        size_t n_vec) {
-    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float lo = prm[0];
-    const float range = prm[2];
-    const float scale = prm[3];
-    float vx = x[i], vy = y[i], vz = z[i];
+// particle i's code in the cube (lo, range, scale = 1023/range)
+__device__ __forceinline__ uint32_t particle_code(float vx, float vy, float vz, float lo,
+                                                 float range, float scale, bool rne) {
     uint32_t qx, qy, qz;
-    if (mode == 1 && i < n_vec) {
+    if (rne) {
         qx = q_rne(vx, lo, scale);
         qy = q_rne(vy, lo, scale);
         qz = q_rne(vz, lo, scale);
@@ -121,7 +117,53 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const float* __restrict_
         qy = q_trunc(vy, lo, range);
         qz = q_trunc(vz, lo, range);
     }
-    codes[i] = ps_morton3(qx, qy, qz);
+    return ps_morton3(qx, qy, qz);
+}
+
+__global__ __launch_bounds__(kBlock) void morton_kernel(const float* __restrict__ x,
+                                                        const float* __restrict__ y,
+                                                        const float* __restrict__ z, size_t n,
+                                                        const float* __restrict__ prm,
+                                                        uint32_t* __restrict__ codes, int mode,
+                                                        size_t n_vec) {
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    codes[i] = particle_code(x[i], y[i], z[i], prm[0], prm[2], prm[3], mode == 1 && i < n_vec);
+}
+
+// The whole prepare pass of n <= kSmallPrep particles in one block: the
+// min/max reduction, the cube's parameters (to prm), and the codes.
+constexpr int kSmallBlock = 1024;
+constexpr size_t kSmallPrep = 4096;
+__global__ __launch_bounds__(kSmallBlock) void prepare_small_kernel(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    uint32_t n, float* __restrict__ prm, uint32_t* __restrict__ codes, int mode) {
+    __shared__ float slo[kSmallBlock / 64], shi[kSmallBlock / 64];
+    __shared__ float sp[4];
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    for (uint32_t i = threadIdx.x; i < n; i += kSmallBlock) {
+        const float a = x[i], b = y[i], c = z[i];
+        lo = fminf(lo, fminf(a, fminf(b, c)));
+        hi = fmaxf(hi, fmaxf(a, fmaxf(b, c)));
+    }
+    wave_minmax(lo, hi);
+    if ((threadIdx.x & 63) == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kSmallBlock / 64; ++w) {
+            lo = fminf(lo, slo[w]);
+            hi = fmaxf(hi, shi[w]);
+        }
+        cube_params(lo, hi, sp);
+        for (int k = 0; k < 4; ++k) prm[k] = sp[k];
+    }
+    __syncthreads();
+    const uint32_t n_vec = n - (n % 8u);
+    for (uint32_t i = threadIdx.x; i < n; i += kSmallBlock)
+        codes[i] = particle_code(x[i], y[i], z[i], sp[0], sp[2], sp[3], mode == 1 && i < n_vec);
 }
 
 // ---- many systems in one set of arrays: system s is [off[s], off[s+1]) ----
@@ -212,6 +254,13 @@ void ps_launch_prepare_seg(const float* x, const float* y, const float* z, size_
                                                                               mode);
 }
 
+bool ps_launch_prepare_small(const float* x, const float* y, const float* z, size_t n,
+                             float* prm, uint32_t* codes, int mode, hipStream_t s) {
+    if (n == 0 || n > kSmallPrep) return false;
+    prepare_small_kernel<<<1, kSmallBlock, 0, s>>>(x, y, z, (uint32_t)n, prm, codes, mode);
+    return true;
+}
+
 void ps_launch_bbox(const float* x, const float* y, const float* z, size_t n, float2* part,
                     float* prm, hipStream_t s) {
     size_t blocks = (n + kBlock - 1) / kBlock;
@@ -273,8 +322,10 @@ hipError_t ps_prepare_noalloc(const float* x, const float* y, const float* z, si
                                           kMaxParts * sizeof(float2));
     // every partial the final pass reads is written by this call's first
     // pass, and the parameters by its final pass: nothing needs clearing
-    ps_launch_bbox(x, y, z, n, part, prm, s);
-    ps_launch_morton(x, y, z, n, prm, codes, mode, s);
+    if (!ps_launch_prepare_small(x, y, z, n, prm, codes, mode, s)) {
+        ps_launch_bbox(x, y, z, n, part, prm, s);
+        ps_launch_morton(x, y, z, n, prm, codes, mode, s);
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(out3, prm, 3 * sizeof(float), hipMemcpyDeviceToHost, s);

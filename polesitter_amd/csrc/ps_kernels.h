@@ -83,6 +83,8 @@ size_t ps_sort_tmp_bytes(size_t n);
 // stably by aux: the order is (aux, key).
 // kcopy (optional): a second copy of the sorted keys.  Payload arrays x
 // (and with it y, z, m) may be null in all three sets: keys and id only.
+// Without aux, n <= 4096 is one block (the same permutation; t0, t1 and tmp
+// untouched).
 hipError_t ps_radix_sort(PsSortArrays io, PsSortArrays t0, PsSortArrays t1, float4* pos4,
                          size_t n, void* tmp, hipStream_t s, uint32_t* kcopy = nullptr);
 // Many systems at once, each a contiguous segment [off[s], off[s + 1]) of

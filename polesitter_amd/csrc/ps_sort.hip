@@ -586,24 +586,35 @@ __device__ __forceinline__ void move_particle(const PsSortArrays& in, const PsSo
 // the unchanged particles: block b takes positions [b kMBlock, (b+1) kMBlock);
 // the changed keys that can fall among them are one slice of the sorted
 // changed keys, [lo, hi), held in LDS when it fits
+// Where each merge block's changed keys start among the sorted ones: block b
+// gets the first (key, position) at or after its first position's, block nb
+// gets nc -- every block's binary search at once, ahead of the merge.
+__global__ __launch_bounds__(256) void delta_range_kernel(const uint32_t* __restrict__ prev,
+                                                          size_t n, uint32_t nb,
+                                                          const unsigned long long* __restrict__ cpk,
+                                                          uint32_t nc,
+                                                          uint32_t* __restrict__ range) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b > nb) return;
+    const size_t q = (size_t)b * kMBlock;
+    range[b] = q < n ? lower_bound_u64(cpk, nc, packed_key(prev[q], (uint32_t)q)) : nc;
+}
+
 __global__ __launch_bounds__(kMBlock) void delta_merge_kernel(
     PsSortArrays in, PsSortArrays out, float4* __restrict__ pos4, uint32_t* __restrict__ kcopy,
     const uint32_t* __restrict__ prev, size_t n, const unsigned long long* __restrict__ mask,
-    const uint32_t* __restrict__ boff, const unsigned long long* __restrict__ cpk, uint32_t nc) {
+    const uint32_t* __restrict__ boff, const unsigned long long* __restrict__ cpk, uint32_t nc,
+    const uint32_t* __restrict__ range) {
     __shared__ unsigned long long slice[kSlice];
-    __shared__ uint32_t range[2];
     __shared__ uint32_t wp[kWordsPerBlock];
     __shared__ unsigned long long wm[kWordsPerBlock];
     block_word_prefix(mask, (n + 63) / 64, wp, wm);
     const size_t b0 = (size_t)blockIdx.x * kMBlock;
     const size_t b1 = b0 + kMBlock < n ? b0 + kMBlock : n;
     const uint32_t tid = threadIdx.x;
-    if (tid < 2) {
-        const size_t q = tid == 0 ? b0 : b1 - 1;
-        range[tid] = lower_bound_u64(cpk, nc, packed_key(prev[q], (uint32_t)q));
-    }
-    __syncthreads();
-    const uint32_t lo = range[0], hi = range[1];
+    // the block's changed keys lie in [lo, hi): the probes of its positions
+    // are at or after its first position's and before the next block's
+    const uint32_t lo = range[blockIdx.x], hi = range[blockIdx.x + 1];
     const bool in_lds = hi - lo <= (uint32_t)kSlice;
     if (in_lds)
         for (uint32_t j = tid; j < hi - lo; j += kMBlock) slice[j] = cpk[lo + j];
@@ -768,6 +779,7 @@ struct DeltaTmp {
     unsigned long long* mask;  // one word per 64 positions, padded to whole blocks
     uint32_t* bcnt;            // per block of kMBlock positions
     uint32_t* boff;            // exclusive prefix of bcnt, nb + 1 entries
+    uint32_t* range;           // per block: its first changed key in the sorted ones, nb + 1
     void* rest;                // the changed keys' radix tables
 };
 size_t delta_blocks(size_t n) { return (n + kMBlock - 1) / kMBlock; }
@@ -781,6 +793,8 @@ DeltaTmp delta_tmp(void* tmp, size_t n) {
     p += align256(nb * 4 + 4);
     d.boff = reinterpret_cast<uint32_t*>(p);
     p += align256(nb * 4 + 8);
+    d.range = reinterpret_cast<uint32_t*>(p);
+    p += align256(nb * 4 + 8);
     d.rest = p;
     return d;
 }
@@ -788,7 +802,8 @@ DeltaTmp delta_tmp(void* tmp, size_t n) {
 
 size_t ps_sort_delta_tmp_bytes(size_t n) {
     const size_t nb = delta_blocks(n);
-    return align256(nb * kWordsPerBlock * 8 + 8) + align256(nb * 4 + 4) + align256(nb * 4 + 8);
+    return align256(nb * kWordsPerBlock * 8 + 8) + align256(nb * 4 + 4) +
+           2 * align256(nb * 4 + 8);
 }
 
 hipError_t ps_sort_delta_count(const uint32_t* key, const uint32_t* prev, size_t n, void* tmp,
@@ -824,8 +839,10 @@ hipError_t ps_sort_delta_finish(PsSortArrays io, PsSortArrays t0, PsSortArrays t
         delta_small_sort_kernel<<<1, 1024, 0, s>>>(ckey, cpos, nc, cpk);
     }
     // the merged order into t0, then back into the caller's arrays
-    delta_merge_kernel<<<(unsigned)delta_blocks(n), kMBlock, 0, s>>>(io, t0, pos4, kcopy, prev, n,
-                                                                   d.mask, d.boff, cpk, nc);
+    const uint32_t mb = (uint32_t)delta_blocks(n);
+    delta_range_kernel<<<(mb + 1 + 255) / 256, 256, 0, s>>>(prev, n, mb, cpk, nc, d.range);
+    delta_merge_kernel<<<mb, kMBlock, 0, s>>>(io, t0, pos4, kcopy, prev, n, d.mask, d.boff, cpk,
+                                              nc, d.range);
     if (nc)
         delta_scatter_kernel<<<(nc + 255) / 256, 256, 0, s>>>(io, t0, pos4, kcopy, prev, n,
                                                               d.mask, d.boff, cpk, nc);

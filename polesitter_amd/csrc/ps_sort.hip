@@ -341,19 +341,31 @@ __device__ __forceinline__ uint32_t lower_bound_u64(const unsigned long long* a,
     return lo;
 }
 
-// first index in prev[0, n) whose key is >= k (or > k when upper)
-__device__ __forceinline__ size_t bound_keys(const uint32_t* prev, size_t n, uint32_t k,
-                                             bool upper) {
-    size_t lo = 0, hi = n;
-    while (lo < hi) {
-        const size_t mid = lo + ((hi - lo) >> 1);
-        const uint32_t v = prev[mid] & kKeyMask;
-        if (v < k || (upper && v == k))
-            lo = mid + 1;
-        else
-            hi = mid;
+// both bounds of key k in prev[0, n): a = first index with key >= k, b = first
+// with key > k; the two binary searches step together so their loads overlap
+__device__ __forceinline__ void bound_keys(const uint32_t* prev, size_t n, uint32_t k, size_t& a,
+                                           size_t& b) {
+    size_t lo0 = 0, hi0 = n, lo1 = 0, hi1 = n;
+    while (lo0 < hi0 || lo1 < hi1) {
+        const size_t m0 = lo0 + ((hi0 - lo0) >> 1), m1 = lo1 + ((hi1 - lo1) >> 1);
+        const bool s0 = lo0 < hi0, s1 = lo1 < hi1;
+        const uint32_t v0 = s0 ? prev[m0] & kKeyMask : 0u;
+        const uint32_t v1 = s1 ? prev[m1] & kKeyMask : 0u;
+        if (s0) {
+            if (v0 < k)
+                lo0 = m0 + 1;
+            else
+                hi0 = m0;
+        }
+        if (s1) {
+            if (v1 <= k)
+                lo1 = m1 + 1;
+            else
+                hi1 = m1;
+        }
     }
-    return lo;
+    a = lo0;
+    b = lo1;
 }
 
 // Changed flags: one 64-bit word per 64 positions (mask[w]); the positions
@@ -643,7 +655,8 @@ __global__ __launch_bounds__(256) void delta_scatter_kernel(
     const uint32_t k = (uint32_t)(v >> 32), src = (uint32_t)v;
     // the unchanged entries before (k, src): prev's positions p with
     // (prev[p], p) < (k, src), minus the changed positions among them
-    const size_t a = bound_keys(prev, n, k, false), b = bound_keys(prev, n, k, true);
+    size_t a, b;
+    bound_keys(prev, n, k, a, b);
     const size_t p = src < a ? a : (src > b ? b : src);
     const size_t dst = (size_t)r + (p - changed_before(mask, boff, p));
     move_particle(in, out, pos4, kcopy, src, dst);
